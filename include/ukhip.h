@@ -1,0 +1,76 @@
+/*
+ * ukhip.h -- libukhip extensions beyond the reference's kalman.h.
+ *
+ * None of these has a counterpart in the reference (c/kalman.h); they serve
+ * callers that keep problems resident on the GPU: the benchmark, batched
+ * smoothing of many independent problems, and the serving loop.
+ */
+#ifndef UKHIP_H
+#define UKHIP_H
+
+#include <stdint.h>
+
+#include "kalman.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" if none). The reference
+ * aborts instead (assert in c/matrix_ops.c and c/kalman_*.c). */
+const char* ukhip_last_error(void);
+void ukhip_clear_error(void);
+
+/* Value recorded by parallel_set_blocksize (include/parallel.h); 16 by default. */
+int ukhip_get_blocksize(void);
+
+/* One problem in packed form: every per-step matrix column-major, steps
+ * concatenated. Step 0 has no evolution equation (its F, H, c, K are ignored). */
+typedef struct {
+  int32_t n;          /* state dimension, 1 .. 64 */
+  int32_t m;          /* observation rows per step (maximum), 0 .. n */
+  int64_t k;          /* steps, >= 1 */
+  const double* F;    /* k * n*n */
+  const double* H;    /* k * n*n, or NULL for H_i = I */
+  const double* c;    /* k * n, or NULL for c_i = 0 */
+  const double* K;    /* k * n*n, or k * n when K_type == 'w' */
+  char K_type;        /* 'W', 'w', 'C', 'U', 'F' (c/kalman_base.c:61-175) */
+  const double* G;    /* k * m*n, ld m */
+  const double* o;    /* k * m */
+  const int32_t* m_i; /* k observation counts (0 = unobserved), or NULL for m on every step */
+  const double* C;    /* k * m*m (ld m), or k * m when C_type == 'w' */
+  char C_type;
+} ukhip_problem_t;
+
+/* A set of independent problems with one state dimension, resident on the GPU.
+ * Create uploads the inputs (host -> HBM); smooth runs entirely on the device. */
+typedef struct ukhip_batch ukhip_batch_t;
+
+ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems);
+void ukhip_batch_free(ukhip_batch_t* b);
+int64_t ukhip_batch_steps(const ukhip_batch_t* b);
+int32_t ukhip_batch_levels(const ukhip_batch_t* b);
+
+/* Odd-even smoothing of every problem of the batch (options: KALMAN_NO_COVARIANCE
+ * honoured). Enqueued on the library stream; returns 0 or -1 (see ukhip_last_error). */
+int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options);
+/* Associative-scan smoothing of every problem of the batch. */
+int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options);
+int ukhip_synchronize(void);
+
+/* Copies problem p's estimates (k*n) and covariances (k*n*n, may be NULL) to the host. */
+int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covariances);
+
+/* Timed odd-even run: per-phase device milliseconds measured with HIP events on
+ * the library stream (weigh, elimination, back substitution) and the total. */
+int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double* ms_weigh, double* ms_elim,
+                                     double* ms_back, double* ms_total);
+
+/* The HIP stream (hipStream_t) every libukhip launch uses. */
+void* ukhip_stream(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UKHIP_H */

@@ -1,0 +1,742 @@
+/*
+ * ukhip_host.cpp -- host side of libukhip: packing step equations into device
+ * structure-of-arrays buffers, planning the odd-even levels of a batch, and
+ * launching the HIP kernels on the library stream.
+ *
+ * Takes the place of the reference's parallel backends under the smoothers
+ * (c/parallel.h:31-46, c/parallel_tbb.cpp, c/parallel_sequential.c): the
+ * per-pair loops of each level run as one kernel launch over all pairs of
+ * all problems of a batch.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/kalman.h"
+#include "../../include/ukhip.h"
+#include "ukhip_internal.h"
+
+extern "C" {
+int uk_bucket(int n);
+hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
+hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
+                               const uk_factors_t* fac, int n, hipStream_t st);
+hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
+                             const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
+                             hipStream_t st);
+int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options);
+void uk_set_error(const char* fmt, ...);
+}
+
+/* ------------------------------------------------------------------------ */
+/* errors and the library stream                                            */
+/* ------------------------------------------------------------------------ */
+
+static thread_local std::string g_err;
+
+extern "C" void uk_set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+extern "C" const char* ukhip_last_error(void) { return g_err.c_str(); }
+extern "C" void ukhip_clear_error(void) { g_err.clear(); }
+
+static hipStream_t g_stream = nullptr;
+static std::once_flag g_stream_once;
+static hipError_t g_stream_err = hipSuccess;
+
+static bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  uk_set_error("%s: %s", what, hipGetErrorString(e));
+  return false;
+}
+
+static bool ensure_stream() {
+  std::call_once(g_stream_once, [] {
+    int count = 0;
+    g_stream_err = hipGetDeviceCount(&count);
+    if (g_stream_err == hipSuccess && count == 0) g_stream_err = hipErrorNoDevice;
+    if (g_stream_err == hipSuccess) g_stream_err = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+  });
+  return hip_ok(g_stream_err, "libukhip needs a HIP device (gfx950)");
+}
+
+extern "C" void* ukhip_stream(void) {
+  ensure_stream();
+  return (void*)g_stream;
+}
+
+extern "C" int ukhip_synchronize(void) {
+  if (!ensure_stream()) return -1;
+  return hip_ok(hipStreamSynchronize(g_stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batch                                                                      */
+/* ------------------------------------------------------------------------ */
+
+struct LevelHost {
+  int32_t nprob = 0;
+  int64_t units = 0, evens = 0, steps = 0, next_steps = 0;
+  std::vector<int32_t> prob;
+  std::vector<int64_t> L, unit_start, even_start, step_start, next_start;
+  int64_t fac_base = 0;
+  uk_level_t dev{};
+};
+
+struct ukhip_batch {
+  int n = 0, mr = 0, nprob = 0;
+  int64_t steps = 0;
+  std::vector<int64_t> k, orig_start;
+  std::vector<LevelHost> levels;
+  /* device memory */
+  void* d_raw = nullptr;
+  void* d_work = nullptr;
+  void* d_tab = nullptr;
+  size_t raw_bytes = 0, work_bytes = 0;
+  uk_raw_t raw{};
+  uk_chain_t chain[2]{};
+  uk_factors_t fac{};
+  double* X[2]{};
+  double* XT[2]{};
+  double* xout = nullptr;
+  double* Sout = nullptr;
+  int64_t total_evens = 0;
+  /* associative-path workspace, allocated on first use */
+  void* d_assoc = nullptr;
+  size_t assoc_bytes = 0;
+};
+
+static int cov_code(char t) {
+  switch (t) {
+    case 'W': return UK_COV_W;
+    case 'w': return UK_COV_w;
+    case 'C': return UK_COV_C;
+    case 'U':
+    case 'F': return UK_COV_L;
+    default: return -1;
+  }
+}
+
+/* Host staging of the raw inputs in the device layout of uk_raw_t. */
+struct Staging {
+  int n = 0, mr = 0;
+  int64_t steps = 0;
+  int H_ident = 1, K_full = 0, C_full = 0;
+  size_t offF = 0, offH = 0, offc = 0, offK = 0, offG = 0, offo = 0, offC = 0, offKt = 0, offCt = 0, offm = 0,
+         offfirst = 0, bytes = 0;
+  char* host = nullptr;
+
+  void layout() {
+    const size_t nn = (size_t)n * n, S = (size_t)steps;
+    size_t o = 0;
+    auto take = [&](size_t b) {
+      size_t r = o;
+      o += (b + 255) & ~(size_t)255;
+      return r;
+    };
+    offF = take(S * nn * 8);
+    offH = H_ident ? 0 : take(S * nn * 8);
+    offc = take(S * n * 8);
+    offK = take(S * (K_full ? nn : (size_t)n) * 8);
+    offG = take(S * (size_t)mr * n * 8);
+    offo = take(S * (size_t)mr * 8);
+    offC = take(S * (C_full ? (size_t)mr * mr : (size_t)mr) * 8);
+    offKt = take(S);
+    offCt = take(S);
+    offm = take(S * 4);
+    offfirst = take(S);
+    bytes = o;
+  }
+  double* F() { return (double*)(host + offF); }
+  double* H() { return (double*)(host + offH); }
+  double* c() { return (double*)(host + offc); }
+  double* K() { return (double*)(host + offK); }
+  double* G() { return (double*)(host + offG); }
+  double* o() { return (double*)(host + offo); }
+  double* C() { return (double*)(host + offC); }
+  int8_t* Kt() { return (int8_t*)(host + offKt); }
+  int8_t* Ct() { return (int8_t*)(host + offCt); }
+  int32_t* m() { return (int32_t*)(host + offm); }
+  int8_t* first() { return (int8_t*)(host + offfirst); }
+};
+
+static void plan_levels(ukhip_batch* b) {
+  b->levels.clear();
+  std::vector<int64_t> Lcur(b->k.begin(), b->k.end());
+  std::vector<int32_t> active(b->nprob);
+  for (int i = 0; i < b->nprob; ++i) active[i] = i;
+  int level = 0;
+  int64_t fac_base = 0;
+  while (!active.empty()) {
+    LevelHost lv;
+    lv.nprob = (int32_t)active.size();
+    int64_t us = 0, es = 0, ss = 0, ns = 0;
+    std::vector<int32_t> next_active;
+    std::vector<int64_t> next_L;
+    for (int32_t p : active) {
+      const int64_t L = Lcur[p];
+      lv.prob.push_back(p);
+      lv.L.push_back(L);
+      lv.unit_start.push_back(us);
+      lv.even_start.push_back(es);
+      lv.step_start.push_back(ss);
+      lv.next_start.push_back(ns);
+      us += (L == 1) ? 1 : L / 2;
+      es += (L + 1) / 2;
+      ss += L;
+      if (L >= 2) {
+        ns += L / 2;
+        next_active.push_back(p);
+      }
+    }
+    lv.unit_start.push_back(us);
+    lv.even_start.push_back(es);
+    lv.units = us;
+    lv.evens = es;
+    lv.steps = ss;
+    lv.next_steps = ns;
+    lv.fac_base = fac_base;
+    fac_base += es;
+    b->levels.push_back(std::move(lv));
+    for (int32_t p : next_active) Lcur[p] = Lcur[p] / 2;
+    active.swap(next_active);
+    ++level;
+  }
+  b->total_evens = fac_base;
+}
+
+static void batch_release(ukhip_batch* b) {
+  if (!b) return;
+  if (b->d_raw) (void)hipFree(b->d_raw);
+  if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_tab) (void)hipFree(b->d_tab);
+  if (b->d_assoc) (void)hipFree(b->d_assoc);
+  delete b;
+}
+
+/* Allocates device memory, uploads staging, builds level tables. */
+static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& lengths) {
+  ukhip_batch* b = new ukhip_batch();
+  b->n = st.n;
+  b->mr = st.mr;
+  b->nprob = (int)lengths.size();
+  b->steps = st.steps;
+  b->k = lengths;
+  b->orig_start.resize(b->nprob);
+  int64_t acc = 0;
+  for (int p = 0; p < b->nprob; ++p) {
+    b->orig_start[p] = acc;
+    acc += lengths[p];
+  }
+  plan_levels(b);
+
+  const int n = st.n;
+  const size_t nn = (size_t)n * n, S = (size_t)st.steps;
+  /* raw inputs */
+  if (!hip_ok(hipMalloc(&b->d_raw, st.bytes), "hipMalloc(raw inputs)")) { batch_release(b); return nullptr; }
+  b->raw_bytes = st.bytes;
+  if (!hip_ok(hipMemcpyAsync(b->d_raw, st.host, st.bytes, hipMemcpyHostToDevice, g_stream), "upload")) {
+    batch_release(b);
+    return nullptr;
+  }
+  char* r = (char*)b->d_raw;
+  b->raw.n = n;
+  b->raw.mr = st.mr;
+  b->raw.steps = st.steps;
+  b->raw.H_ident = st.H_ident;
+  b->raw.K_full = st.K_full;
+  b->raw.C_full = st.C_full;
+  b->raw.F = (const double*)(r + st.offF);
+  b->raw.H = st.H_ident ? nullptr : (const double*)(r + st.offH);
+  b->raw.c = (const double*)(r + st.offc);
+  b->raw.K = (const double*)(r + st.offK);
+  b->raw.G = (const double*)(r + st.offG);
+  b->raw.o = (const double*)(r + st.offo);
+  b->raw.C = (const double*)(r + st.offC);
+  b->raw.Kt = (const int8_t*)(r + st.offKt);
+  b->raw.Ct = (const int8_t*)(r + st.offCt);
+  b->raw.m = (const int32_t*)(r + st.offm);
+  b->raw.first = (const int8_t*)(r + st.offfirst);
+
+  /* workspace: chains (2), factors, cross covariances (2 x 2), outputs */
+  int64_t L1 = b->levels.size() > 1 ? b->levels[1].steps : 1;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t rr = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return rr;
+  };
+  size_t chain_off[2][5];
+  for (int cb = 0; cb < 2; ++cb) {
+    const size_t cs = cb == 0 ? S : (size_t)std::max<int64_t>(L1, 1);
+    chain_off[cb][0] = take(cs * nn * 8);
+    chain_off[cb][1] = take(cs * nn * 8);
+    chain_off[cb][2] = take(cs * n * 8);
+    chain_off[cb][3] = take(cs * nn * 8);
+    chain_off[cb][4] = take(cs * n * 8);
+  }
+  const size_t E = (size_t)b->total_evens;
+  const size_t fR = take(E * nn * 8), fB = take(E * nn * 8), fY = take(E * nn * 8), fc = take(E * n * 8);
+  size_t xo[2], xto[2];
+  for (int cb = 0; cb < 2; ++cb) {
+    const size_t cs = cb == 0 ? S : (size_t)std::max<int64_t>(L1, 1);
+    xo[cb] = take(cs * nn * 8);
+    xto[cb] = take(cs * nn * 8);
+  }
+  const size_t xoo = take(S * n * 8), Soo = take(S * nn * 8);
+  b->work_bytes = o;
+  if (!hip_ok(hipMalloc(&b->d_work, o), "hipMalloc(workspace)")) { batch_release(b); return nullptr; }
+  char* w = (char*)b->d_work;
+  for (int cb = 0; cb < 2; ++cb) {
+    b->chain[cb].B = (double*)(w + chain_off[cb][0]);
+    b->chain[cb].D = (double*)(w + chain_off[cb][1]);
+    b->chain[cb].c = (double*)(w + chain_off[cb][2]);
+    b->chain[cb].C = (double*)(w + chain_off[cb][3]);
+    b->chain[cb].o = (double*)(w + chain_off[cb][4]);
+    b->X[cb] = (double*)(w + xo[cb]);
+    b->XT[cb] = (double*)(w + xto[cb]);
+  }
+  b->fac.R = (double*)(w + fR);
+  b->fac.Bt = (double*)(w + fB);
+  b->fac.Y = (double*)(w + fY);
+  b->fac.cf = (double*)(w + fc);
+  b->xout = (double*)(w + xoo);
+  b->Sout = (double*)(w + Soo);
+
+  /* level tables: one int64 array; prob indices stored widened */
+  std::vector<int64_t> tab;
+  struct Off { size_t prob, L, U, E, S, N; };
+  std::vector<Off> offs;
+  for (auto& lv : b->levels) {
+    Off f;
+    auto put = [&](const int64_t* v, size_t cnt) {
+      size_t at = tab.size();
+      tab.insert(tab.end(), v, v + cnt);
+      return at;
+    };
+    std::vector<int64_t> pw(lv.prob.begin(), lv.prob.end());
+    f.prob = put(pw.data(), pw.size());
+    f.L = put(lv.L.data(), lv.nprob);
+    f.U = put(lv.unit_start.data(), lv.nprob + 1);
+    f.E = put(lv.even_start.data(), lv.nprob + 1);
+    f.S = put(lv.step_start.data(), lv.nprob);
+    f.N = put(lv.next_start.data(), lv.nprob);
+    offs.push_back(f);
+  }
+  const size_t oOrig = tab.size();
+  tab.insert(tab.end(), b->orig_start.begin(), b->orig_start.end());
+  /* prob[] is read as int32 by the kernels: store a packed int32 copy after the int64 data */
+  const size_t oProb32 = tab.size();
+  size_t nprob_all = 0;
+  for (auto& lv : b->levels) nprob_all += lv.nprob;
+  tab.resize(tab.size() + (nprob_all + 1) / 2 + 1, 0);
+  {
+    int32_t* p32 = reinterpret_cast<int32_t*>(tab.data() + oProb32);
+    size_t at = 0;
+    for (auto& lv : b->levels)
+      for (int32_t v : lv.prob) p32[at++] = v;
+  }
+  if (!hip_ok(hipMalloc(&b->d_tab, tab.size() * 8), "hipMalloc(level tables)")) { batch_release(b); return nullptr; }
+  if (!hip_ok(hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, g_stream), "upload tables") ||
+      !hip_ok(hipStreamSynchronize(g_stream), "upload")) {
+    batch_release(b);
+    return nullptr;
+  }
+  const int64_t* d = (const int64_t*)b->d_tab;
+  const int32_t* d32 = reinterpret_cast<const int32_t*>(d + oProb32);
+  size_t at32 = 0;
+  for (size_t l = 0; l < b->levels.size(); ++l) {
+    auto& lv = b->levels[l];
+    const Off& f = offs[l];
+    lv.dev.nprob = lv.nprob;
+    lv.dev.level = (int32_t)l;
+    lv.dev.units = lv.units;
+    lv.dev.evens = lv.evens;
+    lv.dev.prob = d32 + at32;
+    at32 += lv.nprob;
+    lv.dev.L = d + f.L;
+    lv.dev.unit_start = d + f.U;
+    lv.dev.even_start = d + f.E;
+    lv.dev.step_start = d + f.S;
+    lv.dev.next_start = d + f.N;
+    lv.dev.orig_start = d + oOrig;
+  }
+  return b;
+}
+
+/* ------------------------------------------------------------------------ */
+/* packing                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static bool staging_alloc(Staging& st) {
+  st.layout();
+  return hip_ok(hipHostMalloc((void**)&st.host, st.bytes, hipHostMallocDefault), "hipHostMalloc(staging)");
+}
+static void staging_free(Staging& st) {
+  if (st.host) (void)hipHostFree(st.host);
+  st.host = nullptr;
+}
+
+static void copy_cols(double* dst, int ld_dst, const kalman_matrix_t* A, int rows, int cols) {
+  for (int j = 0; j < cols; ++j)
+    for (int i = 0; i < rows; ++i) dst[(size_t)j * ld_dst + i] = A->elements[(size_t)j * A->ld + i];
+}
+
+static bool is_identity(const kalman_matrix_t* A) {
+  for (int j = 0; j < A->col_dim; ++j)
+    for (int i = 0; i < A->row_dim; ++i)
+      if (A->elements[(size_t)j * A->ld + i] != (i == j ? 1.0 : 0.0)) return false;
+  return true;
+}
+
+#define CHECK_SHAPE(A, r, c, what)                                                                            \
+  do {                                                                                                          \
+    if ((A) == NULL || (A)->row_dim != (r) || (A)->col_dim != (c)) {                                           \
+      uk_set_error("step %lld: %s must be %d x %d", (long long)i, what, (int)(r), (int)(c));                     \
+      return false;                                                                                             \
+    }                                                                                                           \
+  } while (0)
+
+static bool cov_shape_ok(const kalman_matrix_t* A, char t, int dim) {
+  if (!A) return false;
+  if (t == 'w') return A->row_dim == dim && A->col_dim == 1;
+  return A->row_dim == dim && A->col_dim == dim;
+}
+
+/* Validates and packs kalman_step_equations_t (c/kalman.h:56-75). */
+static bool pack_equations(kalman_step_equations_t** eq, int64_t k, Staging& st) {
+  if (k < 1) { uk_set_error("length must be >= 1"); return false; }
+  const int n = eq[0]->dimension;
+  if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("state dimension %d outside 1..%d", n, UKHIP_MAX_N); return false; }
+  int mr = 0;
+  st.H_ident = 1;
+  st.K_full = 0;
+  st.C_full = 0;
+  for (int64_t i = 0; i < k; ++i) {
+    kalman_step_equations_t* e = eq[i];
+    if (e->dimension != n) { uk_set_error("step %lld: dimension %d differs from %d", (long long)i, e->dimension, n); return false; }
+    if (i > 0) {
+      CHECK_SHAPE(e->H, n, n, "H");
+      CHECK_SHAPE(e->F, n, n, "F");
+      CHECK_SHAPE(e->c, n, 1, "c");
+      if (cov_code(e->K_type) < 0 || !cov_shape_ok(e->K, e->K_type, n)) {
+        uk_set_error("step %lld: K of type '%c' has the wrong shape or type", (long long)i, e->K_type);
+        return false;
+      }
+      if (e->K_type != 'w') st.K_full = 1;
+      if (st.H_ident && !is_identity(e->H)) st.H_ident = 0;
+    }
+    if (e->o != NULL) {
+      const int m = e->o->row_dim;
+      if (m > n) { uk_set_error("step %lld: %d observations exceed the state dimension %d", (long long)i, m, n); return false; }
+      CHECK_SHAPE(e->o, m, 1, "o");
+      CHECK_SHAPE(e->G, m, n, "G");
+      if (cov_code(e->C_type) < 0 || !cov_shape_ok(e->C, e->C_type, m)) {
+        uk_set_error("step %lld: C of type '%c' has the wrong shape or type", (long long)i, e->C_type);
+        return false;
+      }
+      if (e->C_type != 'w') st.C_full = 1;
+      mr = std::max(mr, m);
+    }
+  }
+  st.n = n;
+  st.mr = std::max(mr, 1);
+  st.steps = k;
+  if (!staging_alloc(st)) return false;
+  const size_t nn = (size_t)n * n;
+  const int Kw = st.K_full ? n * n : n, Cw = st.C_full ? st.mr * st.mr : st.mr;
+  std::memset(st.host, 0, st.bytes);
+  for (int64_t i = 0; i < k; ++i) {
+    kalman_step_equations_t* e = eq[i];
+    st.first()[i] = i == 0;
+    if (i > 0) {
+      copy_cols(st.F() + i * nn, n, e->F, n, n);
+      if (!st.H_ident) copy_cols(st.H() + i * nn, n, e->H, n, n);
+      copy_cols(st.c() + i * n, n, e->c, n, 1);
+      copy_cols(st.K() + i * Kw, n, e->K, e->K->row_dim, e->K->col_dim);
+      st.Kt()[i] = (int8_t)cov_code(e->K_type);
+    }
+    if (e->o != NULL) {
+      const int m = e->o->row_dim;
+      st.m()[i] = m;
+      copy_cols(st.G() + i * (size_t)st.mr * n, st.mr, e->G, m, n);
+      copy_cols(st.o() + i * (size_t)st.mr, st.mr, e->o, m, 1);
+      if (e->C_type == 'w') copy_cols(st.C() + i * (size_t)Cw, st.mr, e->C, m, 1);
+      else copy_cols(st.C() + i * (size_t)Cw, st.mr, e->C, m, m);
+      st.Ct()[i] = (int8_t)cov_code(e->C_type);
+    }
+  }
+  return true;
+}
+
+/* Validates and packs ukhip_problem_t arrays. */
+static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, std::vector<int64_t>& lengths) {
+  if (nprob < 1) { uk_set_error("no problems"); return false; }
+  const int n = P[0].n;
+  if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("state dimension %d outside 1..%d", n, UKHIP_MAX_N); return false; }
+  int mr = 1;
+  int64_t S = 0;
+  st.H_ident = 1;
+  st.K_full = 0;
+  st.C_full = 0;
+  for (int p = 0; p < nprob; ++p) {
+    const ukhip_problem_t& q = P[p];
+    if (q.n != n) { uk_set_error("problem %d: n = %d differs from %d (one n per batch)", p, q.n, n); return false; }
+    if (q.k < 1 || q.m < 0 || q.m > n) { uk_set_error("problem %d: need k >= 1 and 0 <= m <= n", p); return false; }
+    if (cov_code(q.K_type) < 0 || cov_code(q.C_type) < 0) { uk_set_error("problem %d: unknown covariance type", p); return false; }
+    if (!q.F || !q.K || (q.m > 0 && (!q.G || !q.o || !q.C))) { uk_set_error("problem %d: missing input array", p); return false; }
+    if (q.H) st.H_ident = 0;
+    if (q.K_type != 'w') st.K_full = 1;
+    if (q.C_type != 'w') st.C_full = 1;
+    mr = std::max(mr, q.m);
+    S += q.k;
+    lengths.push_back(q.k);
+  }
+  st.n = n;
+  st.mr = mr;
+  st.steps = S;
+  if (!staging_alloc(st)) return false;
+  std::memset(st.host, 0, st.bytes);
+  const size_t nn = (size_t)n * n;
+  const int Kw = st.K_full ? n * n : n, Cw = st.C_full ? mr * mr : mr;
+  int64_t s = 0;
+  for (int p = 0; p < nprob; ++p) {
+    const ukhip_problem_t& q = P[p];
+    const int m = q.m;
+    const int kw = q.K_type == 'w' ? n : n * n;
+    const int cw = q.C_type == 'w' ? m : m * m;
+    for (int64_t i = 0; i < q.k; ++i, ++s) {
+      st.first()[s] = i == 0;
+      if (i > 0) {
+        std::memcpy(st.F() + s * nn, q.F + i * nn, nn * 8);
+        if (!st.H_ident) {
+          if (q.H) std::memcpy(st.H() + s * nn, q.H + i * nn, nn * 8);
+          else for (int d = 0; d < n; ++d) st.H()[s * nn + (size_t)d * n + d] = 1.0;
+        }
+        if (q.c) std::memcpy(st.c() + s * n, q.c + i * n, (size_t)n * 8);
+        std::memcpy(st.K() + s * Kw, q.K + i * kw, (size_t)kw * 8);
+        st.Kt()[s] = (int8_t)cov_code(q.K_type);
+      }
+      const int mi = q.m_i ? q.m_i[i] : m;
+      if (mi > 0) {
+        st.m()[s] = mi;
+        for (int col = 0; col < n; ++col)
+          std::memcpy(st.G() + s * (size_t)mr * n + (size_t)col * mr, q.G + i * (size_t)m * n + (size_t)col * m,
+                      (size_t)mi * 8);
+        std::memcpy(st.o() + s * (size_t)mr, q.o + i * (size_t)m, (size_t)mi * 8);
+        if (q.C_type == 'w') std::memcpy(st.C() + s * (size_t)Cw, q.C + i * (size_t)cw, (size_t)mi * 8);
+        else
+          for (int col = 0; col < mi; ++col)
+            std::memcpy(st.C() + s * (size_t)Cw + (size_t)col * mr, q.C + i * (size_t)cw + (size_t)col * m,
+                        (size_t)mi * 8);
+        st.Ct()[s] = (int8_t)cov_code(q.C_type);
+      }
+    }
+  }
+  return true;
+}
+
+/* ------------------------------------------------------------------------ */
+/* running                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* 4 events or NULL */) {
+  hipStream_t st = g_stream;
+  if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
+  if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
+  if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
+  const int nl = (int)b->levels.size();
+  for (int l = 0; l < nl; ++l) {
+    uk_chain_t in = b->chain[l & 1];
+    in.mr = l == 0 ? b->mr : b->n;
+    uk_chain_t out = b->chain[(l + 1) & 1];
+    out.mr = b->n;
+    uk_factors_t f = b->fac;
+    const int64_t fb = b->levels[l].fac_base;
+    const size_t nn = (size_t)b->n * b->n;
+    f.R += fb * nn;
+    f.Bt += fb * nn;
+    f.Y += fb * nn;
+    f.cf += fb * b->n;
+    uk_level_t lv = b->levels[l].dev; /* even_start is relative to the level's fac_base */
+    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->n, st), "k_eliminate launch")) return false;
+  }
+  if (ev && !hip_ok(hipEventRecord(ev[2], st), "hipEventRecord")) return false;
+  for (int l = nl - 1; l >= 0; --l) {
+    uk_factors_t f = b->fac;
+    const int64_t fb = b->levels[l].fac_base;
+    const size_t nn = (size_t)b->n * b->n;
+    f.R += fb * nn;
+    f.Bt += fb * nn;
+    f.Y += fb * nn;
+    f.cf += fb * b->n;
+    uk_level_t lv = b->levels[l].dev;
+    if (!hip_ok(uk_launch_backsub(&lv, &f, b->xout, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], b->X[l & 1],
+                                  b->XT[l & 1], b->n, cov, st),
+                "k_backsub launch"))
+      return false;
+  }
+  if (ev && !hip_ok(hipEventRecord(ev[3], st), "hipEventRecord")) return false;
+  return true;
+}
+
+extern "C" ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems) {
+  if (!ensure_stream()) return nullptr;
+  Staging st;
+  std::vector<int64_t> lengths;
+  if (!pack_problems(nprob, problems, st, lengths)) {
+    staging_free(st);
+    return nullptr;
+  }
+  ukhip_batch* b = batch_from_staging(st, lengths);
+  if (b && !hip_ok(hipStreamSynchronize(g_stream), "upload")) {
+    batch_release(b);
+    b = nullptr;
+  }
+  staging_free(st);
+  return b;
+}
+
+extern "C" void ukhip_batch_free(ukhip_batch_t* b) {
+  if (b && g_stream) (void)hipStreamSynchronize(g_stream);
+  batch_release(b);
+}
+extern "C" int64_t ukhip_batch_steps(const ukhip_batch_t* b) { return b ? b->steps : 0; }
+extern "C" int32_t ukhip_batch_levels(const ukhip_batch_t* b) { return b ? (int32_t)b->levels.size() : 0; }
+
+extern "C" int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options) {
+  if (!b) { uk_set_error("NULL batch"); return -1; }
+  if (!ensure_stream()) return -1;
+  return run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, nullptr) ? 0 : -1;
+}
+
+extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options) {
+  if (!b) { uk_set_error("NULL batch"); return -1; }
+  if (!ensure_stream()) return -1;
+  return uk_assoc_smooth(b, options);
+}
+
+extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double* ms_weigh,
+                                                double* ms_elim, double* ms_back, double* ms_total) {
+  if (!b) { uk_set_error("NULL batch"); return -1; }
+  if (!ensure_stream()) return -1;
+  hipEvent_t ev[4];
+  for (auto& e : ev)
+    if (!hip_ok(hipEventCreate(&e), "hipEventCreate")) return -1;
+  bool ok = run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev) &&
+            hip_ok(hipEventSynchronize(ev[3]), "hipEventSynchronize");
+  if (ok) {
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+    (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+    (void)hipEventElapsedTime(&t03, ev[0], ev[3]);
+    if (ms_weigh) *ms_weigh = t01;
+    if (ms_elim) *ms_elim = t12;
+    if (ms_back) *ms_back = t23;
+    if (ms_total) *ms_total = t03;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return ok ? 0 : -1;
+}
+
+extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covs) {
+  if (!b || p < 0 || p >= b->nprob) { uk_set_error("bad batch or problem index"); return -1; }
+  const size_t n = b->n, nn = n * n;
+  const int64_t o = b->orig_start[p], k = b->k[p];
+  if (states && !hip_ok(hipMemcpyAsync(states, b->xout + o * n, k * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
+    return -1;
+  if (covs && !hip_ok(hipMemcpyAsync(covs, b->Sout + o * nn, k * nn * 8, hipMemcpyDeviceToHost, g_stream), "download"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(g_stream), "download") ? 0 : -1;
+}
+
+/* Device pointers for sibling translation units (associative path). */
+extern "C" void uk_batch_view(ukhip_batch* b, const uk_raw_t** raw, double** xout, double** Sout, int64_t* steps,
+                              int* nprob, const int64_t** k, void** assoc_ws, size_t* assoc_bytes) {
+  *raw = &b->raw;
+  *xout = b->xout;
+  *Sout = b->Sout;
+  *steps = b->steps;
+  *nprob = b->nprob;
+  *k = b->k.data();
+  *assoc_ws = b->d_assoc;
+  *assoc_bytes = b->assoc_bytes;
+}
+extern "C" void uk_batch_set_assoc_ws(ukhip_batch* b, void* p, size_t bytes) {
+  b->d_assoc = p;
+  b->assoc_bytes = bytes;
+}
+
+/* ------------------------------------------------------------------------ */
+/* the reference's smoother entry points (c/kalman.h:148-149)                */
+/* ------------------------------------------------------------------------ */
+
+static void smooth_equations(kalman_options_t options, kalman_step_equations_t** equations, kalman_step_index_t length,
+                             bool associative) {
+  if (!ensure_stream()) return;
+  if (!equations || length < 1) {
+    uk_set_error("kalman_smooth: need length >= 1 and an equations array");
+    return;
+  }
+  Staging st;
+  if (!pack_equations(equations, length, st)) {
+    staging_free(st);
+    return;
+  }
+  std::vector<int64_t> lengths{(int64_t)length};
+  ukhip_batch* b = batch_from_staging(st, lengths);
+  staging_free(st);
+  if (!b) return;
+  const int cov = associative ? 1 : ((options & KALMAN_NO_COVARIANCE) == 0);
+  int rc = associative ? uk_assoc_smooth(b, options) : (run_oddeven(b, cov, nullptr) ? 0 : -1);
+  if (rc != 0) {
+    batch_release(b);
+    return;
+  }
+  const int n = b->n;
+  std::vector<double> xs((size_t)length * n), Ss(cov ? (size_t)length * n * n : 0);
+  if (ukhip_batch_download(b, 0, xs.data(), cov ? Ss.data() : nullptr) != 0) {
+    batch_release(b);
+    return;
+  }
+  batch_release(b);
+  for (kalman_step_index_t i = 0; i < length; ++i) {
+    kalman_step_equations_t* e = equations[i];
+    if (associative) { /* c/kalman_associative_smoother.c:1084-1086 frees earlier results */
+      matrix_free(e->state);
+      matrix_free(e->covariance);
+    }
+    e->state = matrix_create(n, 1);
+    std::memcpy(e->state->elements, &xs[(size_t)i * n], (size_t)n * 8);
+    if (cov) {
+      e->covariance = matrix_create(n, n);
+      std::memcpy(e->covariance->elements, &Ss[(size_t)i * n * n], (size_t)n * n * 8);
+    } else {
+      e->covariance = NULL;
+    }
+    e->covariance_type = 'C';
+  }
+}
+
+extern "C" void kalman_smooth_oddeven(kalman_options_t options, kalman_step_equations_t** equations,
+                                      kalman_step_index_t length) {
+  smooth_equations(options, equations, length, false);
+}
+
+extern "C" void kalman_smooth_associative(kalman_options_t options, kalman_step_equations_t** equations,
+                                          kalman_step_index_t length) {
+  smooth_equations(options, equations, length, true);
+}

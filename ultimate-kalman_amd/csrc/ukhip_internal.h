@@ -1,0 +1,81 @@
+/*
+ * ukhip_internal.h -- device data layout shared by the host planner and the
+ * HIP kernels of libukhip.
+ *
+ * A "batch" is a set of independent problems with one state dimension n.
+ * Steps of all problems are concatenated; every per-step matrix is stored
+ * column-major in structure-of-arrays buffers (one buffer per field, step s
+ * at s * stride). The odd-even elimination runs level by level over the whole
+ * batch: level l holds, per problem, L_l steps (L_0 = k, L_{l+1} = L_l / 2).
+ */
+#ifndef UKHIP_INTERNAL_H
+#define UKHIP_INTERNAL_H
+
+#include <stdint.h>
+
+#define UKHIP_MAX_N 64
+
+/* covariance representation codes (c/kalman_base.c:61-175) */
+enum { UK_COV_W = 0, UK_COV_w = 1, UK_COV_C = 2, UK_COV_L = 3 /* 'U' and 'F' */ };
+
+/* Raw (unweighed) inputs of a batch, as packed by the host. */
+typedef struct {
+  int32_t n;        /* state dimension */
+  int32_t mr;       /* observation rows per step (max over steps) */
+  int64_t steps;    /* total steps over all problems */
+  int32_t H_ident;  /* 1: every H_i is the identity (H not stored) */
+  int32_t K_full;   /* 1: K stored n*n per step, else n (all 'w') */
+  int32_t C_full;   /* 1: C stored mr*mr per step, else mr (all 'w') */
+  const double* F;  /* steps * n*n            (first step of a problem unused) */
+  const double* H;  /* steps * n*n or NULL */
+  const double* c;  /* steps * n */
+  const double* K;  /* steps * (n*n | n) */
+  const double* G;  /* steps * mr*n  (ld = mr; rows >= m_i unused) */
+  const double* o;  /* steps * mr */
+  const double* C;  /* steps * (mr*mr | mr), ld = mr */
+  const int8_t* Kt; /* steps: UK_COV_* */
+  const int8_t* Ct; /* steps: UK_COV_* */
+  const int32_t* m; /* steps: observation count m_i (0 = unobserved) */
+  const int8_t* first; /* steps: 1 on the first step of a problem */
+} uk_raw_t;
+
+/* Per-level step arrays: the whitened chain of one elimination level.
+ *   B = -V F, D = V H, c = V c   (n x n, n x n, n; absent on a chain's first step)
+ *   C = W G (mr x n, ld mr), o = W o (mr)                                        */
+typedef struct {
+  double* B;
+  double* D;
+  double* c;
+  double* C;
+  double* o;
+  int32_t mr;
+} uk_chain_t;
+
+/* Per-level factors kept for the back substitution (one slot per even step):
+ *   R_j u_j + Bt_j u_{j-1} + Y_j u_{j+1} = cf_j */
+typedef struct {
+  double* R;
+  double* Bt;
+  double* Y;
+  double* cf;
+} uk_factors_t;
+
+/* Level plan: problem p of the batch has L[p] steps at this level. Work units
+ * of the elimination are numbered unit_start[p] .. unit_start[p+1]-1; even
+ * steps (back substitution) even_start[p] ..; steps step_start[p] ..; the
+ * next level's steps next_start[p] ..; original step numbers orig_start[p] + t. */
+typedef struct {
+  int32_t nprob;        /* problems active at this level */
+  int32_t level;
+  int64_t units;        /* elimination work units */
+  int64_t evens;        /* even steps */
+  const int32_t* prob;  /* nprob: batch problem index (for orig_start) */
+  const int64_t* L;
+  const int64_t* unit_start; /* nprob + 1 */
+  const int64_t* even_start; /* nprob + 1 */
+  const int64_t* step_start;
+  const int64_t* next_start;
+  const int64_t* orig_start; /* indexed by batch problem */
+} uk_level_t;
+
+#endif

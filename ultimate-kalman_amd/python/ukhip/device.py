@@ -1,0 +1,168 @@
+"""ctypes binding of libukhip (include/kalman.h, include/ukhip.h).
+
+Loads the in-tree ultimate-kalman_amd/libukhip.so. There is no fallback: if
+the library or a HIP device is missing, the calls fail loudly.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(os.path.dirname(HERE))
+LIB_PATH = os.path.join(PKG, "libukhip.so")
+
+
+class ProblemStruct(C.Structure):
+    """ukhip_problem_t (include/ukhip.h)."""
+    _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("k", C.c_int64),
+                ("F", C.c_void_p), ("H", C.c_void_p), ("c", C.c_void_p), ("K", C.c_void_p),
+                ("K_type", C.c_char), ("G", C.c_void_p), ("o", C.c_void_p), ("m_i", C.c_void_p),
+                ("C", C.c_void_p), ("C_type", C.c_char)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.ukhip_last_error.restype = C.c_char_p
+        L.ukhip_batch_create.restype = C.c_void_p
+        L.ukhip_batch_create.argtypes = [C.c_int32, C.POINTER(ProblemStruct)]
+        L.ukhip_batch_free.argtypes = [C.c_void_p]
+        L.ukhip_batch_steps.restype = C.c_int64
+        L.ukhip_batch_steps.argtypes = [C.c_void_p]
+        L.ukhip_batch_levels.restype = C.c_int32
+        L.ukhip_batch_levels.argtypes = [C.c_void_p]
+        for f in ("ukhip_batch_smooth_oddeven", "ukhip_batch_smooth_associative"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_uint32]
+            getattr(L, f).restype = C.c_int
+        L.ukhip_batch_smooth_oddeven_timed.argtypes = [C.c_void_p, C.c_uint32] + [C.POINTER(C.c_double)] * 4
+        L.ukhip_batch_smooth_oddeven_timed.restype = C.c_int
+        L.ukhip_batch_download.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.ukhip_batch_download.restype = C.c_int
+        L.ukhip_synchronize.restype = C.c_int
+        L.ukhip_stream.restype = C.c_void_p
+        capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().ukhip_last_error().decode()
+
+
+class UkhipError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise UkhipError(f"{what}: {last_error()}")
+
+
+def smooth_equations(problem, algorithm="oddeven", options=0):
+    """The reference's entry point on the GPU: kalman_smooth_<algorithm> over
+    kalman_step_equations_t built from `problem`. Returns (states, covs)."""
+    L = lib()
+    L.ukhip_clear_error()
+    fn = L.kalman_smooth_oddeven if algorithm == "oddeven" else L.kalman_smooth_associative
+    eqs = capi.EquationSet(problem)
+    fn(options, eqs.ptrs, problem.k)
+    err = last_error()
+    states, covs = eqs.collect(L.matrix_free, want_cov=not (options & capi.KALMAN_NO_COVARIANCE))
+    if err:
+        raise UkhipError(err)
+    return states, covs
+
+
+class Batch:
+    """Device-resident set of problems with one state dimension."""
+
+    def __init__(self, problems):
+        L = lib()
+        L.ukhip_clear_error()
+        self._keep = []
+        arr = (ProblemStruct * len(problems))()
+        for i, p in enumerate(problems):
+            arr[i] = self._struct(p)
+        self.problems = problems
+        self.h = L.ukhip_batch_create(len(problems), arr)
+        if not self.h:
+            raise UkhipError("ukhip_batch_create: " + last_error())
+
+    def _ptr(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._keep.append(a)
+        return a.ctypes.data
+
+    def _struct(self, p):
+        pk = p.packed
+        n, m, k = pk["n"], pk["m"], p.k
+        s = ProblemStruct()
+        s.n, s.m, s.k = n, m, k
+        s.F = self._ptr(pk["F"])
+        s.H = None
+        s.c = self._ptr(pk["c"])
+        K = pk["K"]
+        s.K_type = pk["K_type"].encode()
+        s.K = self._ptr(np.broadcast_to(np.asfortranarray(K).T.ravel(), (k, K.size)))
+        s.G = self._ptr(pk["G"])
+        s.o = self._ptr(pk["o"])
+        mi = np.where(pk["observed"], m, 0).astype(np.int32)
+        self._keep.append(mi)
+        s.m_i = mi.ctypes.data
+        Cm = pk["C"]
+        s.C_type = pk["C_type"].encode()
+        s.C = self._ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
+        return s
+
+    @property
+    def steps(self):
+        return lib().ukhip_batch_steps(self.h)
+
+    @property
+    def levels(self):
+        return lib().ukhip_batch_levels(self.h)
+
+    def smooth(self, algorithm="oddeven", options=0, sync=True):
+        L = lib()
+        fn = L.ukhip_batch_smooth_oddeven if algorithm == "oddeven" else L.ukhip_batch_smooth_associative
+        _check(fn(self.h, options), f"smooth_{algorithm}")
+        if sync:
+            _check(L.ukhip_synchronize(), "synchronize")
+
+    def smooth_timed(self, options=0):
+        t = [C.c_double() for _ in range(4)]
+        _check(lib().ukhip_batch_smooth_oddeven_timed(self.h, options, *[C.byref(x) for x in t]), "smooth_timed")
+        return [x.value for x in t]
+
+    def download(self, i, want_cov=True):
+        p = self.problems[i]
+        n = p.n[0]
+        x = np.empty((p.k, n))
+        cv = np.empty((p.k, n, n)) if want_cov else None
+        _check(lib().ukhip_batch_download(self.h, i, x.ctypes.data, cv.ctypes.data if want_cov else None),
+               "download")
+        if want_cov:
+            cv = np.swapaxes(cv, 1, 2)  # column-major per step
+        return x, cv
+
+    def free(self):
+        if self.h:
+            lib().ukhip_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
