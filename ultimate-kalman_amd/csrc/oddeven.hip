@@ -7,19 +7,19 @@
  * matlab/KalmanOddevenSmoother.m:21-358.
  *
  * k_weigh      whitening of one step per workgroup (c/kalman_base.c:61-175).
- * k_eliminate  one workgroup per (even, odd) step pair of a level: the three
- *              Householder QRs of the reference's phases fused, writing the
+ * k_phase1..3  the reference's three elimination phases of one level, one
+ *              workgroup per step pair (or even step), writing the
  *              back-substitution factors and the next level's chain.
  * k_backsub    one workgroup per even step: estimate and, optionally, the
  *              selected-inversion covariance and the two cross covariances.
  *
- * Register layout of the QR kernels: every matrix column is owned by a pair
- * of adjacent lanes. The column has two row blocks of NB rows (block 0 holds
- * the pivot rows, block 1 the stacked rows); sub-lane b holds the rows with
- * parity b of both blocks, NB doubles. A Householder reflector is published
- * through double-buffered LDS by the pivot column's lanes; every other lane
- * only runs uniform chunk loops (no dynamic register indexing), combining its
- * two half dot products with one DPP swap.
+ * Register layout of the QR kernels: every matrix column is owned by SL = 2
+ * or 4 adjacent lanes. The column has two row blocks of NB rows (block 0 holds
+ * the pivot rows, block 1 the stacked rows); sub-lane b holds the rows
+ * r = b (mod SL) of both blocks, 2 NB / SL doubles. A Householder reflector is
+ * published through double-buffered LDS by the pivot column's lanes; every
+ * other lane runs branch-free loops over its rows (no dynamic register
+ * indexing), combining its partial dot products with DPP quad permutations.
  *
  * Rows that are zero in every column stay zero under all reflectors (their
  * reflector entries are zero), so padding blocks with zero rows yields an
@@ -32,12 +32,25 @@
 
 namespace {
 
-__device__ __forceinline__ double pair_swap(double v) {
-  int lo = __double2loint(v);
-  int hi = __double2hiint(v);
+/* x + (x of the lane with index ^1) + ... over groups of SL = 2 or 4 adjacent
+ * lanes (DPP quad permutations, no LDS). */
+__device__ __forceinline__ double dpp_xor1(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false); /* quad_perm [1,0,3,2] */
   hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double dpp_xor2(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false); /* quad_perm [2,3,0,1] */
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int SL>
+__device__ __forceinline__ double group_sum(double v) {
+  v += dpp_xor1(v);
+  if (SL == 4) v += dpp_xor2(v);
+  return v;
 }
 
 __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
@@ -50,28 +63,29 @@ __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob,
   return lo;
 }
 
-template <int NB>
+/* One matrix column as held by one of its SL lanes: rows r = SL*l + sub of
+ * block 0 in locals [0,H) and of block 1 in [H,2H), H = NB/SL. */
+template <int NB, int SL>
 struct Col {
-  static constexpr int H = NB / 2; /* locals per block */
-  double a[NB];                    /* [0,H): block 0 rows 2l+sub; [H,NB): block 1 rows 2(l-H)+sub */
+  static constexpr int H = NB / SL;
+  double a[2 * H];
 
   __device__ __forceinline__ void zero_block(int blk) {
 #pragma unroll
     for (int l = 0; l < H; ++l) a[blk * H + l] = 0.0;
   }
   /* block blk <- rows [0, rows) of a column at src (unit stride), zero elsewhere */
-  __device__ __forceinline__ void load_block(int blk, const double* __restrict__ src, int rows, int sub,
-                                             double scale = 1.0) {
+  __device__ __forceinline__ void load_block(int blk, const double* __restrict__ src, int rows, int sub) {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
-      int r = 2 * l + sub;
-      a[blk * H + l] = (r < rows) ? scale * src[r] : 0.0;
+      const int r = SL * l + sub;
+      a[blk * H + l] = (r < rows) ? src[r] : 0.0;
     }
   }
   __device__ __forceinline__ void store_block(int blk, double* __restrict__ dst, int rows, int sub) const {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
-      int r = 2 * l + sub;
+      const int r = SL * l + sub;
       if (r < rows) dst[r] = a[blk * H + l];
     }
   }
@@ -79,7 +93,7 @@ struct Col {
   __device__ __forceinline__ void store_triu(double* __restrict__ dst, int rows, int col, int sub) const {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
-      int r = 2 * l + sub;
+      const int r = SL * l + sub;
       if (r < rows) dst[r] = (r <= col) ? a[l] : 0.0;
     }
   }
@@ -87,56 +101,49 @@ struct Col {
 #pragma unroll
     for (int l = 0; l < H; ++l) a[H + l] = a[l];
   }
-  __device__ __forceinline__ void move_1_to_0() {
-#pragma unroll
-    for (int l = 0; l < H; ++l) a[l] = a[H + l];
-  }
 };
 
-/* LDS reflector buffers: [buffer][sub][NB + 4] (the +4 skews the two
- * sub-lane arrays onto different banks), then [buffer][2] scalars. */
-template <int NB>
+/* LDS reflector buffers: [buffer][sub][2H + 4] (the +4 skews the sub-lane
+ * arrays onto different banks), then [buffer][2] scalars. */
+template <int NB, int SL>
 struct Refl {
-  static constexpr int SUBSTRIDE = NB + 4;
-  double v[2][2 * SUBSTRIDE];
+  static constexpr int SUBSTRIDE = 2 * (NB / SL) + 4;
+  double v[2][SL * SUBSTRIDE];
   double sc[2][2];
 };
 
 /*
  * One Householder step for pivot k (target row k of block 0).
- *   n0: block-0 rows that can be nonzero (pivot rows included), n1: block-1 rows.
- *   is_piv: this lane pair owns pivot column k; upd: this column is updated.
- * Publishes v (v_k = 1, v_i = 0 for i < k) and tau, then applies
- * H = I - tau v v^T to every updated column. One barrier.
+ *   is_piv: these lanes own pivot column k; upd: this column is updated.
+ * The pivot lanes publish the whole reflector, v_k = 1, v_i = 0 for i < k,
+ * v_i = x_i / (alpha - beta) below, and tau; then every updated column
+ * applies H = I - tau v v^T. The dot and the update run over all locals
+ * without branches (entries outside the active rows are exact zeros), so the
+ * LDS reads of v issue back to back. One barrier.
  */
-template <int NB>
-__device__ __forceinline__ void house_step(Col<NB>& col, Refl<NB>& R, int buf, int sub, bool is_piv, bool upd,
-                                           int k, int n0, int n1) {
-  constexpr int H = NB / 2;
-  const int c0 = (k >> 1) & ~1;     /* first chunk (2 locals) of block 0 holding rows >= k */
-  const int c0e = (n0 + 3) >> 1;    /* locals up to here may hold block-0 rows < n0 (rounded to chunks) */
-  const int c1e = (n1 + 3) >> 1;
-  double* vv = &R.v[buf][sub * Refl<NB>::SUBSTRIDE];
+template <int NB, int SL>
+__device__ __forceinline__ void house_step(Col<NB, SL>& col, Refl<NB, SL>& R, int buf, int sub, bool is_piv, bool upd,
+                                           int k) {
+  constexpr int H = NB / SL;
+  double* vv = &R.v[buf][sub * Refl<NB, SL>::SUBSTRIDE];
   if (is_piv) {
-    double s = 0.0, alpha = 0.0;
+    double s0 = 0.0, s1 = 0.0, alpha = 0.0;
 #pragma unroll
-    for (int l = 0; l < H; l += 2) {
-      if (l >= c0 && l < c0e) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const int r = 2 * (l + t) + sub;
-          const double x = col.a[l + t];
-          alpha = (r == k) ? x : alpha;
-          s = (r > k) ? fma(x, x, s) : s;
-        }
-      }
+    for (int l = 0; l < H; ++l) {
+      const int r = SL * l + sub;
+      const double x = col.a[l];
+      alpha = (r == k) ? x : alpha;
+      const double xm = (r > k) ? x : 0.0;
+      if (l & 1) s1 = fma(xm, xm, s1);
+      else s0 = fma(xm, xm, s0);
     }
 #pragma unroll
-    for (int l = H; l < NB; l += 2) {
-      if (l - H < c1e) s = fma(col.a[l], col.a[l], fma(col.a[l + 1], col.a[l + 1], s));
+    for (int l = H; l < 2 * H; ++l) {
+      if (l & 1) s1 = fma(col.a[l], col.a[l], s1);
+      else s0 = fma(col.a[l], col.a[l], s0);
     }
-    s += pair_swap(s);
-    alpha += pair_swap(alpha);
+    const double s = group_sum<SL>(s0 + s1);
+    alpha = group_sum<SL>(alpha);
     double tau = 0.0, beta = alpha, scale = 0.0;
     if (s != 0.0) {
       beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
@@ -144,22 +151,12 @@ __device__ __forceinline__ void house_step(Col<NB>& col, Refl<NB>& R, int buf, i
       scale = 1.0 / (alpha - beta);
     }
 #pragma unroll
-    for (int l = 0; l < H; l += 2) {
-      if (l >= c0 && l < c0e) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const int r = 2 * (l + t) + sub;
-          vv[l + t] = (r > k) ? col.a[l + t] * scale : (r == k ? 1.0 : 0.0);
-        }
-      }
+    for (int l = 0; l < H; ++l) {
+      const int r = SL * l + sub;
+      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? 1.0 : 0.0);
     }
 #pragma unroll
-    for (int l = H; l < NB; l += 2) {
-      if (l - H < c1e) {
-        vv[l] = col.a[l] * scale;
-        vv[l + 1] = col.a[l + 1] * scale;
-      }
-    }
+    for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l] * scale;
     if (sub == 0) {
       R.sc[buf][0] = tau;
       R.sc[buf][1] = beta;
@@ -169,268 +166,267 @@ __device__ __forceinline__ void house_step(Col<NB>& col, Refl<NB>& R, int buf, i
   if (upd) {
     const double tau = R.sc[buf][0];
     if (tau != 0.0) {
-      double d = 0.0;
+      /* v is read from LDS in chunks of CH locals issued back to back */
+      constexpr int NL = 2 * H;
+      constexpr int CH = NL <= 24 ? NL : 16;
+      const double2* v2 = reinterpret_cast<const double2*>(vv);
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
-      for (int l = 0; l < H; l += 2)
-        if (l >= c0 && l < c0e) d = fma(vv[l], col.a[l], fma(vv[l + 1], col.a[l + 1], d));
+      for (int c0 = 0; c0 < NL; c0 += CH) {
+        double vr[CH];
 #pragma unroll
-      for (int l = H; l < NB; l += 2)
-        if (l - H < c1e) d = fma(vv[l], col.a[l], fma(vv[l + 1], col.a[l + 1], d));
-      d += pair_swap(d);
+        for (int l = 0; l < CH; l += 2) {
+          const double2 t = v2[(c0 + l) / 2];
+          vr[l] = t.x;
+          vr[l + 1] = t.y;
+        }
+#pragma unroll
+        for (int l = 0; l < CH; l += 4) {
+          d0 = fma(vr[l], col.a[c0 + l], d0);
+          d1 = fma(vr[l + 1], col.a[c0 + l + 1], d1);
+          if (l + 2 < CH) d2 = fma(vr[l + 2], col.a[c0 + l + 2], d2);
+          if (l + 3 < CH) d3 = fma(vr[l + 3], col.a[c0 + l + 3], d3);
+        }
+        if (CH == NL) { /* whole reflector in registers: update without re-reading */
+          const double d = group_sum<SL>((d0 + d1) + (d2 + d3));
+          const double tw = tau * d;
+#pragma unroll
+          for (int l = 0; l < NL; ++l) col.a[l] = fma(-tw, vr[l], col.a[l]);
+          return;
+        }
+      }
+      const double d = group_sum<SL>((d0 + d1) + (d2 + d3));
       const double tw = tau * d;
-      __asm__ volatile("" ::: "memory"); /* re-read v from LDS instead of holding it in VGPRs */
 #pragma unroll
-      for (int l = 0; l < H; l += 2)
-        if (l >= c0 && l < c0e) {
-          col.a[l] = fma(-tw, vv[l], col.a[l]);
-          col.a[l + 1] = fma(-tw, vv[l + 1], col.a[l + 1]);
+      for (int c0 = 0; c0 < NL; c0 += CH) {
+        double vr[CH];
+#pragma unroll
+        for (int l = 0; l < CH; l += 2) {
+          const double2 t = v2[(c0 + l) / 2];
+          vr[l] = t.x;
+          vr[l + 1] = t.y;
         }
 #pragma unroll
-      for (int l = H; l < NB; l += 2)
-        if (l - H < c1e) {
-          col.a[l] = fma(-tw, vv[l], col.a[l]);
-          col.a[l + 1] = fma(-tw, vv[l + 1], col.a[l + 1]);
-        }
+        for (int l = 0; l < CH; ++l) col.a[c0 + l] = fma(-tw, vr[l], col.a[c0 + l]);
+      }
     }
   }
 }
 
-/* QR of the stacked matrix over npiv pivot columns piv0 .. piv0+npiv-1. */
-template <int NB>
-__device__ __forceinline__ void house_qr(Col<NB>& col, Refl<NB>& R, int mycol, int sub, int piv0, int npiv, bool upd,
-                                         int n0, int n1) {
-  for (int k = 0; k < npiv; ++k) {
-    house_step<NB>(col, R, k & 1, sub, mycol == piv0 + k, upd, k, n0, n1);
-  }
+/* Lanes per column and threads of a QR kernel over COLS matrix columns:
+ * 4 lanes per column unless that would exceed 640 threads (10 waves), whose
+ * VGPR budget would not hold the column plus a prefetched reflector. */
+template <int NB, int COLS>
+struct QRGeom {
+  static constexpr int SL = (NB <= 16 || 4 * COLS > 640) ? 2 : 4;
+  static constexpr int NT = ((SL * COLS + 63) / 64) * 64;
+};
+
+template <int NB, int SL>
+__device__ __forceinline__ void house_qr(Col<NB, SL>& col, Refl<NB, SL>& R, int mycol, int sub, int piv0, int n,
+                                         bool upd) {
+  for (int k = 0; k < n; ++k) house_step<NB, SL>(col, R, k & 1, sub, mycol == piv0 + k, upd, k);
   __syncthreads();
 }
 
-template <int NB>
-struct ElimShared {
-  Refl<NB> refl;
-  double stashD[NB * NB + NB]; /* D~_{j+1} columns (mr each) then c~_{j+1} */
-  double stashZ[NB * NB + NB]; /* Z_{L-1} columns then o_{L-1} (fix-last) */
-};
-
-/* QR tasks of one elimination work unit */
-enum { T_LONE1 = 0, T_LONE2 = 1, T_P1 = 2, T_P2 = 3, T_P3 = 4, T_FIX = 5, T_BASE = 6 };
+template <int NB, int SL, int NT>
+__device__ __forceinline__ void refl_clear(Refl<NB, SL>& R, int tid) {
+  for (int i = tid; i < (int)(sizeof(R) / sizeof(double)); i += NT) reinterpret_cast<double*>(&R)[i] = 0.0;
+  __syncthreads();
+}
 
 /*
- * Fused elimination of one level. Work unit u of problem p:
- *   L == 1      : the chain's single step (base case, :1360-1423)
- *   otherwise   : pair (j, j+1), j = 2u; the last unit of an odd-length
- *                 chain also eliminates the lone last step L-1 and folds it
- *                 into the pair's odd step (fix-last, :1452-1476).
- * Columns (lane pairs): P = [0,n) pivots of u_j; Q = [n,2n) u_{j+1};
- * R1 = 2n right-hand side; Bl = [2n+1, 3n+1) u_{j-1}.
- * Tasks: LONE1 = QR of [0; C_{L-1}], LONE2 = QR of [D_{L-1}; R~_{L-1}],
- *        P1 = QR of [B_{j+1}; C_j] (:379-457), P2 = QR of [D_j; R~_j]
- *        (:460-523), P3 = QR of [D~_{j+1}; C_{j+1}] (:526-560),
- *        FIX = QR of [G~_{L-2}; Z_{L-1}], BASE = QR of [0; C_0] when L == 1.
+ * Elimination of one level, three launches (the reference's three phases):
+ *
+ * k_phase1, one workgroup per even step j (:379-457). Pair (j, j+1): QR of
+ *   [B_{j+1}; C_j] applied to [D_{j+1}; 0] and [c_{j+1}; o_j]. A lone last
+ *   step (j = L-1) or a one-step chain: QR of [0; C_j] applied to [0; o_j].
+ *   R~_j, X_j, o_j go to the factor slot of j (R, Y, cf), D~_{j+1} and
+ *   c~_{j+1} to the next level's C and o slots of the new step.
+ * k_phase2, one workgroup per even step j > 0 (:460-523): QR of [D_j; R~_j]
+ *   applied to [B_j; 0], [c_j; o_j], [0; X_j]: final R, B~, Y, cf; the rest
+ *   rows Z_j, X~_j, o_j become the new step's B, D, c (a lone last step's Z
+ *   and o go to a per-problem scratch for the fix-last fold).
+ * k_phase3, one workgroup per pair (:526-560): QR of [D~_{j+1}; C_{j+1}]
+ *   applied to [c~_{j+1}; o_{j+1}]; in the last pair of an odd chain it then
+ *   folds the lone step's [Z_{L-1}, o_{L-1}] in (:1452-1476). G~, c give the
+ *   new step's C and o.
+ * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
+ * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
 template <int NB>
-__global__ void __launch_bounds__(((2 * (3 * NB + 1) + 63) / 64) * 64)
-    k_eliminate(uk_level_t lv, uk_chain_t in, uk_chain_t out, uk_factors_t fac, int n) {
-  __shared__ ElimShared<NB> sh;
-  constexpr int NT = ((2 * (3 * NB + 1) + 63) / 64) * 64;
-  const int tid = threadIdx.x;
-  const int sub = tid & 1;
-  const int c = tid >> 1;
-  const int64_t unit = blockIdx.x;
-  const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
-  const int64_t u = unit - lv.unit_start[p];
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
+    k_phase1(uk_level_t lv, uk_chain_t in, uk_chain_t out, uk_factors_t fac, int n) {
+  constexpr int SL = QRGeom<NB, 2 * NB + 1>::SL;
+  constexpr int NT = QRGeom<NB, 2 * NB + 1>::NT;
+  __shared__ Refl<NB, SL> refl;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t e = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t u = e - lv.even_start[p];
+  const int64_t j = 2 * u;
   const int64_t L = lv.L[p];
   const int64_t s0 = lv.step_start[p];
+  const int64_t t = lv.next_start[p] + u;
   const int mr = in.mr;
   const int64_t nn = (int64_t)n * n;
+  const bool lone = j + 1 == L;
+  const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
+  const int qc = isP ? c : (isQ ? c - n : 0);
+  refl_clear<NB, SL, NT>(refl, tid);
 
-  const bool isP = c < n;
-  const bool isQ = c >= n && c < 2 * n;
-  const bool isR1 = c == 2 * n;
-  const bool isBl = c > 2 * n && c <= 3 * n;
-  const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0)); /* column within its block */
+  Col<NB, SL> col;
+  const int64_t sj = s0 + j, sp = s0 + j + 1;
+  if (isP) {
+    if (lone) col.zero_block(0);
+    else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
+    col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
+  }
+  if (isQ) {
+    col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
+    col.zero_block(1);
+  }
+  if (isR1) {
+    if (lone) col.zero_block(0);
+    else col.load_block(0, in.c + sp * n, n, sub);
+    col.load_block(1, in.o + sj * n, mr, sub);
+  }
+  house_qr<NB, SL>(col, refl, c, sub, 0, n, isP || isQ || isR1);
+  if (isP) {
+    col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
+    for (int r = sub; r < n; r += SL) {
+      if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = 0.0;
+      if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = 0.0;
+    }
+  }
+  if (isQ) {
+    col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);     /* X_j */
+    col.store_block(1, out.C + t * nn + (int64_t)qc * mr, mr, sub);  /* D~_{j+1} */
+  }
+  if (isR1) {
+    col.store_block(0, fac.cf + e * n, n, sub);                       /* o_j */
+    if (!lone) col.store_block(1, out.o + t * n, mr, sub);           /* c~_{j+1} */
+  }
+}
 
-  for (int i = tid; i < (int)(sizeof(Refl<NB>) / sizeof(double)); i += NT)
-    reinterpret_cast<double*>(&sh.refl)[i] = 0.0;
-
-  int tasks[6];
-  int ntask = 0;
+template <int NB>
+__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
+    k_phase2(uk_level_t lv, uk_chain_t in, uk_chain_t out, uk_factors_t fac, double* __restrict__ fix, int n) {
+  constexpr int SL = QRGeom<NB, 3 * NB + 1>::SL;
+  constexpr int NT = QRGeom<NB, 3 * NB + 1>::NT;
+  __shared__ Refl<NB, SL> refl;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t e = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t u = e - lv.even_start[p];
   const int64_t j = 2 * u;
-  const bool do_fix = L > 1 && (L & 1) && (u == L / 2 - 1);
-  if (L == 1) {
-    tasks[ntask++] = T_BASE;
-  } else {
-    if (do_fix) { tasks[ntask++] = T_LONE1; tasks[ntask++] = T_LONE2; }
-    tasks[ntask++] = T_P1;
-    if (j != 0) tasks[ntask++] = T_P2; /* the chain's first step has no [D; R~] */
-    tasks[ntask++] = T_P3;
-    if (do_fix) tasks[ntask++] = T_FIX;
+  if (j == 0) return; /* the chain's first step has no [D; R~] (uniform per block) */
+  const int64_t L = lv.L[p];
+  const int64_t t = lv.next_start[p] + u;
+  const int64_t nn = (int64_t)n * n;
+  const bool lone = j + 1 == L;
+  const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
+  const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
+  refl_clear<NB, SL, NT>(refl, tid);
+
+  Col<NB, SL> col;
+  const int64_t sj = lv.step_start[p] + j;
+  if (isP) {
+    col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
+    col.load_block(1, fac.R + e * nn + (int64_t)qc * n, n, sub); /* R~_j */
   }
-  const int64_t jl = L - 1;
-  const int64_t t_next = lv.next_start[p] + u; /* new step for old step j+1 */
-  const int64_t e_pair = lv.even_start[p] + u;
-  const int64_t e_lone = lv.even_start[p] + jl / 2;
-
-  Col<NB> col;
-#pragma unroll
-  for (int l = 0; l < NB; ++l) col.a[l] = 0.0;
-
-  for (int ti = 0; ti < ntask; ++ti) {
-    const int task = tasks[ti];
-    /* ---- set up the stacked matrix ---- */
-    int piv0 = 0, n1 = n;
-    bool upd = false;
-    switch (task) {
-      case T_BASE:
-      case T_LONE1: {
-        const int64_t s = s0 + (task == T_BASE ? 0 : jl);
-        if (isP) { col.zero_block(0); col.load_block(1, in.C + s * nn + (int64_t)qc * mr, mr, sub); }
-        if (isR1) { col.zero_block(0); col.load_block(1, in.o + s * n, mr, sub); }
-        upd = isP || isR1;
-        n1 = mr;
-      } break;
-      case T_LONE2: {
-        const int64_t s = s0 + jl;
-        if (isP) { col.move_0_to_1(); col.load_block(0, in.D + s * nn + (int64_t)qc * n, n, sub); }
-        if (isR1) { col.move_0_to_1(); col.load_block(0, in.c + s * n, n, sub); }
-        if (isBl) { col.load_block(0, in.B + s * nn + (int64_t)qc * n, n, sub); col.zero_block(1); }
-        upd = isP || isR1 || isBl;
-      } break;
-      case T_P1: {
-        const int64_t sj = s0 + j, sp = s0 + j + 1;
-        if (isP) {
-          col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
-          col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
-        }
-        if (isQ) { col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub); col.zero_block(1); }
-        if (isR1) { col.load_block(0, in.c + sp * n, n, sub); col.load_block(1, in.o + sj * n, mr, sub); }
-        upd = isP || isQ || isR1;
-        n1 = mr;
-      } break;
-      case T_P2: {
-        const int64_t sj = s0 + j;
-        if (isP) { col.move_0_to_1(); col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub); }
-        if (isQ) { col.move_0_to_1(); col.zero_block(0); }
-        if (isR1) { col.move_0_to_1(); col.load_block(0, in.c + sj * n, n, sub); }
-        if (isBl) { col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub); col.zero_block(1); }
-        upd = isP || isQ || isR1 || isBl;
-      } break;
-      case T_P3: {
-        const int64_t sp = s0 + j + 1;
-        if (isQ) {
-          col.load_block(0, sh.stashD + qc * mr, mr, sub);
-          col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
-        }
-        if (isR1) {
-          col.load_block(0, sh.stashD + (int64_t)n * mr, mr, sub);
-          col.load_block(1, in.o + sp * n, mr, sub);
-        }
-        piv0 = n;
-        upd = isQ || isR1;
-        n1 = mr;
-      } break;
-      case T_FIX: {
-        if (isQ) col.load_block(1, sh.stashZ + qc * n, n, sub);
-        if (isR1) col.load_block(1, sh.stashZ + nn, n, sub);
-        piv0 = n;
-        upd = isQ || isR1;
-      } break;
-    }
-    __syncthreads();
-
-    /* ---- Householder QR over n pivot columns ---- */
-    for (int k = 0; k < n; ++k) house_step<NB>(col, sh.refl, k & 1, sub, c == piv0 + k, upd, k, n, n1);
-    __syncthreads();
-
-    /* ---- store results ---- */
-    switch (task) {
-      case T_BASE:
-        if (isP) {
-          col.store_triu(fac.R + e_pair * nn + (int64_t)qc * n, n, qc, sub);
-          for (int r = sub; r < n; r += 2) {
-            fac.Bt[e_pair * nn + (int64_t)qc * n + r] = 0.0;
-            fac.Y[e_pair * nn + (int64_t)qc * n + r] = 0.0;
-          }
-        }
-        if (isR1) col.store_block(0, fac.cf + e_pair * n, n, sub);
-        break;
-      case T_LONE2:
-        if (isP) {
-          col.store_triu(fac.R + e_lone * nn + (int64_t)qc * n, n, qc, sub);
-          for (int r = sub; r < n; r += 2) fac.Y[e_lone * nn + (int64_t)qc * n + r] = 0.0;
-        }
-        if (isBl) {
-          col.store_block(0, fac.Bt + e_lone * nn + (int64_t)qc * n, n, sub);
-          col.store_block(1, sh.stashZ + qc * n, n, sub); /* Z_{L-1} */
-        }
-        if (isR1) {
-          col.store_block(0, fac.cf + e_lone * n, n, sub);
-          col.store_block(1, sh.stashZ + nn, n, sub); /* o''_{L-1} */
-        }
-        break;
-      case T_P1:
-        if (isQ) col.store_block(1, sh.stashD + qc * mr, mr, sub);          /* D~_{j+1} */
-        if (isR1) col.store_block(1, sh.stashD + (int64_t)n * mr, mr, sub); /* c~_{j+1} */
-        if (j == 0) { /* the chain's first step: R_0 = R~_0, Y_0 = X_0, cf_0 = o_0 */
-          if (isP) {
-            col.store_triu(fac.R + e_pair * nn + (int64_t)qc * n, n, qc, sub);
-            for (int r = sub; r < n; r += 2) fac.Bt[e_pair * nn + (int64_t)qc * n + r] = 0.0;
-          }
-          if (isQ) col.store_block(0, fac.Y + e_pair * nn + (int64_t)qc * n, n, sub);
-          if (isR1) col.store_block(0, fac.cf + e_pair * n, n, sub);
-        }
-        break;
-      case T_P2: {
-          if (isP) col.store_triu(fac.R + e_pair * nn + (int64_t)qc * n, n, qc, sub);
-          if (isBl) {
-            col.store_block(0, fac.Bt + e_pair * nn + (int64_t)qc * n, n, sub);
-            col.store_block(1, out.B + t_next * nn + (int64_t)qc * n, n, sub); /* Z_j -> new B */
-          }
-          if (isQ) {
-            col.store_block(0, fac.Y + e_pair * nn + (int64_t)qc * n, n, sub);
-            col.store_block(1, out.D + t_next * nn + (int64_t)qc * n, n, sub); /* X~_j -> new D */
-          }
-          if (isR1) {
-            col.store_block(0, fac.cf + e_pair * n, n, sub);
-            col.store_block(1, out.c + t_next * n, n, sub); /* o_j -> new c */
-          }
-        }
-        break;
-      case T_P3:
-      case T_FIX:
-        if (task == T_FIX || !do_fix) {
-          if (isQ) col.store_triu(out.C + t_next * nn + (int64_t)qc * n, n, qc, sub); /* G~ -> new C */
-          if (isR1) col.store_block(0, out.o + t_next * n, n, sub);                    /* c_{j+1} -> new o */
-        }
-        break;
-      default:
-        break;
-    }
+  if (isQ) {
+    col.zero_block(0);
+    col.load_block(1, fac.Y + e * nn + (int64_t)qc * n, n, sub); /* X_j (zero for a lone step) */
   }
+  if (isR1) {
+    col.load_block(0, in.c + sj * n, n, sub);
+    col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
+  }
+  if (isBl) {
+    col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
+    col.zero_block(1);
+  }
+  house_qr<NB, SL>(col, refl, c, sub, 0, n, isP || isQ || isR1 || isBl);
+  double* fz = fix + (int64_t)p * (nn + n);
+  if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
+  if (isBl) {
+    col.store_block(0, fac.Bt + e * nn + (int64_t)qc * n, n, sub);
+    col.store_block(1, (lone ? fz : out.B + t * nn) + (int64_t)qc * n, n, sub); /* Z_j */
+  }
+  if (isQ) {
+    col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);
+    if (!lone) col.store_block(1, out.D + t * nn + (int64_t)qc * n, n, sub); /* X~_j */
+  }
+  if (isR1) {
+    col.store_block(0, fac.cf + e * n, n, sub);
+    col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
+  }
+}
+
+template <int NB>
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1>::NT))
+    k_phase3(uk_level_t lv, uk_chain_t in, uk_chain_t out, const double* __restrict__ fix, int n) {
+  constexpr int SL = QRGeom<NB, NB + 1>::SL;
+  constexpr int NT = QRGeom<NB, NB + 1>::NT;
+  __shared__ Refl<NB, SL> refl;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t unit = blockIdx.x;
+  const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
+  const int64_t L = lv.L[p];
+  if (L == 1) return; /* uniform per block */
+  const int64_t u = unit - lv.unit_start[p];
+  const int64_t j = 2 * u;
+  const int64_t t = lv.next_start[p] + u;
+  const int mr = in.mr;
+  const int64_t nn = (int64_t)n * n;
+  const bool do_fix = (L & 1) && (u == L / 2 - 1);
+  const bool isQ = c < n, isR1 = c == n; /* Q columns numbered from 0 here */
+  const int qc = isQ ? c : 0;
+  refl_clear<NB, SL, NT>(refl, tid);
+
+  Col<NB, SL> col;
+  const int64_t sp = lv.step_start[p] + j + 1;
+  if (isQ) {
+    col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
+    col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
+  }
+  if (isR1) {
+    col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
+    col.load_block(1, in.o + sp * n, mr, sub);
+  }
+  house_qr<NB, SL>(col, refl, c, sub, 0, n, isQ || isR1);
+  if (do_fix) {
+    const double* fz = fix + (int64_t)p * (nn + n);
+    if (isQ) col.load_block(1, fz + (int64_t)qc * n, n, sub);
+    if (isR1) col.load_block(1, fz + nn, n, sub);
+    house_qr<NB, SL>(col, refl, c, sub, 0, n, isQ || isR1);
+  }
+  if (isQ) col.store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
+  if (isR1) col.store_block(0, out.o + t * n, n, sub);
 }
 
 /* ------------------------------------------------------------------------ */
 /* back substitution                                                          */
 /* ------------------------------------------------------------------------ */
 
-/* x <- R^{-1} x for upper-triangular R (ld n) in LDS, rinv = 1/diag(R). */
+/* x <- R^{-1} x for upper-triangular R in LDS (ld NB, identity-padded beyond
+ * n, so padded entries of x stay zero), rinv = 1/diag(R). */
 template <int NB>
-__device__ __forceinline__ void back_solve(double (&x)[NB], const double* R, const double* rinv, int n) {
+__device__ __forceinline__ void back_solve(double (&x)[NB], const double* R, const double* rinv) {
 #pragma unroll
   for (int pp = NB - 1; pp >= 0; --pp) {
-    if (pp < n) {
-      const double xq = x[pp] * rinv[pp];
-      x[pp] = xq;
+    const double xq = x[pp] * rinv[pp];
+    x[pp] = xq;
 #pragma unroll
-      for (int i = 0; i < pp; ++i) x[i] = fma(-R[i + pp * n], xq, x[i]);
-    }
+    for (int i = 0; i < pp; ++i) x[i] = fma(-R[i + pp * NB], xq, x[i]);
   }
 }
 
 template <int NB>
 struct BackShared {
-  double R[NB * NB];
-  double T[2 * NB * NB]; /* Bt | Y, later W (n x 2n), later K (n x n) */
+  double R[NB * NB];     /* ld NB, identity beyond n */
+  double T[2 * NB * NB]; /* Bt | Y (ld NB, zero-padded), later W (n x 2n), later K */
   double rinv[NB];
   double xm[NB], xp[NB];
 };
@@ -452,6 +448,7 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
               double* __restrict__ XTcur, int n, int cov) {
   __shared__ BackShared<NB> sh;
   constexpr int NT = ((2 * NB + 63) / 64) * 64;
+  constexpr int NN = NB * NB;
   const int tid = threadIdx.x;
   const int64_t ev = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, ev);
@@ -465,23 +462,25 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
   const int64_t oj = ob + (j + 1) * stride - 1;
   const int64_t om = ob + j * stride - 1;
   const int64_t op = ob + (j + 2) * stride - 1;
-  const int64_t sj = lv.step_start[p] + j; /* this level's step index of j */
-  const int64_t sup = lv.next_start[p] + (j >> 1) - 1; /* level+1 index of u_{j-1}: pair (j-1, j+1) */
+  const int64_t sj = lv.step_start[p] + j;             /* this level's step index of j */
+  const int64_t sup = lv.next_start[p] + (j >> 1) - 1; /* level+1 index of u_{j-1} */
 
   const double* Rg = fac.R + ev * nn;
   const double* Bg = fac.Bt + ev * nn;
   const double* Yg = fac.Y + ev * nn;
-  for (int i = tid; i < n * n; i += NT) {
-    sh.R[i] = Rg[i];
-    sh.T[i] = has_m ? Bg[i] : 0.0;
-    sh.T[nn + i] = has_p ? Yg[i] : 0.0;
+  for (int idx = tid; idx < NN; idx += NT) {
+    const int i = idx % NB, c = idx / NB;
+    const bool in = i < n && c < n;
+    sh.R[idx] = in ? Rg[i + c * n] : (i == c ? 1.0 : 0.0);
+    sh.T[idx] = (in && has_m) ? Bg[i + c * n] : 0.0;
+    sh.T[NN + idx] = (in && has_p) ? Yg[i + c * n] : 0.0;
   }
-  for (int i = tid; i < n; i += NT) {
-    sh.xm[i] = has_m ? xout[om * n + i] : 0.0;
-    sh.xp[i] = has_p ? xout[op * n + i] : 0.0;
+  for (int i = tid; i < NB; i += NT) {
+    sh.xm[i] = (has_m && i < n) ? xout[om * n + i] : 0.0;
+    sh.xp[i] = (has_p && i < n) ? xout[op * n + i] : 0.0;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += NT) sh.rinv[i] = 1.0 / sh.R[i * n + i];
+  for (int i = tid; i < NB; i += NT) sh.rinv[i] = 1.0 / sh.R[i * NB + i];
   __syncthreads();
 
   /* ---- estimate (lanes 0..n-1 of wave 0) ---- */
@@ -489,18 +488,18 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
     double r = 0.0;
     if (tid < n) {
       r = fac.cf[ev * n + tid];
-      for (int q = 0; q < n; ++q) r -= sh.T[tid + q * n] * sh.xm[q] + sh.T[nn + tid + q * n] * sh.xp[q];
+      for (int q = 0; q < n; ++q) r -= sh.T[tid + q * NB] * sh.xm[q] + sh.T[NN + tid + q * NB] * sh.xp[q];
     }
     for (int q = n - 1; q >= 0; --q) {
       const double xq = __shfl(r, q) * sh.rinv[q];
       if (tid == q) r = xq;
-      else if (tid < q) r -= sh.R[tid + q * n] * xq;
+      else if (tid < q) r -= sh.R[tid + q * NB] * xq;
     }
     if (tid < n) xout[oj * n + tid] = r;
   }
   if (!cov) return;
 
-  /* ---- W = T Sigma: lane q < 2n owns column q (n rows) ---- */
+  /* ---- W = T Sigma: lane q < 2n owns column q (NB rows, zero beyond n) ---- */
   const int q = tid;
   const bool act = q < 2 * n;
   const bool left = q < n;
@@ -514,12 +513,11 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
       const double* Spm = XTup + sup * nn; /* Cov(u_{j+1}, u_{j-1}) */
       for (int pp = 0; pp < n; ++pp) {
         double sig;
-        if (left) sig = Smm[qq + (int64_t)pp * n];            /* S_mm[pp, q] (symmetric) */
-        else sig = has_p ? Spm[qq + (int64_t)pp * n] : 0.0;   /* S_mp[pp, qq] = S_pm[qq, pp] */
-        const double* tc = &sh.T[pp * n];
+        if (left) sig = Smm[qq + (int64_t)pp * n];          /* S_mm[pp, q] (symmetric) */
+        else sig = has_p ? Spm[qq + (int64_t)pp * n] : 0.0; /* S_mp[pp, qq] = S_pm[qq, pp] */
+        const double* tc = &sh.T[pp * NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-          if (i < n) v[i] = fma(tc[i], sig, v[i]);
+        for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sig, v[i]);
       }
     }
     if (has_p) { /* rows p >= n: S_pm (left) or S_pp (right) */
@@ -529,10 +527,9 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
         double sig;
         if (left) sig = has_m ? Smp[qq + (int64_t)pp * n] : 0.0; /* S_pm[pp, q] = S_mp[q, pp] */
         else sig = Spp[qq + (int64_t)pp * n];
-        const double* tc = &sh.T[nn + pp * n];
+        const double* tc = &sh.T[NN + pp * NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-          if (i < n) v[i] = fma(tc[i], sig, v[i]);
+        for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sig, v[i]);
       }
     }
   }
@@ -541,9 +538,8 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
   /* ---- W to LDS; U = R^{-1} W in place -> cross covariances ---- */
   if (act) {
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < n) sh.T[i + q * n] = v[i];
-    back_solve<NB>(v, sh.R, sh.rinv, n);
+    for (int i = 0; i < NB; ++i) sh.T[i + q * NB] = v[i];
+    back_solve<NB>(v, sh.R, sh.rinv);
     if (left && has_m) { /* Cov(u_j, u_{j-1}) = -U_1: XT[j-1] = -U_1, X[j-1] = -U_1^T */
       double* xt = XTcur + (sj - 1) * nn;
       double* x = Xcur + (sj - 1) * nn;
@@ -575,25 +571,23 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
       double tq;
       if (pp < n) tq = has_m ? Bg[q + (int64_t)pp * n] : 0.0; /* Bt[q, pp] */
       else tq = has_p ? Yg[q + (int64_t)(pp - n) * n] : 0.0; /* Y[q, pp-n] */
-      const double* wc = &sh.T[pp * n];
+      const double* wc = &sh.T[pp * NB];
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (i < n) v[i] = fma(wc[i], tq, v[i]);
+      for (int i = 0; i < NB; ++i) v[i] = fma(wc[i], tq, v[i]);
     }
-    back_solve<NB>(v, sh.R, sh.rinv, n);
+    back_solve<NB>(v, sh.R, sh.rinv);
   }
   __syncthreads(); /* W in LDS consumed */
   if (q < n) {
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < n) sh.T[i + q * n] = v[i];
+    for (int i = 0; i < NB; ++i) sh.T[i + q * NB] = v[i];
   }
   __syncthreads();
-  /* ---- S = R^{-1} K^T: lane q takes row q of K ---- */
+  /* ---- S = R^{-1} K^T: lane q takes row q of K (rows beyond n read as zero) ---- */
   if (q < n) {
 #pragma unroll
-    for (int i = 0; i < NB; ++i) v[i] = (i < n) ? sh.T[q + i * n] : 0.0;
-    back_solve<NB>(v, sh.R, sh.rinv, n);
+    for (int i = 0; i < NB; ++i) v[i] = (i < n) ? sh.T[q + i * NB] : 0.0;
+    back_solve<NB>(v, sh.R, sh.rinv);
     double* S = Sout + oj * nn;
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -780,10 +774,13 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
 }
 
 extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
-                                          const uk_factors_t* fac, int n, hipStream_t st) {
+                                          const uk_factors_t* fac, double* fix, int n, hipStream_t st) {
   const int nb = uk_bucket(n > in->mr ? n : in->mr);
-#define L_(NBV) \
-  k_eliminate<NBV><<<dim3((unsigned)lv->units), dim3(((2 * (3 * NBV + 1) + 63) / 64) * 64), 0, st>>>(*lv, *in, *out, *fac, n)
+#define L_(NBV)                                                                                                      \
+  k_phase1<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, n); \
+  k_phase2<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, fix, \
+                                                                                        n);                        \
+  k_phase3<NBV><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, *in, *out, fix, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;

@@ -26,7 +26,7 @@ extern "C" {
 int uk_bucket(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
-                               const uk_factors_t* fac, int n, hipStream_t st);
+                               const uk_factors_t* fac, double* fix, int n, hipStream_t st);
 hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
                              hipStream_t st);
@@ -112,6 +112,7 @@ struct ukhip_batch {
   double* XT[2]{};
   double* xout = nullptr;
   double* Sout = nullptr;
+  double* fix = nullptr; /* per problem: Z (n x n) and o (n) of a lone last step */
   int64_t total_evens = 0;
   /* associative-path workspace, allocated on first use */
   void* d_assoc = nullptr;
@@ -296,6 +297,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
     xto[cb] = take(cs * nn * 8);
   }
   const size_t xoo = take(S * n * 8), Soo = take(S * nn * 8);
+  const size_t fxo = take((size_t)b->nprob * (nn + n) * 8);
   b->work_bytes = o;
   if (!hip_ok(hipMalloc(&b->d_work, o), "hipMalloc(workspace)")) { batch_release(b); return nullptr; }
   char* w = (char*)b->d_work;
@@ -314,6 +316,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->fac.cf = (double*)(w + fc);
   b->xout = (double*)(w + xoo);
   b->Sout = (double*)(w + Soo);
+  b->fix = (double*)(w + fxo);
 
   /* level tables: one int64 array; prob indices stored widened */
   std::vector<int64_t> tab;
@@ -571,7 +574,7 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* 4 events or N
     f.Y += fb * nn;
     f.cf += fb * b->n;
     uk_level_t lv = b->levels[l].dev; /* even_start is relative to the level's fac_base */
-    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->n, st), "k_eliminate launch")) return false;
+    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, b->n, st), "elimination launch")) return false;
   }
   if (ev && !hip_ok(hipEventRecord(ev[2], st), "hipEventRecord")) return false;
   for (int l = nl - 1; l >= 0; --l) {
