@@ -508,28 +508,41 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
 #pragma unroll
   for (int i = 0; i < NB; ++i) v[i] = 0.0;
   if (act) {
+    /* Sigma's column q streams from global memory in batches of PF rows,
+     * all loads of a batch issued before its FMAs. */
+    constexpr int PF = 8;
     if (has_m) { /* rows p < n of Sigma: S_mm (left) or S_mp (right) */
-      const double* Smm = Sout + om * nn;
-      const double* Spm = XTup + sup * nn; /* Cov(u_{j+1}, u_{j-1}) */
-      for (int pp = 0; pp < n; ++pp) {
-        double sig;
-        if (left) sig = Smm[qq + (int64_t)pp * n];          /* S_mm[pp, q] (symmetric) */
-        else sig = has_p ? Spm[qq + (int64_t)pp * n] : 0.0; /* S_mp[pp, qq] = S_pm[qq, pp] */
-        const double* tc = &sh.T[pp * NB];
+      const double* src = left ? Sout + om * nn : XTup + sup * nn; /* S_mm, or S_pm = Cov(u_{j+1}, u_{j-1}) */
+      const bool live = left || has_p;
+      for (int p0 = 0; p0 < n; p0 += PF) {
+        double sg[PF];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sig, v[i]);
+        for (int t = 0; t < PF; ++t) sg[t] = (live && p0 + t < n) ? src[qq + (int64_t)(p0 + t) * n] : 0.0;
+#pragma unroll
+        for (int t = 0; t < PF; ++t) {
+          if (p0 + t < n) {
+            const double* tc = &sh.T[(p0 + t) * NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sg[t], v[i]);
+          }
+        }
       }
     }
     if (has_p) { /* rows p >= n: S_pm (left) or S_pp (right) */
-      const double* Spp = Sout + op * nn;
-      const double* Smp = Xup + sup * nn; /* Cov(u_{j-1}, u_{j+1}) */
-      for (int pp = 0; pp < n; ++pp) {
-        double sig;
-        if (left) sig = has_m ? Smp[qq + (int64_t)pp * n] : 0.0; /* S_pm[pp, q] = S_mp[q, pp] */
-        else sig = Spp[qq + (int64_t)pp * n];
-        const double* tc = &sh.T[NN + pp * NB];
+      const double* src = left ? Xup + sup * nn : Sout + op * nn; /* S_mp = Cov(u_{j-1}, u_{j+1}), or S_pp */
+      const bool live = !left || has_m;
+      for (int p0 = 0; p0 < n; p0 += PF) {
+        double sg[PF];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sig, v[i]);
+        for (int t = 0; t < PF; ++t) sg[t] = (live && p0 + t < n) ? src[qq + (int64_t)(p0 + t) * n] : 0.0;
+#pragma unroll
+        for (int t = 0; t < PF; ++t) {
+          if (p0 + t < n) {
+            const double* tc = &sh.T[NN + (p0 + t) * NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) v[i] = fma(tc[i], sg[t], v[i]);
+          }
+        }
       }
     }
   }
@@ -567,13 +580,24 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
   if (q < n) {
 #pragma unroll
     for (int i = 0; i < NB; ++i) v[i] = (i == q) ? 1.0 : 0.0;
-    for (int pp = 0; pp < 2 * n; ++pp) {
-      double tq;
-      if (pp < n) tq = has_m ? Bg[q + (int64_t)pp * n] : 0.0; /* Bt[q, pp] */
-      else tq = has_p ? Yg[q + (int64_t)(pp - n) * n] : 0.0; /* Y[q, pp-n] */
-      const double* wc = &sh.T[pp * NB];
+    constexpr int PF = 8;
+    for (int p0 = 0; p0 < 2 * n; p0 += PF) {
+      double tq[PF];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) v[i] = fma(wc[i], tq, v[i]);
+      for (int t = 0; t < PF; ++t) {
+        const int pp = p0 + t;
+        tq[t] = 0.0;
+        if (pp < n) tq[t] = has_m ? Bg[q + (int64_t)pp * n] : 0.0;            /* Bt[q, pp] */
+        else if (pp < 2 * n) tq[t] = has_p ? Yg[q + (int64_t)(pp - n) * n] : 0.0; /* Y[q, pp-n] */
+      }
+#pragma unroll
+      for (int t = 0; t < PF; ++t) {
+        if (p0 + t < 2 * n) {
+          const double* wc = &sh.T[(p0 + t) * NB];
+#pragma unroll
+          for (int i = 0; i < NB; ++i) v[i] = fma(wc[i], tq[t], v[i]);
+        }
+      }
     }
     back_solve<NB>(v, sh.R, sh.rinv);
   }

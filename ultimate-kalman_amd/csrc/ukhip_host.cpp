@@ -682,6 +682,8 @@ extern "C" void uk_batch_set_assoc_ws(ukhip_batch* b, void* p, size_t bytes) {
   b->d_assoc = p;
   b->assoc_bytes = bytes;
 }
+/* the level-0 chain buffers, which hold the whitened steps of the batch */
+extern "C" void uk_batch_chain0(ukhip_batch* b, uk_chain_t* ch) { *ch = b->chain[0]; }
 
 /* ------------------------------------------------------------------------ */
 /* the reference's smoother entry points (c/kalman.h:148-149)                */
