@@ -54,11 +54,60 @@ def make_problem(name, k=None):
     return P.iid_problem(k, 64, 32, seed=14, F=F, G_scale=1 / 8.0, Kvar=1e-2, Cvar=1.0, name=name)
 
 
+def bucket(n):
+    """Block-size bucket of the kernels' template instance for state dimension n."""
+    return next(b for b in (8, 16, 32, 48, 64) if n <= b)
+
+
+def traffic_of(kernel, workload):
+    """HBM bytes per step of `kernel` on `workload` from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE runs, calibrated with a copy kernel),
+    or None when no summary covers it."""
+    if workload is None:
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get(workload, {}).get(kernel)
+    if not e:
+        return None
+    return {"bytes_per_step": e["bytes_per_step"], "fetch_bytes": e["fetch_bytes"], "write_bytes": e["write_bytes"],
+            "algorithmic_bytes_per_step": e.get("algorithmic_bytes_per_step"), "source": "profiles/traffic.json"}
+
+
+def copy_calibration(mib):
+    """One device-to-device copy (the HIP runtime's copy kernel) of `mib` MiB
+    whose source was last touched before a 1 GiB fill, so that it is read
+    from HBM and not from a cache."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    size = mib * 1048576
+    bufs = [C.c_void_p() for _ in range(3)]
+    sizes = (size, 1 << 30, size)
+    for p, sz in zip(bufs, sizes):
+        if hip.hipMalloc(C.byref(p), sz) != 0:
+            raise RuntimeError("hipMalloc failed in the copy calibration")
+    hip.hipMemset(bufs[0], 1, size)
+    hip.hipMemset(bufs[1], 2, 1 << 30)
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(bufs[2], bufs[0], size, 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed in the copy calibration")
+    hip.hipDeviceSynchronize()
+    for p in bufs:
+        hip.hipFree(p)
+    print(f"[bench] copy calibration: {mib} MiB = {size} bytes copied", file=sys.stderr, flush=True)
+
+
 def m_list_of(pr):
     return [0 if o is None else o.shape[0] for o in pr.o]
 
 
-def cpu_baseline(name, sample_k):
+def cpu_baseline(name, sample_k, algorithm="oddeven"):
     """The reference itself (oracle/_ref: its C smoothers, sequential backend,
     one core) on a bounded sample of the workload; the oracle port if the
     reference build is absent."""
@@ -69,18 +118,25 @@ def cpu_baseline(name, sample_k):
     ref = uk_libs.reference()
     kind = "reference"
     lib = ref
+    m0 = 0 if pr.o[0] is None else pr.o[0].shape[0]
+    if algorithm == "associative" and m0 < pr.n[0]:
+        ref = None  # the reference's associative prior needs m_0 >= n (it asserts otherwise)
     if ref is None:
         lib = uk_libs.oracle()
         lib.lib.oracle_set_threads(1)
         kind = "port"
     eqs = capi.EquationSet(pr)
+    fn = lib.oe if algorithm == "oddeven" else lib.asc
     t0 = time.perf_counter()
-    lib.oe(0, eqs.ptrs, pr.k)
+    fn(0, eqs.ptrs, pr.k)
     dt = time.perf_counter() - t0
     eqs.collect(lib.free)
+    why = " (a power of two: the reference odd-even aborts on other lengths)" if algorithm == "oddeven" else ""
+    if kind == "port" and algorithm == "associative" and m0 < pr.n[0]:
+        why += f"; the oracle's restatement, since the reference's associative smoother asserts when m_0={m0} < n"
     return {"value": pr.k / dt, "unit": "timesteps/s", "cores": 1, "kind": kind,
-            "sample": f"{WORKLOADS[name]['desc'].split(' k=')[0]} k={sample_k} (a power of two: the reference "
-                      f"odd-even aborts on other lengths), kalman_smooth_oddeven with covariances, {dt:.1f} s"}
+            "sample": f"{WORKLOADS[name]['desc'].split(' k=')[0]} k={sample_k}{why}, "
+                      f"kalman_smooth_{algorithm} with covariances, {dt:.1f} s"}
 
 
 def cpu_port_allcores(name, sample_k):
@@ -112,6 +168,12 @@ def main():
     ap.add_argument("--k", type=int, default=None, help="override the number of timesteps (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-k", type=int, default=8192)
+    ap.add_argument("--no-associative", action="store_true", help="skip the associative smoother's timing")
+    ap.add_argument("--assoc-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample-k-assoc", type=int, default=2048)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the timed region, copy MIB MiB device-to-device once (a plain copy kernel of "
+                         "known size for calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); --gpus {args.gpus} is not supported",
@@ -127,24 +189,55 @@ def main():
     for i in range(args.warmup):
         b.smooth("oddeven")
     print("[bench] warmup done", file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
-    phase = np.zeros(4)
+    # Timed region: K complete odd-even smoothings, inputs resident in HBM. Each
+    # is enqueued with HIP events around every launch (library stream), so the
+    # same K runs give the wall-clock figure and the per-kernel durations.
+    kern = {name: 0.0 for name in device.OE_KERNELS}
+    launches = None
     device.lib().ukhip_synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        phase += np.array(b.smooth_timed(0))  # HIP events on the library stream around each kernel class
+        ms, launches = b.smooth_timed(0)
+        for name in kern:
+            kern[name] += ms[name] / args.steps
     device.lib().ukhip_synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    ms_weigh, ms_elim, ms_back, ms_total = phase / args.steps
-    print(f"[bench] {ms_per_step:.2f} ms/step (events: weigh {ms_weigh:.2f}, eliminate {ms_elim:.2f}, "
-          f"back {ms_back:.2f}, total {ms_total:.2f})", file=sys.stderr, flush=True)
+    print(f"[bench] {ms_per_step:.2f} ms/step; events: " +
+          ", ".join(f"{k} {v:.3f}" for k, v in kern.items()), file=sys.stderr, flush=True)
 
     n, ml = pr.n[0], m_list_of(pr)
     fl = flops.oddeven_flops(n, ml, pr.K_type, pr.C_type, cov=True)
-    cls = "eliminate" if ms_elim >= ms_back else "backsub"
-    cls_ms = ms_elim if cls == "eliminate" else ms_back
-    achieved = fl[cls] / (cls_ms * 1e-3) / 1e12
+    fl_of = {"k_weigh": fl["weigh"], "k_phase1": fl["phase1"], "k_phase2": fl["phase2"],
+             "k_phase3": fl["phase3"], "k_backsub": fl["backsub"]}
+    dom = max(fl_of, key=lambda k: kern[k])
+    nb = bucket(n)
+    achieved = fl_of[dom] / (kern[dom] * 1e-3) / 1e12
+    traffic = traffic_of(f"{dom}<{nb}>", args.workload if args.k is None else None)
+    if traffic is not None and dom == "k_backsub":
+        traffic["algorithmic_bytes_per_step"] = flops.oddeven_backsub_bytes(n, pr.k)
+
+    assoc = None
+    if not args.no_associative:
+        for i in range(max(1, args.warmup // 2)):
+            b.smooth("associative")
+        device.lib().ukhip_synchronize()
+        ta = time.perf_counter()
+        for i in range(args.assoc_steps):
+            b.smooth("associative", sync=False)
+        device.lib().ukhip_synchronize()
+        ams = (time.perf_counter() - ta) * 1e3 / args.assoc_steps
+        assoc = {"algorithm": "associative scan (Saerkkae-Garcia-Fernandez), FP64, with covariances",
+                 "ms_per_step": ams, "value": pr.k / (ams * 1e-3), "unit": "timesteps/s", "steps": args.assoc_steps}
+        if not args.no_cpu_baseline:
+            try:
+                assoc["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_k_assoc, "associative")
+            except Exception as e:  # the GPU figure stands without it
+                assoc["cpu_baseline"] = {"error": str(e)}
+        print(f"[bench] associative {ams:.2f} ms/step", file=sys.stderr, flush=True)
     b.free()
 
     out = {
@@ -164,14 +257,19 @@ def main():
                    f"{WORKLOADS[args.workload]['desc']} (k overridden to {pr.k})",
                    "algorithm": "odd-even (Gargir-Toledo), FP64, with covariances", "problems": 1,
                    "timesteps": pr.k},
-        "roofline": {"bound": "mfma", "kernel": f"k_{cls} (all levels)",
+        "roofline": {"bound": "mfma", "kernel": f"{dom}<{nb}>",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "algorithmic_flops": fl[cls], "kernel_ms": cls_ms},
-        "phases_ms": {"weigh": ms_weigh, "eliminate": ms_elim, "backsub": ms_back, "events_total": ms_total},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "compute": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                     "algorithmic_flops_per_step": fl_of[dom], "launches_per_step": launches[dom],
+                     "kernel_ms_per_step": kern[dom], "avg_launch_ms": kern[dom] / launches[dom]},
+        "kernels_ms_per_step": {k: v for k, v in kern.items()},
+        "kernels_tflops": {k: fl_of[k] / (kern[k] * 1e-3) / 1e12 for k in fl_of if kern[k] > 0},
         "algorithmic_flops_per_timestep": fl["total"] / pr.k,
-        "whole_path_tflops": fl["total"] / (ms_total * 1e-3) / 1e12,
+        "whole_path_tflops": fl["total"] / (kern["total"] * 1e-3) / 1e12,
     }
+    if assoc:
+        out["associative"] = assoc
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_k)

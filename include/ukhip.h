@@ -61,10 +61,16 @@ int ukhip_synchronize(void);
 /* Copies problem p's estimates (k*n) and covariances (k*n*n, may be NULL) to the host. */
 int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covariances);
 
-/* Timed odd-even run: per-phase device milliseconds measured with HIP events on
- * the library stream (weigh, elimination, back substitution) and the total. */
-int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double* ms_weigh, double* ms_elim,
-                                     double* ms_back, double* ms_total);
+/* Timed odd-even run: device milliseconds per kernel, measured with HIP events
+ * recorded on the library stream around every launch and summed over the
+ * levels. ms[UKHIP_OE_WEIGH .. UKHIP_OE_BACKSUB] are the kernels k_weigh,
+ * k_phase1, k_phase2, k_phase3 and k_backsub; ms[UKHIP_OE_TOTAL] is the span
+ * from the first launch to the end of the last. launches[] (may be NULL)
+ * receives the launch count of each kernel. */
+enum { UKHIP_OE_WEIGH = 0, UKHIP_OE_PHASE1, UKHIP_OE_PHASE2, UKHIP_OE_PHASE3, UKHIP_OE_BACKSUB, UKHIP_OE_TOTAL,
+       UKHIP_OE_SLOTS };
+int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
+                                     int32_t launches[UKHIP_OE_SLOTS]);
 
 /* The HIP stream (hipStream_t) every libukhip launch uses. */
 void* ukhip_stream(void);

@@ -797,14 +797,23 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
   return hipGetLastError();
 }
 
+/* ev (NULL or 4 events) is recorded before phase 1 and after each phase. */
 extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
-                                          const uk_factors_t* fac, double* fix, int n, hipStream_t st) {
+                                          const uk_factors_t* fac, double* fix, int n, hipStream_t st,
+                                          hipEvent_t* ev) {
   const int nb = uk_bucket(n > in->mr ? n : in->mr);
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
 #define L_(NBV)                                                                                                      \
+  mark(0);                                                                                                           \
   k_phase1<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, n); \
+  mark(1);                                                                                                           \
   k_phase2<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, fix, \
                                                                                         n);                        \
-  k_phase3<NBV><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, *in, *out, fix, n)
+  mark(2);                                                                                                           \
+  k_phase3<NBV><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, *in, *out, fix, n);      \
+  mark(3)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;

@@ -26,7 +26,7 @@ extern "C" {
 int uk_bucket(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
-                               const uk_factors_t* fac, double* fix, int n, hipStream_t st);
+                               const uk_factors_t* fac, double* fix, int n, hipStream_t st, hipEvent_t* ev);
 hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
                              hipStream_t st);
@@ -555,12 +555,17 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
 /* running                                                                    */
 /* ------------------------------------------------------------------------ */
 
-static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* 4 events or NULL */) {
+/* Events of a timed run: [0,1] around k_weigh, then per level l four around
+ * the elimination phases at 2 + 4 l and two around the back substitution at
+ * 2 + 4 L + 2 l. */
+static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
+
+static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */) {
   hipStream_t st = g_stream;
+  const int nl = (int)b->levels.size();
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
   if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
   if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
-  const int nl = (int)b->levels.size();
   for (int l = 0; l < nl; ++l) {
     uk_chain_t in = b->chain[l & 1];
     in.mr = l == 0 ? b->mr : b->n;
@@ -574,9 +579,10 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* 4 events or N
     f.Y += fb * nn;
     f.cf += fb * b->n;
     uk_level_t lv = b->levels[l].dev; /* even_start is relative to the level's fac_base */
-    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, b->n, st), "elimination launch")) return false;
+    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, b->n, st, ev ? ev + 2 + 4 * l : nullptr),
+                "elimination launch"))
+      return false;
   }
-  if (ev && !hip_ok(hipEventRecord(ev[2], st), "hipEventRecord")) return false;
   for (int l = nl - 1; l >= 0; --l) {
     uk_factors_t f = b->fac;
     const int64_t fb = b->levels[l].fac_base;
@@ -586,12 +592,14 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* 4 events or N
     f.Y += fb * nn;
     f.cf += fb * b->n;
     uk_level_t lv = b->levels[l].dev;
+    hipEvent_t* eb = ev ? ev + 2 + 4 * nl + 2 * l : nullptr;
+    if (eb && !hip_ok(hipEventRecord(eb[0], st), "hipEventRecord")) return false;
     if (!hip_ok(uk_launch_backsub(&lv, &f, b->xout, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], b->X[l & 1],
                                   b->XT[l & 1], b->n, cov, st),
                 "k_backsub launch"))
       return false;
+    if (eb && !hip_ok(hipEventRecord(eb[1], st), "hipEventRecord")) return false;
   }
-  if (ev && !hip_ok(hipEventRecord(ev[3], st), "hipEventRecord")) return false;
   return true;
 }
 
@@ -631,27 +639,43 @@ extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options
   return uk_assoc_smooth(b, options);
 }
 
-extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double* ms_weigh,
-                                                double* ms_elim, double* ms_back, double* ms_total) {
-  if (!b) { uk_set_error("NULL batch"); return -1; }
+extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
+                                                int32_t launches[UKHIP_OE_SLOTS]) {
+  if (!b || !ms) { uk_set_error("NULL batch or result array"); return -1; }
   if (!ensure_stream()) return -1;
-  hipEvent_t ev[4];
+  const size_t nl = b->levels.size();
+  std::vector<hipEvent_t> ev(timed_events(nl), nullptr);
+  bool ok = true;
   for (auto& e : ev)
-    if (!hip_ok(hipEventCreate(&e), "hipEventCreate")) return -1;
-  bool ok = run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev) &&
-            hip_ok(hipEventSynchronize(ev[3]), "hipEventSynchronize");
+    if (ok && !hip_ok(hipEventCreate(&e), "hipEventCreate")) ok = false;
+  ok = ok && run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev.data()) &&
+       hip_ok(hipEventSynchronize(ev[2 + 4 * nl + 1]), "hipEventSynchronize");
   if (ok) {
-    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-    (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
-    (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
-    (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
-    (void)hipEventElapsedTime(&t03, ev[0], ev[3]);
-    if (ms_weigh) *ms_weigh = t01;
-    if (ms_elim) *ms_elim = t12;
-    if (ms_back) *ms_back = t23;
-    if (ms_total) *ms_total = t03;
+    auto span = [&](size_t a, size_t c) {
+      float t = 0;
+      (void)hipEventElapsedTime(&t, ev[a], ev[c]);
+      return (double)t;
+    };
+    for (int i = 0; i < UKHIP_OE_SLOTS; ++i) ms[i] = 0.0;
+    ms[UKHIP_OE_WEIGH] = span(0, 1);
+    for (size_t l = 0; l < nl; ++l) {
+      const size_t e = 2 + 4 * l;
+      ms[UKHIP_OE_PHASE1] += span(e, e + 1);
+      ms[UKHIP_OE_PHASE2] += span(e + 1, e + 2);
+      ms[UKHIP_OE_PHASE3] += span(e + 2, e + 3);
+      const size_t eb = 2 + 4 * nl + 2 * l;
+      ms[UKHIP_OE_BACKSUB] += span(eb, eb + 1);
+    }
+    ms[UKHIP_OE_TOTAL] = span(0, 2 + 4 * nl + 1); /* the back substitution of level 0 ends last */
+    if (launches) {
+      launches[UKHIP_OE_WEIGH] = 1;
+      launches[UKHIP_OE_PHASE1] = launches[UKHIP_OE_PHASE2] = launches[UKHIP_OE_PHASE3] = (int32_t)nl;
+      launches[UKHIP_OE_BACKSUB] = (int32_t)nl;
+      launches[UKHIP_OE_TOTAL] = 1 + 4 * (int32_t)nl;
+    }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   return ok ? 0 : -1;
 }
 

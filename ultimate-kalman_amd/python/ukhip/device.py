@@ -25,6 +25,9 @@ class ProblemStruct(C.Structure):
 
 _lib = None
 
+# slots of ukhip_batch_smooth_oddeven_timed (include/ukhip.h)
+OE_KERNELS = ("k_weigh", "k_phase1", "k_phase2", "k_phase3", "k_backsub", "total")
+
 
 def lib():
     global _lib
@@ -43,7 +46,8 @@ def lib():
         for f in ("ukhip_batch_smooth_oddeven", "ukhip_batch_smooth_associative"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_uint32]
             getattr(L, f).restype = C.c_int
-        L.ukhip_batch_smooth_oddeven_timed.argtypes = [C.c_void_p, C.c_uint32] + [C.POINTER(C.c_double)] * 4
+        L.ukhip_batch_smooth_oddeven_timed.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
+                                                       C.POINTER(C.c_int32)]
         L.ukhip_batch_smooth_oddeven_timed.restype = C.c_int
         L.ukhip_batch_download.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ukhip_batch_download.restype = C.c_int
@@ -141,9 +145,13 @@ class Batch:
             _check(L.ukhip_synchronize(), "synchronize")
 
     def smooth_timed(self, options=0):
-        t = [C.c_double() for _ in range(4)]
-        _check(lib().ukhip_batch_smooth_oddeven_timed(self.h, options, *[C.byref(x) for x in t]), "smooth_timed")
-        return [x.value for x in t]
+        """Odd-even smoothing timed with HIP events around every launch.
+        Returns ({kernel: ms}, {kernel: launches}) for k_weigh, k_phase1,
+        k_phase2, k_phase3, k_backsub and the whole span ("total")."""
+        ms = (C.c_double * len(OE_KERNELS))()
+        nl = (C.c_int32 * len(OE_KERNELS))()
+        _check(lib().ukhip_batch_smooth_oddeven_timed(self.h, options, ms, nl), "smooth_timed")
+        return dict(zip(OE_KERNELS, list(ms))), dict(zip(OE_KERNELS, list(nl)))
 
     def download(self, i, want_cov=True):
         p = self.problems[i]

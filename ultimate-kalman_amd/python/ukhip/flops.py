@@ -10,8 +10,9 @@ block *shapes* only (no arithmetic) and sums
     dtrtrs(N,P)      P N^2
     dpotrf(N)        N^3 / 3
     dgemm(M,N,K)     2MNK
-split into the whitening, elimination (phases 1-3, fix-last, base case) and
-back-substitution parts. It is the definition of "algorithmic FLOPs" used for
+split into the whitening, the three elimination phases (phase 1 with the
+single-step base case, phase 3 with the fix-last step of an odd level, as the
+kernels k_phase1 and k_phase3 do them) and the back substitution. It is the definition of "algorithmic FLOPs" used for
 the roofline: the GPU's own operation count (padding, redundant work) is not
 credited.
 """
@@ -49,11 +50,12 @@ def _weigh(kind, dim, cols):
 def oddeven_flops(n, m_list, K_types=None, C_types=None, cov=True):
     """FLOPs of kalman_smooth_oddeven on one problem with constant state
     dimension n (H square), per-step observation counts m_list (0 =
-    unobserved) and covariance types. Returns dict(weigh, eliminate, backsub, total)."""
+    unobserved) and covariance types. Returns dict(weigh, phase1, phase2, phase3,
+    eliminate = phase1 + phase2 + phase3, backsub, total)."""
     k = len(m_list)
     K_types = K_types or ["w"] * k
     C_types = C_types or ["w"] * k
-    f = {"weigh": 0.0, "eliminate": 0.0, "backsub": 0.0}
+    f = {"weigh": 0.0, "phase1": 0.0, "phase2": 0.0, "phase3": 0.0, "backsub": 0.0}
     # steps_weigh (c/kalman_oddeven_smoother.c:1620-1659)
     for i in range(k):
         if i > 0:
@@ -63,13 +65,14 @@ def oddeven_flops(n, m_list, K_types=None, C_types=None, cov=True):
     # per-step row counts of the chain: l (B, D, c rows), m (C, o rows)
     steps = [{"l": (n if i > 0 else 0), "m": m_list[i]} for i in range(k)]
     _level(steps, n, cov, f)
+    f["eliminate"] = f["phase1"] + f["phase2"] + f["phase3"]
     f["total"] = f["weigh"] + f["eliminate"] + f["backsub"]
     return f
 
 
 def _level(s, n, cov, f):
     L = len(s)
-    E = "eliminate"
+    E = "phase1"
     if L == 1:  # :1360-1423
         m = s[0]["m"]
         f[E] += _qr(m, n) + _ormqr(m, min(m, n), 1)
@@ -97,6 +100,7 @@ def _level(s, n, cov, f):
         sp["dt"] = rows - ll
         sp["c_rows"] = rows - ll
     # phase 2 (:460-523)
+    E = "phase2"
     for j in range(0, L, 2):
         si = s[j]
         if j == 0:
@@ -114,6 +118,7 @@ def _level(s, n, cov, f):
             si["y_rows"] = mm
             si["xt_rows"] = rows - mm
     # phase 3 (:526-560)
+    E = "phase3"
     for j in range(0, L - 1, 2):
         sp = s[j + 1]
         rows = sp["dt"] + sp["m"]
@@ -174,3 +179,28 @@ def oddeven_bytes(n, m_list):
     k = len(m_list)
     inputs = sum((n * n + n if i > 0 else 0) + m * n + m for i, m in enumerate(m_list))
     return 8.0 * (inputs + k * (n + n * n))
+
+
+def oddeven_backsub_bytes(n, k, cov=True):
+    """Algorithmic HBM bytes of the back substitution (all levels) of one
+    problem of k steps in FP64: per even step j of a level, it reads R_j, cf_j,
+    B~_j (j > 0) and Y_j (j < L-1), the neighbours' estimates, covariances and
+    (both neighbours present) their cross-covariance; it writes x_j, S_j and
+    the cross-covariances with its neighbours. Odd steps are written by the
+    level above, so each output is counted once."""
+    nn = n * n
+    total = 0
+    L = k
+    while L >= 1:
+        for j in range(0, L, 2):
+            nb = (j > 0) + (j < L - 1)
+            r = nn + n + nb * nn + nb * n
+            w = n
+            if cov:
+                r += nb * nn + (nn if nb == 2 else 0)
+                w += nn + nb * nn
+            total += r + w
+        if L == 1:
+            break
+        L //= 2
+    return 8.0 * total
