@@ -40,10 +40,13 @@ typedef struct {
   const int32_t* m_i; /* k observation counts (0 = unobserved), or NULL for m on every step */
   const double* C;    /* k * m*m (ld m), or k * m when C_type == 'w' */
   char C_type;
+  int32_t F_shared;   /* 1: F is one n*n matrix used on every step (time-invariant dynamics) */
 } ukhip_problem_t;
 
-/* A set of independent problems with one state dimension, resident on the GPU.
- * Create uploads the inputs (host -> HBM); smooth runs entirely on the device. */
+/* A set of independent problems, resident on the GPU. Create uploads the
+ * inputs (host -> HBM); smooth runs entirely on the device. Problems of one
+ * state dimension share every launch of a level; a batch with several state
+ * dimensions keeps one part per dimension, and the parts run concurrently. */
 typedef struct ukhip_batch ukhip_batch_t;
 
 ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems);
@@ -52,8 +55,13 @@ int64_t ukhip_batch_steps(const ukhip_batch_t* b);
 int32_t ukhip_batch_levels(const ukhip_batch_t* b);
 
 /* Odd-even smoothing of every problem of the batch (options: KALMAN_NO_COVARIANCE
- * honoured). Enqueued on the library stream; returns 0 or -1 (see ukhip_last_error). */
+ * honoured). Enqueued on the library stream; returns 0 or -1 (see ukhip_last_error).
+ * The first call per covariance setting captures the launch sequence into a
+ * HIP graph, later calls replay it (see ukhip_use_graphs). */
 int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options);
+/* 0: ukhip_batch_smooth_oddeven launches kernel by kernel instead of replaying
+ * a captured graph (default 1). */
+void ukhip_use_graphs(int enable);
 /* Associative-scan smoothing of every problem of the batch. */
 int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options);
 int ukhip_synchronize(void);

@@ -5,7 +5,10 @@ device-resident batch API.
 
 Bars: against the oracle's associative smoother (same formulas, other
 rounding order) per-step normwise relative error <= 1e-9 on estimates and
-covariances; against the reference's stored outputs the oracle's own bar,
+covariances, or, on problems where the associative algorithm itself departs
+from least squares, checks.assoc_parity (no further from the QR-based
+odd-even result than the oracle's own associative smoother);
+against the reference's stored outputs the oracle's own bar,
 1e-7 (the reference's covariance-form combines depart from the global
 least-squares solution by up to ~1e-8 on some random problems). The last
 step's covariance is the filtered one, which the reference does not return.
@@ -17,7 +20,7 @@ import sys
 import numpy as np
 import pytest
 
-from checks import max_rel_err, rel_err
+from checks import assoc_parity, max_rel_err, rel_err
 from ukhip import capi
 from ukhip import problems as P
 
@@ -87,9 +90,9 @@ def _compare_batch(dev, oracle, probs, tol=TOL):
         for i, pr in enumerate(probs):
             x, cv = b.download(i)
             ox, oc = oracle.associative(pr)
-            ex, ec = max_rel_err(list(x), ox), max_rel_err(list(cv), oc)
-            say(f"  problem {i} ({pr.name}, k={pr.k}): state err {ex:.2e} cov err {ec:.2e}")
-            assert ex < tol and ec < tol, pr.name
+            ok, what = assoc_parity((list(x), list(cv)), (ox, oc), oracle.oddeven(pr), tol=tol)
+            say(f"  problem {i} ({pr.name}, k={pr.k}): {what}")
+            assert ok, pr.name
     finally:
         oracle.lib.oracle_set_threads(1)
         b.free()

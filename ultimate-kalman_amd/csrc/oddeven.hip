@@ -439,7 +439,8 @@ struct BackShared {
  *   S_j = R^{-1} (I + W T^T) R^{-T}.
  * (:614-975; the same quantities as the reference's S, S_off, S1_off.)
  * Cross covariances of a level are kept as X[s] = Cov(u_s, u_{s+1}) and
- * XT[s] = Cov(u_{s+1}, u_s), column-major n x n.
+ * XT[s] = Cov(u_{s+1}, u_s), column-major n x n; Xcur = NULL (level 0, which
+ * no level below reads) skips writing them.
  */
 template <int NB>
 __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
@@ -553,7 +554,7 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
 #pragma unroll
     for (int i = 0; i < NB; ++i) sh.T[i + q * NB] = v[i];
     back_solve<NB>(v, sh.R, sh.rinv);
-    if (left && has_m) { /* Cov(u_j, u_{j-1}) = -U_1: XT[j-1] = -U_1, X[j-1] = -U_1^T */
+    if (Xcur && left && has_m) { /* Cov(u_j, u_{j-1}) = -U_1: XT[j-1] = -U_1, X[j-1] = -U_1^T */
       double* xt = XTcur + (sj - 1) * nn;
       double* x = Xcur + (sj - 1) * nn;
 #pragma unroll
@@ -563,7 +564,7 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
           x[qq + (int64_t)i * n] = -v[i];
         }
     }
-    if (!left && has_p) { /* Cov(u_j, u_{j+1}) = -U_2: X[j] = -U_2, XT[j] = -U_2^T */
+    if (Xcur && !left && has_p) { /* Cov(u_j, u_{j+1}) = -U_2: X[j] = -U_2, XT[j] = -U_2^T */
       double* x = Xcur + sj * nn;
       double* xt = XTcur + sj * nn;
 #pragma unroll

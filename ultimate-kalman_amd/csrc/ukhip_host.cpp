@@ -117,6 +117,14 @@ struct ukhip_batch {
   /* associative-path workspace, allocated on first use */
   void* d_assoc = nullptr;
   size_t assoc_bytes = 0;
+  /* A batch of problems with different state dimensions is a composite: one
+   * part per dimension (each a batch of its own); the composite itself then
+   * holds no device arrays. where[p] = (part, index in part). */
+  std::vector<ukhip_batch*> parts;
+  std::vector<std::pair<int32_t, int32_t>> where;
+  std::vector<hipStream_t> part_streams;
+  /* captured odd-even smoothing, [0] without and [1] with covariances */
+  hipGraphExec_t graph[2]{};
 };
 
 static int cov_code(char t) {
@@ -220,6 +228,10 @@ static void plan_levels(ukhip_batch* b) {
 
 static void batch_release(ukhip_batch* b) {
   if (!b) return;
+  for (auto* q : b->parts) batch_release(q);
+  for (auto st : b->part_streams) (void)hipStreamDestroy(st);
+  for (auto g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   if (b->d_raw) (void)hipFree(b->d_raw);
   if (b->d_work) (void)hipFree(b->d_work);
   if (b->d_tab) (void)hipFree(b->d_tab);
@@ -290,9 +302,12 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   }
   const size_t E = (size_t)b->total_evens;
   const size_t fR = take(E * nn * 8), fB = take(E * nn * 8), fY = take(E * nn * 8), fc = take(E * n * 8);
+  /* cross covariances of level l go to X[l & 1]; level 0's are not needed
+   * (no level below reads them), so X[0] holds level 2's at most */
+  const int64_t L2 = b->levels.size() > 2 ? b->levels[2].steps : 1;
   size_t xo[2], xto[2];
   for (int cb = 0; cb < 2; ++cb) {
-    const size_t cs = cb == 0 ? S : (size_t)std::max<int64_t>(L1, 1);
+    const size_t cs = (size_t)std::max<int64_t>(cb == 0 ? L2 : L1, 1);
     xo[cb] = take(cs * nn * 8);
     xto[cb] = take(cs * nn * 8);
   }
@@ -523,7 +538,7 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
     for (int64_t i = 0; i < q.k; ++i, ++s) {
       st.first()[s] = i == 0;
       if (i > 0) {
-        std::memcpy(st.F() + s * nn, q.F + i * nn, nn * 8);
+        std::memcpy(st.F() + s * nn, q.F + (q.F_shared ? 0 : i * nn), nn * 8);
         if (!st.H_ident) {
           if (q.H) std::memcpy(st.H() + s * nn, q.H + i * nn, nn * 8);
           else for (int d = 0; d < n; ++d) st.H()[s * nn + (size_t)d * n + d] = 1.0;
@@ -560,8 +575,8 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
  * 2 + 4 L + 2 l. */
 static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
 
-static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */) {
-  hipStream_t st = g_stream;
+static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
+                        hipStream_t st) {
   const int nl = (int)b->levels.size();
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
   if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
@@ -594,8 +609,10 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(
     uk_level_t lv = b->levels[l].dev;
     hipEvent_t* eb = ev ? ev + 2 + 4 * nl + 2 * l : nullptr;
     if (eb && !hip_ok(hipEventRecord(eb[0], st), "hipEventRecord")) return false;
-    if (!hip_ok(uk_launch_backsub(&lv, &f, b->xout, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], b->X[l & 1],
-                                  b->XT[l & 1], b->n, cov, st),
+    double* Xc = l == 0 ? nullptr : b->X[l & 1];
+    double* XTc = l == 0 ? nullptr : b->XT[l & 1];
+    if (!hip_ok(uk_launch_backsub(&lv, &f, b->xout, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], Xc, XTc, b->n,
+                                  cov, st),
                 "k_backsub launch"))
       return false;
     if (eb && !hip_ok(hipEventRecord(eb[1], st), "hipEventRecord")) return false;
@@ -603,8 +620,7 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(
   return true;
 }
 
-extern "C" ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems) {
-  if (!ensure_stream()) return nullptr;
+static ukhip_batch* batch_create_one(int32_t nprob, const ukhip_problem_t* problems) {
   Staging st;
   std::vector<int64_t> lengths;
   if (!pack_problems(nprob, problems, st, lengths)) {
@@ -620,35 +636,151 @@ extern "C" ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_
   return b;
 }
 
+extern "C" ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems) {
+  if (!ensure_stream()) return nullptr;
+  if (nprob < 1 || !problems) { uk_set_error("ukhip_batch_create: no problems"); return nullptr; }
+  bool mixed = false;
+  for (int32_t p = 1; p < nprob; ++p) mixed |= problems[p].n != problems[0].n;
+  if (!mixed) return batch_create_one(nprob, problems);
+  /* one part per state dimension, problems in input order within a part */
+  std::vector<int32_t> dims;
+  for (int32_t p = 0; p < nprob; ++p)
+    if (std::find(dims.begin(), dims.end(), problems[p].n) == dims.end()) dims.push_back(problems[p].n);
+  std::sort(dims.begin(), dims.end());
+  ukhip_batch* b = new ukhip_batch();
+  b->nprob = nprob;
+  b->where.resize(nprob);
+  for (int32_t d : dims) {
+    std::vector<ukhip_problem_t> sub;
+    for (int32_t p = 0; p < nprob; ++p)
+      if (problems[p].n == d) {
+        b->where[p] = {(int32_t)b->parts.size(), (int32_t)sub.size()};
+        sub.push_back(problems[p]);
+      }
+    ukhip_batch* q = batch_create_one((int32_t)sub.size(), sub.data());
+    if (!q) {
+      batch_release(b);
+      return nullptr;
+    }
+    b->parts.push_back(q);
+    b->steps += q->steps;
+    b->n = std::max(b->n, q->n);
+  }
+  for (size_t i = 0; i < b->parts.size(); ++i) {
+    hipStream_t st = nullptr;
+    if (!hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate")) {
+      batch_release(b);
+      return nullptr;
+    }
+    b->part_streams.push_back(st);
+  }
+  return b;
+}
+
 extern "C" void ukhip_batch_free(ukhip_batch_t* b) {
   if (b && g_stream) (void)hipStreamSynchronize(g_stream);
   batch_release(b);
 }
 extern "C" int64_t ukhip_batch_steps(const ukhip_batch_t* b) { return b ? b->steps : 0; }
-extern "C" int32_t ukhip_batch_levels(const ukhip_batch_t* b) { return b ? (int32_t)b->levels.size() : 0; }
+extern "C" int32_t ukhip_batch_levels(const ukhip_batch_t* b) {
+  if (!b) return 0;
+  int32_t l = (int32_t)b->levels.size();
+  for (auto* q : b->parts) l = std::max(l, (int32_t)q->levels.size());
+  return l;
+}
+
+static bool g_use_graphs = true;
+extern "C" void ukhip_use_graphs(int enable) { g_use_graphs = enable != 0; }
+
+/* Enqueues a whole odd-even smoothing of b on the library stream: the parts
+ * of a composite run concurrently, each on a stream forked from it. */
+static bool enqueue_oddeven(ukhip_batch* b, int cov) {
+  if (b->parts.empty()) return run_oddeven(b, cov, nullptr, g_stream);
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins(b->parts.size(), nullptr);
+  bool ok = hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate") &&
+            hip_ok(hipEventRecord(fork, g_stream), "hipEventRecord");
+  for (size_t i = 0; ok && i < b->parts.size(); ++i) {
+    ok = hip_ok(hipStreamWaitEvent(b->part_streams[i], fork, 0), "hipStreamWaitEvent") &&
+         run_oddeven(b->parts[i], cov, nullptr, b->part_streams[i]) &&
+         hip_ok(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming), "hipEventCreate") &&
+         hip_ok(hipEventRecord(joins[i], b->part_streams[i]), "hipEventRecord") &&
+         hip_ok(hipStreamWaitEvent(g_stream, joins[i], 0), "hipStreamWaitEvent");
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  for (auto e : joins)
+    if (e) (void)hipEventDestroy(e);
+  return ok;
+}
+
+/* Captures enqueue_oddeven into a graph once per covariance setting; replays
+ * it afterwards (the captured launches reference b's device buffers, which
+ * live as long as b). */
+static bool graph_oddeven(ukhip_batch* b, int cov) {
+  if (!b->graph[cov]) {
+    if (!hip_ok(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return false;
+    const bool ok = enqueue_oddeven(b, cov);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(g_stream, &g);
+    if (!ok || !hip_ok(e, "hipStreamEndCapture")) {
+      if (g) (void)hipGraphDestroy(g);
+      return false;
+    }
+    const hipError_t ei = hipGraphInstantiate(&b->graph[cov], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (!hip_ok(ei, "hipGraphInstantiate")) {
+      b->graph[cov] = nullptr;
+      return false;
+    }
+  }
+  return hip_ok(hipGraphLaunch(b->graph[cov], g_stream), "hipGraphLaunch");
+}
 
 extern "C" int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options) {
   if (!b) { uk_set_error("NULL batch"); return -1; }
   if (!ensure_stream()) return -1;
-  return run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, nullptr) ? 0 : -1;
+  const int cov = (options & KALMAN_NO_COVARIANCE) == 0;
+  return (g_use_graphs ? graph_oddeven(b, cov) : enqueue_oddeven(b, cov)) ? 0 : -1;
 }
 
 extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options) {
   if (!b) { uk_set_error("NULL batch"); return -1; }
   if (!ensure_stream()) return -1;
-  return uk_assoc_smooth(b, options);
+  if (b->parts.empty()) return uk_assoc_smooth(b, options);
+  for (auto* q : b->parts)
+    if (uk_assoc_smooth(q, options) != 0) return -1;
+  return 0;
 }
 
 extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
                                                 int32_t launches[UKHIP_OE_SLOTS]) {
   if (!b || !ms) { uk_set_error("NULL batch or result array"); return -1; }
   if (!ensure_stream()) return -1;
+  if (!b->parts.empty()) { /* parts one after another, times and launches summed */
+    double acc[UKHIP_OE_SLOTS] = {0};
+    int32_t nacc[UKHIP_OE_SLOTS] = {0};
+    for (auto* q : b->parts) {
+      double m1[UKHIP_OE_SLOTS];
+      int32_t n1[UKHIP_OE_SLOTS];
+      if (ukhip_batch_smooth_oddeven_timed(q, options, m1, n1) != 0) return -1;
+      for (int i = 0; i < UKHIP_OE_SLOTS; ++i) {
+        acc[i] += m1[i];
+        nacc[i] += n1[i];
+      }
+    }
+    for (int i = 0; i < UKHIP_OE_SLOTS; ++i) {
+      ms[i] = acc[i];
+      if (launches) launches[i] = nacc[i];
+    }
+    return 0;
+  }
   const size_t nl = b->levels.size();
   std::vector<hipEvent_t> ev(timed_events(nl), nullptr);
   bool ok = true;
   for (auto& e : ev)
     if (ok && !hip_ok(hipEventCreate(&e), "hipEventCreate")) ok = false;
-  ok = ok && run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev.data()) &&
+  ok = ok && run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev.data(), g_stream) &&
        hip_ok(hipEventSynchronize(ev[2 + 4 * nl + 1]), "hipEventSynchronize");
   if (ok) {
     auto span = [&](size_t a, size_t c) {
@@ -681,6 +813,7 @@ extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t optio
 
 extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covs) {
   if (!b || p < 0 || p >= b->nprob) { uk_set_error("bad batch or problem index"); return -1; }
+  if (!b->parts.empty()) return ukhip_batch_download(b->parts[b->where[p].first], b->where[p].second, states, covs);
   const size_t n = b->n, nn = n * n;
   const int64_t o = b->orig_start[p], k = b->k[p];
   if (states && !hip_ok(hipMemcpyAsync(states, b->xout + o * n, k * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
@@ -730,7 +863,7 @@ static void smooth_equations(kalman_options_t options, kalman_step_equations_t**
   staging_free(st);
   if (!b) return;
   const int cov = associative ? 1 : ((options & KALMAN_NO_COVARIANCE) == 0);
-  int rc = associative ? uk_assoc_smooth(b, options) : (run_oddeven(b, cov, nullptr) ? 0 : -1);
+  int rc = associative ? uk_assoc_smooth(b, options) : (run_oddeven(b, cov, nullptr, g_stream) ? 0 : -1);
   if (rc != 0) {
     batch_release(b);
     return;

@@ -20,7 +20,7 @@ class ProblemStruct(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("k", C.c_int64),
                 ("F", C.c_void_p), ("H", C.c_void_p), ("c", C.c_void_p), ("K", C.c_void_p),
                 ("K_type", C.c_char), ("G", C.c_void_p), ("o", C.c_void_p), ("m_i", C.c_void_p),
-                ("C", C.c_void_p), ("C_type", C.c_char)]
+                ("C", C.c_void_p), ("C_type", C.c_char), ("F_shared", C.c_int32)]
 
 
 _lib = None
@@ -53,6 +53,7 @@ def lib():
         L.ukhip_batch_download.restype = C.c_int
         L.ukhip_synchronize.restype = C.c_int
         L.ukhip_stream.restype = C.c_void_p
+        L.ukhip_use_graphs.argtypes = [C.c_int]
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
@@ -113,7 +114,11 @@ class Batch:
         n, m, k = pk["n"], pk["m"], p.k
         s = ProblemStruct()
         s.n, s.m, s.k = n, m, k
-        s.F = self._ptr(pk["F"])
+        if pk.get("F_shared") is not None:
+            s.F = self._ptr(np.asfortranarray(pk["F_shared"]).T.ravel())  # column-major n x n
+            s.F_shared = 1
+        else:
+            s.F = self._ptr(pk["F"])
         s.H = None
         s.c = self._ptr(pk["c"])
         K = pk["K"]

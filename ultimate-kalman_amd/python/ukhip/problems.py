@@ -219,11 +219,13 @@ def random_problem(seed, k, n, m, *, K_kind="W", C_kind="W", unobserved=(), H_ki
 # ---- deterministic large problems and (de)serialization -----------------------------
 
 def iid_problem(k, n, m, *, seed, F, G_scale=None, Kvar=1.0, Cvar=1.0, K_kind="w", C_kind="w",
-                unobserved=None, name=""):
+                unobserved=None, name="", lists=True):
     """Like uniform_problem, but observations are i.i.d. N(0,1) vectors and G_i
     is either I (G_scale None) or N(0,1)*G_scale per step. Only Philox normals
     and exact IEEE scalings are used, so the inputs are bit-identical on every
-    machine with this numpy (no BLAS on the generation path)."""
+    machine with this numpy (no BLAS on the generation path). F is the same on
+    every step (packed["F_shared"] holds it once). lists=False keeps only the
+    packed arrays (for the device batch API on large inputs)."""
     rng = rng_for(seed)
     Fp = np.broadcast_to(np.asfortranarray(F).T, (k, n, n))
     if G_scale is None:
@@ -237,6 +239,12 @@ def iid_problem(k, n, m, *, seed, F, G_scale=None, Kvar=1.0, Cvar=1.0, K_kind="w
         obs = np.array([not unobserved(i) for i in range(k)])
     Kmat, K_type = _cov_repr(K_kind, Kvar, n)
     Cmat, C_type = _cov_repr(C_kind, Cvar, m)
+    packed = dict(n=n, m=m, F=Fp, G=Gp, c=cp, o=op, observed=obs,
+                  K=Kmat, K_type=K_type, C=Cmat, C_type=C_type, F_shared=np.asfortranarray(F))
+    if not lists:
+        P = Problem(k=k, n=[n] * k, H=[], F=[], c=[], K=[], K_type=[], G=[], o=[], C=[], C_type=[], name=name)
+        P.packed = packed
+        return P
     P = Problem(k=k, n=[n] * k,
                 H=[None] + [np.eye(n)] * (k - 1),
                 F=[None] + [F] * (k - 1),
@@ -248,9 +256,31 @@ def iid_problem(k, n, m, *, seed, F, G_scale=None, Kvar=1.0, Cvar=1.0, K_kind="w
                 C=[Cmat if obs[i] else None for i in range(k)],
                 C_type=[C_type if obs[i] else None for i in range(k)],
                 name=name)
-    P.packed = dict(n=n, m=m, F=Fp, G=Gp, c=cp, o=op, observed=obs,
-                    K=Kmat, K_type=K_type, C=Cmat, C_type=C_type)
+    P.packed = packed
     return P
+
+
+def batch_specs(count=4096, seed=3, lmin=64, lmax=16384, nmin=4, nmax=32, nstep=4):
+    """The batching workload's problem list, without the data: (n, m, k, F
+    scale, seed) per problem. n uniform over {nmin, nmin+nstep, ..., nmax},
+    m = n/2, k log-uniform in [lmin, lmax]."""
+    rng = rng_for(seed)
+    specs = []
+    for p in range(count):
+        n = int(rng.choice(np.arange(nmin, nmax + 1, nstep)))
+        k = int(np.exp(rng.uniform(np.log(lmin), np.log(lmax))))
+        specs.append(dict(n=n, m=max(n // 2, 1), k=k, s=float(rng.uniform(0.95, 1.0)),
+                          seed=int(rng.integers(1 << 62)), name=f"batch{p}_n{n}_k{k}"))
+    return specs
+
+
+def batch_problem(spec, lists=True):
+    """One problem of batch_specs: F = s Q (Q random orthogonal), G ~ N(0, 1/n)
+    per step, K = C = I, i.i.d. N(0,1) observations."""
+    n, m = spec["n"], spec["m"]
+    F = spec["s"] * random_orthogonal(rng_for(spec["seed"] ^ 0x5A5A), n)
+    return iid_problem(spec["k"], n, m, seed=spec["seed"], F=F, G_scale=1.0 / np.sqrt(n),
+                       name=spec["name"], lists=lists)
 
 
 def input_digest(problem):
