@@ -21,11 +21,15 @@
  *                information form
  *   smoothing scan (same scheme, last step first) -> estimates, covariances
  *
- * Every element build and every combine runs in one 256-thread workgroup
- * with its blocks in LDS (the filtering work blocks of the 64 bucket go to a
- * per-workgroup global slice: 6 blocks of 64 x 64 exceed the 160 KiB LDS).
- * Observations are used whitened (noise covariance I), which is the
- * reference's S = G K G^T + C after the congruence with the whitening W.
+ * Every element build and every combine runs in one 256-thread workgroup.
+ * Its blocks live in LDS with the odd leading dimension NB + 1, so that the
+ * column-per-thread and transposed accesses spread over the LDS banks; the
+ * other operand of a combine is staged from HBM into LDS once. (The
+ * filtering work blocks of the 64 bucket go to a per-workgroup global slice:
+ * six 64 x 65 blocks exceed the 160 KiB of LDS.) Triangular solves keep each
+ * right-hand side in one thread's registers. Observations are used whitened
+ * (noise covariance I), which is the reference's S = G K G^T + C after the
+ * congruence with the whitening W.
  */
 #include <hip/hip_runtime.h>
 
@@ -43,8 +47,8 @@ extern "C" void* ukhip_stream(void);
 
 namespace {
 
-constexpr int AT = 256;                   /* threads of every associative kernel */
-constexpr int LDS_BYTES = 160 * 1024;     /* gfx950 LDS per workgroup */
+constexpr int AT = 256;               /* threads of every associative kernel */
+constexpr int LDS_BYTES = 160 * 1024; /* gfx950 LDS per workgroup */
 
 /* ------------------------------------------------------------------------ */
 /* workgroup-level dense algebra on blocks of order <= NB                    */
@@ -56,35 +60,54 @@ __device__ __forceinline__ double at(const double* A, int ld, bool t, int i, int
 }
 
 /* C (m x q, ld ldc) = alpha op(A) op(B) + beta C with op(A) m x kk and
- * op(B) kk x q. Each thread accumulates a TD x TD register tile; every
- * thread has read its inputs before any output is written, so C may alias
- * A or B. Ends with a barrier. */
+ * op(B) kk x q. Each thread accumulates a TD x TD register tile. The k loop
+ * runs in groups of KU whose loads are all issued before the group's FMAs
+ * (one wave per SIMD is typical here, so latency has to be hidden by
+ * independent loads); indices are clamped into range and the extra terms
+ * zeroed, so there are no branches. Every thread has read its inputs before
+ * any output is written, so C may alias A or B. Ends with a barrier. */
 template <int NB>
 __device__ void mm(double* C, int ldc, const double* A, int lda, bool tA, const double* B, int ldb, bool tB, int m,
                    int q, int kk, double alpha, double beta) {
   constexpr int TD = (NB + 15) / 16;
+  constexpr int KU = TD >= 4 ? 4 : 8;
   const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
+  int ia[TD], jb[TD];
+#pragma unroll
+  for (int a = 0; a < TD; ++a) {
+    const int i = ti + 16 * a;
+    ia[a] = i < m ? i : m - 1;
+  }
+#pragma unroll
+  for (int b = 0; b < TD; ++b) {
+    const int j = tj + 16 * b;
+    jb[b] = j < q ? j : q - 1;
+  }
   double acc[TD][TD];
 #pragma unroll
   for (int a = 0; a < TD; ++a)
 #pragma unroll
     for (int b = 0; b < TD; ++b) acc[a][b] = 0.0;
-  for (int k = 0; k < kk; ++k) {
-    double av[TD], bv[TD];
+  for (int k0 = 0; k0 < kk; k0 += KU) {
+    double av[KU][TD], bv[KU][TD];
 #pragma unroll
-    for (int a = 0; a < TD; ++a) {
-      const int i = ti + 16 * a;
-      av[a] = i < m ? at(A, lda, tA, i, k) : 0.0;
+    for (int u = 0; u < KU; ++u) {
+      const int k = k0 + u < kk ? k0 + u : kk - 1;
+#pragma unroll
+      for (int a = 0; a < TD; ++a) av[u][a] = at(A, lda, tA, ia[a], k);
+#pragma unroll
+      for (int b = 0; b < TD; ++b) bv[u][b] = at(B, ldb, tB, k, jb[b]);
     }
 #pragma unroll
-    for (int b = 0; b < TD; ++b) {
-      const int j = tj + 16 * b;
-      bv[b] = j < q ? at(B, ldb, tB, k, j) : 0.0;
+    for (int u = 0; u < KU; ++u) {
+      if (k0 + u >= kk) /* uniform: only the last group has a tail */
+#pragma unroll
+        for (int a = 0; a < TD; ++a) av[u][a] = 0.0;
+#pragma unroll
+      for (int a = 0; a < TD; ++a)
+#pragma unroll
+        for (int b = 0; b < TD; ++b) acc[a][b] = fma(av[u][a], bv[u][b], acc[a][b]);
     }
-#pragma unroll
-    for (int a = 0; a < TD; ++a)
-#pragma unroll
-      for (int b = 0; b < TD; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
   }
   __syncthreads();
 #pragma unroll
@@ -100,32 +123,70 @@ __device__ void mm(double* C, int ldc, const double* A, int lda, bool tA, const 
   __syncthreads();
 }
 
-/* y (m) = alpha op(A) x + beta y, one thread per row; y may alias x. */
+/* y (m) = alpha op(A) x + beta y, one thread per row, the k loop in groups
+ * of 8 loads with two partial sums; y may alias x. */
 __device__ void mv(double* y, const double* A, int lda, bool tA, const double* x, int m, int kk, double alpha,
                    double beta) {
-  double r = 0.0;
+  double r0 = 0.0, r1 = 0.0;
   const int i = threadIdx.x;
-  if (i < m)
-    for (int k = 0; k < kk; ++k) r = fma(at(A, lda, tA, i, k), x[k], r);
+  if (i < m) {
+    for (int k0 = 0; k0 < kk; k0 += 8) {
+      double av[8], xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u < kk ? k0 + u : kk - 1;
+        av[u] = at(A, lda, tA, i, k);
+        xv[u] = k0 + u < kk ? x[k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        r0 = fma(av[u], xv[u], r0);
+        r1 = fma(av[u + 1], xv[u + 1], r1);
+      }
+    }
+  }
+  const double r = r0 + r1;
   __syncthreads();
   if (i < m) y[i] = beta == 0.0 ? alpha * r : fma(beta, y[i], alpha * r);
   __syncthreads();
 }
 
-/* D = op(S) (m x q; op(S) transposed when t) */
+/* D = op(S) (m x q; op(S) transposed when t). Threads walk an NB x NB grid
+ * (compile-time strides) so that S is read along its columns; all of a
+ * thread's loads are issued before its stores. */
+template <int NB>
 __device__ void copy_mat(double* D, int ldd, const double* S, int lds, bool t, int m, int q) {
-  for (int idx = threadIdx.x; idx < m * q; idx += AT) {
-    const int i = idx % m, j = idx / m;
-    D[i + (int64_t)j * ldd] = at(S, lds, t, i, j);
+  constexpr int PER = (NB * NB + AT - 1) / AT;
+  double v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int idx = (int)threadIdx.x + u * AT;
+    const int r = idx % NB, c = idx / NB;
+    int i = t ? c : r, j = t ? r : c;
+    i = i < m ? i : m - 1;
+    j = j < q ? j : q - 1;
+    v[u] = at(S, lds, t, i, j);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int idx = (int)threadIdx.x + u * AT;
+    const int r = idx % NB, c = idx / NB;
+    const int i = t ? c : r, j = t ? r : c;
+    if (idx < NB * NB && i < m && j < q) D[i + (int64_t)j * ldd] = v[u];
   }
   __syncthreads();
 }
 
-/* D = d I (n x n) */
+/* D = d I on the leading n x n block; the padding rows and columns
+ * [n, NB) of the NB x NB block become the identity. */
+template <int NB>
 __device__ void set_identity(double* D, int ld, int n, double d) {
-  for (int idx = threadIdx.x; idx < n * n; idx += AT) {
-    const int i = idx % n, j = idx / n;
-    D[i + (int64_t)j * ld] = (i == j) ? d : 0.0;
+  constexpr int PER = (NB * NB + AT - 1) / AT;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int idx = (int)threadIdx.x + u * AT;
+    const int i = idx % NB, j = idx / NB;
+    if (idx < NB * NB) D[i + (int64_t)j * ld] = (i == j) ? (i < n ? d : 1.0) : 0.0;
   }
   __syncthreads();
 }
@@ -136,38 +197,128 @@ __device__ void axpy(double* y, const double* x, int m, double a) {
   __syncthreads();
 }
 
-/* LU with partial pivoting (LAPACK dgetrf order) of M (n x n, ld) in place;
- * row k was swapped with row piv[k]. */
-__device__ void lu(double* M, int ld, int n, int* piv) {
+/* y (m) = x */
+__device__ void copy_vec(double* y, const double* x, int m) {
+  for (int i = threadIdx.x; i < m; i += AT) y[i] = x[i];
+  __syncthreads();
+}
+
+/* Pivot keys: |v| with its low 7 mantissa bits replaced by (127 - row), so
+ * that an unsigned 64-bit maximum picks the largest magnitude and, among
+ * (near-)equal magnitudes, the first row. NaN sorts above every number. */
+__device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+  return (b & ~127ull) | (unsigned long long)(127 - row);
+}
+__device__ __forceinline__ int pivot_row(unsigned long long key) { return 127 - (int)(key & 127ull); }
+
+/* Workgroup scratch of gj_inverse: pivot keys, pivot rows, column order. */
+template <int NB>
+struct GJScratch {
+  unsigned long long key[NB + 1];
+  int piv[NB];
+  int col[NB];
+};
+
+/*
+ * Inverse of the leading n x n block of A (ld) by Gauss-Jordan elimination
+ * with partial pivoting (row interchanges, undone on the columns at the
+ * end). Each step maps the whole matrix from one buffer into the other
+ * (A <-> W, both ld) in one pass:
+ *   new row k  = (row p, with 1 in column k) / a_pk,
+ *   new row r  = (row r', with 0 in column k) - a_r'k * new row k,
+ * r' being row r after the interchange of rows k and p. The pivot of the
+ * next column is found in the same pass with LDS atomic maxima over the
+ * candidates' keys, so a step costs one barrier. Returns the buffer (A or
+ * W) holding the inverse; the other one is free. n <= NB <= 127.
+ */
+template <int NB>
+__device__ double* gj_inverse(double* A, double* W, int ld, int n, GJScratch<NB>& g) {
+  for (int i = threadIdx.x; i <= NB; i += AT) g.key[i] = 0ull;
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += AT) atomicMax(&g.key[0], pivot_key(A[r], r));
+  __syncthreads();
+  double* cur = A;
+  double* nxt = W;
+  constexpr int PER = (NB * NB + AT - 1) / AT;
   for (int k = 0; k < n; ++k) {
-    if (threadIdx.x < 64) { /* one wave finds the pivot row */
-      double best = -1.0;
-      int bi = k;
-      for (int i = k + (int)threadIdx.x; i < n; i += 64) {
-        const double v = fabs(M[i + k * ld]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
+    const int p = pivot_row(g.key[k]);
+    if (threadIdx.x == 0) g.piv[k] = p;
+    const double rp = 1.0 / cur[p + k * ld];
+    double pj[PER], fk[PER], aj[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) { /* all loads first, indices clamped into the block */
+      const int idx = (int)threadIdx.x + u * AT;
+      const int r = idx < NB * NB ? idx % NB : 0, j = idx < NB * NB ? idx / NB : 0;
+      const int sr = (r == p) ? k : (r == k ? p : r);
+      pj[u] = cur[p + j * ld];
+      fk[u] = cur[sr + k * ld];
+      aj[u] = cur[sr + j * ld];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int idx = (int)threadIdx.x + u * AT;
+      const int r = idx % NB, j = idx / NB;
+      if (idx < NB * NB && r < n && j < n) {
+        const double rowk = (j == k ? 1.0 : pj[u]) * rp;
+        const double v = r == k ? rowk : fma(-fk[u], rowk, j == k ? 0.0 : aj[u]);
+        nxt[r + j * ld] = v;
+        if (j == k + 1 && r > k) atomicMax(&g.key[k + 1], pivot_key(v, r));
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_down(best, off);
-        const int oi = __shfl_down(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (threadIdx.x == 0) piv[k] = bi;
     }
     __syncthreads();
-    const int pr = piv[k];
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  /* undo the interchanges on the columns, last first */
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < n; ++j) g.col[j] = j;
+    for (int k = n - 1; k >= 0; --k) {
+      const int t = g.col[k];
+      g.col[k] = g.col[g.piv[k]];
+      g.col[g.piv[k]] = t;
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NB * NB; idx += AT) {
+    const int r = idx % NB, j = idx / NB;
+    if (r < n && j < n) nxt[r + j * ld] = cur[r + g.col[j] * ld];
+  }
+  __syncthreads();
+  return nxt;
+}
+
+/* X = A^{-1} for the leading n x n block of A (ld), A left unchanged (the
+ * Gauss-Jordan inverse of a copy). W1 and W2 are scratch blocks; the
+ * result is in one of them (returned), the other is free afterwards. */
+template <int NB>
+__device__ double* inverse(const double* A, double* W1, double* W2, int ld, int n, GJScratch<NB>& g) {
+  copy_mat<NB>(W1, ld, A, ld, false, n, n);
+  return gj_inverse<NB>(W1, W2, ld, n, g);
+}
+
+/* LU with partial pivoting (LAPACK dgetrf order, whole rows swapped) of the
+ * leading n x n block of M (ld) in place; row k was interchanged with row
+ * piv[k] at step k. Used where the matrices can be ill-conditioned
+ * (information-form steps, smoothing gains): solving with the factors keeps
+ * the error at the level of a backward-stable solve, where multiplying by an
+ * explicit inverse can lose a further factor. */
+template <int NB>
+__device__ void lu(double* M, int ld, int n, int* piv, GJScratch<NB>& g) {
+  for (int i = threadIdx.x; i <= NB; i += AT) g.key[i] = 0ull;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    for (int r = k + (int)threadIdx.x; r < n; r += AT) atomicMax(&g.key[k], pivot_key(M[r + k * ld], r));
+    __syncthreads();
+    const int pr = pivot_row(g.key[k]);
     if (pr != k)
       for (int j = threadIdx.x; j < n; j += AT) {
         const double t = M[k + j * ld];
         M[k + j * ld] = M[pr + j * ld];
         M[pr + j * ld] = t;
       }
+    if (threadIdx.x == 0) piv[k] = pr;
     __syncthreads();
     const double d = M[k + k * ld];
     for (int i = k + 1 + (int)threadIdx.x; i < n; i += AT) M[i + k * ld] /= d;
@@ -181,8 +332,8 @@ __device__ void lu(double* M, int ld, int n, int* piv) {
   }
 }
 
-/* X (n x q, ld ldx) <- M^{-1} X, or M^{-T} X when trans, for M = P^T L U
- * from lu(). One thread per right-hand side. */
+/* X (n x q, ld ldx) <- A^{-1} X, or A^{-T} X when trans, with the factors of
+ * lu(A): one thread per right-hand side, in place. */
 __device__ void lu_solve(const double* M, int ld, const int* piv, int n, double* X, int ldx, int q, bool trans) {
   for (int j = threadIdx.x; j < q; j += AT) {
     double* x = X + (int64_t)j * ldx;
@@ -225,6 +376,31 @@ __device__ void lu_solve(const double* M, int ld, const int* piv, int n, double*
   __syncthreads();
 }
 
+/* Per-thread registers holding an n x n global block (ld n) distributed over
+ * the NB x NB grid: entry idx = threadIdx.x + u * AT. */
+template <int NB>
+struct RegBlk {
+  static constexpr int PER = (NB * NB + AT - 1) / AT;
+  double v[PER];
+  __device__ void load(const double* G, int n) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int idx = (int)threadIdx.x + u * AT;
+      const int i = idx % NB, j = idx / NB;
+      v[u] = (idx < NB * NB && i < n && j < n) ? G[i + (int64_t)j * n] : 0.0;
+    }
+  }
+  /* D (ld) <- the block, or its transpose; no barrier */
+  __device__ void store(double* D, int ld, int n, bool t) const {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int idx = (int)threadIdx.x + u * AT;
+      const int i = idx % NB, j = idx / NB;
+      if (idx < NB * NB && i < n && j < n) D[t ? j + i * ld : i + j * ld] = v[u];
+    }
+  }
+};
+
 /* ------------------------------------------------------------------------ */
 /* batch data                                                                */
 /* ------------------------------------------------------------------------ */
@@ -236,7 +412,7 @@ struct AsProb {
   int64_t e0; /* first filtering element (elements of steps p+1 .. k-1) */
 };
 
-/* covariance-form model and filtered distributions, one slot per step */
+/* covariance-form model and filtered distributions, one slot per step (ld n) */
 struct AsModel {
   double* F;   /* n x n (unused on a problem's first step) */
   double* c;   /* n */
@@ -247,7 +423,7 @@ struct AsModel {
   double* eta; /* filtered information vector, steps < p */
 };
 
-/* Element arrays: filtering (A, b, Z, e, J) or smoothing (E = A, g = b, L = Z). */
+/* Element arrays (ld n): filtering (A, b, Z, e, J) or smoothing (E = A, g = b, L = Z). */
 struct AsElems {
   double* A;
   double* b;
@@ -256,24 +432,31 @@ struct AsElems {
   double* J;
 };
 
+template <int NB>
+struct Ld {
+  static constexpr int L = NB + 1; /* LDS leading dimension of an NB x NB block */
+  static constexpr int SZ = NB * L;
+};
+
 /* ------------------------------------------------------------------------ */
 /* model                                                                     */
 /* ------------------------------------------------------------------------ */
 
 template <int NB>
 __global__ void __launch_bounds__(AT) ka_model(uk_chain_t wch, const int8_t* first, AsModel md, int n) {
-  __shared__ double Dm[NB * NB], Di[NB * NB];
-  __shared__ int piv[NB];
+  constexpr int L = Ld<NB>::L;
+  __shared__ double B0[Ld<NB>::SZ], B1[Ld<NB>::SZ], B2[Ld<NB>::SZ];
+  __shared__ GJScratch<NB> g;
   const int64_t s = blockIdx.x;
   if (first[s]) return;
   const int64_t nn = (int64_t)n * n;
-  copy_mat(Dm, n, wch.D + s * nn, n, false, n, n);
-  lu(Dm, n, n, piv);
-  set_identity(Di, n, n, 1.0);
-  lu_solve(Dm, n, piv, n, Di, n, n, false);                                   /* Di = D^{-1} */
-  mm<NB>(md.F + s * nn, n, Di, n, false, wch.B + s * nn, n, false, n, n, n, -1.0, 0.0);
-  mv(md.c + s * n, Di, n, false, wch.c + s * n, n, n, 1.0, 0.0);
-  mm<NB>(md.Q + s * nn, n, Di, n, false, Di, n, true, n, n, n, 1.0, 0.0);
+  copy_mat<NB>(B0, L, wch.D + s * nn, n, false, n, n);
+  double* Di = inverse<NB>(B0, B1, B2, L, n, g);                              /* D^{-1} */
+  double* Bw = B0;
+  copy_mat<NB>(Bw, L, wch.B + s * nn, n, false, n, n);
+  mm<NB>(md.F + s * nn, n, Di, L, false, Bw, L, false, n, n, n, -1.0, 0.0);   /* F = -D^{-1} B */
+  mv(md.c + s * n, Di, L, false, wch.c + s * n, n, n, 1.0, 0.0);             /* c = D^{-1} c_w */
+  mm<NB>(md.Q + s * nn, n, Di, L, false, Di, L, true, n, n, n, 1.0, 0.0);     /* Q = D^{-1} D^{-T} */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -283,12 +466,14 @@ __global__ void __launch_bounds__(AT) ka_model(uk_chain_t wch, const int8_t* fir
 template <int NB>
 __global__ void __launch_bounds__(AT) ka_prior(const AsProb* probs, uk_chain_t wch, const int32_t* m_obs, AsModel md,
                                                int n, int mr) {
-  __shared__ double Lm[NB * NB], Mx[NB * NB], Qi[NB * NB], T2[NB * NB];
+  constexpr int L = Ld<NB>::L;
+  __shared__ double Lm[Ld<NB>::SZ], Mx[Ld<NB>::SZ], Qi[Ld<NB>::SZ], T2[Ld<NB>::SZ];
   __shared__ double et[NB], v1[NB], v2[NB];
   __shared__ int piv[NB];
+  __shared__ GJScratch<NB> g;
   const AsProb pr = probs[blockIdx.x];
   const int64_t nn = (int64_t)n * n;
-  for (int i = threadIdx.x; i < n * n; i += AT) Lm[i] = 0.0;
+  set_identity<NB>(Lm, L, n, 0.0);
   for (int i = threadIdx.x; i < n; i += AT) et[i] = 0.0;
   __syncthreads();
   for (int64_t t = 0; t <= pr.p; ++t) {
@@ -297,45 +482,45 @@ __global__ void __launch_bounds__(AT) ka_prior(const AsProb* probs, uk_chain_t w
       /* predict: with Mx = Lam + F^T Q^{-1} F and r = eta - F^T Q^{-1} c,
        * Lam <- Q^{-1} - Q^{-1} F Mx^{-1} F^T Q^{-1}, eta <- Q^{-1} c + Q^{-1} F Mx^{-1} r */
       const double* F = md.F + s * nn;
-      copy_mat(Mx, n, md.Q + s * nn, n, false, n, n);
-      lu(Mx, n, n, piv);
-      set_identity(Qi, n, n, 1.0);
-      lu_solve(Mx, n, piv, n, Qi, n, n, false);                               /* Qi = Q^{-1} */
-      mm<NB>(T2, n, Qi, n, false, F, n, false, n, n, n, 1.0, 0.0);             /* T2 = Q^{-1} F */
-      copy_mat(Mx, n, Lm, n, false, n, n);
-      mm<NB>(Mx, n, F, n, true, T2, n, false, n, n, n, 1.0, 1.0);              /* Mx = Lam + F^T T2 */
-      mv(v1, Qi, n, false, md.c + s * n, n, n, 1.0, 0.0);                     /* v1 = Q^{-1} c */
+      copy_mat<NB>(Mx, L, md.Q + s * nn, n, false, n, n);
+      lu<NB>(Mx, L, n, piv, g);
+      set_identity<NB>(Qi, L, n, 1.0);
+      lu_solve(Mx, L, piv, n, Qi, L, n, false);                              /* Qi = Q^{-1} */
+      mm<NB>(T2, L, Qi, L, false, F, n, false, n, n, n, 1.0, 0.0);            /* T2 = Q^{-1} F */
+      copy_mat<NB>(Mx, L, Lm, L, false, n, n);
+      mm<NB>(Mx, L, F, n, true, T2, L, false, n, n, n, 1.0, 1.0);             /* Mx = Lam + F^T T2 */
+      mv(v1, Qi, L, false, md.c + s * n, n, n, 1.0, 0.0);                    /* v1 = Q^{-1} c */
       mv(v2, F, n, true, v1, n, n, -1.0, 0.0);
-      axpy(v2, et, n, 1.0);                                                   /* v2 = r */
-      lu(Mx, n, n, piv);
-      lu_solve(Mx, n, piv, n, v2, n, 1, false);                               /* v2 = Mx^{-1} r */
-      copy_mat(Lm, n, T2, n, true, n, n);                                     /* Lm = T2^T */
-      lu_solve(Mx, n, piv, n, Lm, n, n, false);                               /* Lm = Mx^{-1} T2^T */
-      mm<NB>(Qi, n, T2, n, false, Lm, n, false, n, n, n, -1.0, 1.0);
-      copy_mat(Lm, n, Qi, n, false, n, n);                                    /* Lam- */
-      mv(et, T2, n, false, v2, n, n, 1.0, 0.0);
-      axpy(et, v1, n, 1.0);                                                   /* eta- */
+      axpy(v2, et, n, 1.0);                                                  /* v2 = r */
+      lu<NB>(Mx, L, n, piv, g);
+      lu_solve(Mx, L, piv, n, v2, NB, 1, false);                             /* v2 = Mx^{-1} r */
+      copy_mat<NB>(Lm, L, T2, L, true, n, n);                                /* Lm = T2^T */
+      lu_solve(Mx, L, piv, n, Lm, L, n, false);                              /* Lm = Mx^{-1} T2^T */
+      mm<NB>(Qi, L, T2, L, false, Lm, L, false, n, n, n, -1.0, 1.0);
+      copy_mat<NB>(Lm, L, Qi, L, false, n, n);                               /* Lam- */
+      mv(et, T2, L, false, v2, n, n, 1.0, 0.0);
+      axpy(et, v1, n, 1.0);                                                  /* eta- */
     }
     const int m = m_obs[s];
     if (m > 0) { /* update: Lam += C^T C, eta += C^T o (whitened) */
       const double* Cw = wch.C + s * nn;
-      mm<NB>(Lm, n, Cw, mr, true, Cw, mr, false, n, n, m, 1.0, 1.0);
+      mm<NB>(Lm, L, Cw, mr, true, Cw, mr, false, n, n, m, 1.0, 1.0);
       mv(et, Cw, mr, true, wch.o + s * n, n, m, 1.0, 1.0);
     }
     if (t < pr.p) {
-      copy_mat(md.Lam + s * nn, n, Lm, n, false, n, n);
+      copy_mat<NB>(md.Lam + s * nn, n, Lm, L, false, n, n);
       for (int i = threadIdx.x; i < n; i += AT) md.eta[s * n + i] = et[i];
       __syncthreads();
     }
   }
   /* P_p = Lam^{-1}, x_p = P_p eta */
   const int64_t sp = pr.s0 + pr.p;
-  copy_mat(Mx, n, Lm, n, false, n, n);
-  lu(Mx, n, n, piv);
-  set_identity(Qi, n, n, 1.0);
-  lu_solve(Mx, n, piv, n, Qi, n, n, false);
-  copy_mat(md.Pf + sp * nn, n, Qi, n, false, n, n);
-  mv(md.xf + sp * n, Qi, n, false, et, n, n, 1.0, 0.0);
+  copy_mat<NB>(Mx, L, Lm, L, false, n, n);
+  lu<NB>(Mx, L, n, piv, g);
+  set_identity<NB>(Qi, L, n, 1.0);
+  lu_solve(Mx, L, piv, n, Qi, L, n, false);
+  copy_mat<NB>(md.Pf + sp * nn, n, Qi, L, false, n, n);
+  mv(md.xf + sp * n, Qi, L, false, et, n, n, 1.0, 0.0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -355,38 +540,42 @@ __global__ void __launch_bounds__(AT) ka_prior(const AsProb* probs, uk_chain_t w
 template <int NB>
 __global__ void __launch_bounds__(AT) ka_felem(const AsProb* probs, const int32_t* elem_prob, uk_chain_t wch,
                                                const int32_t* m_obs, AsModel md, AsElems el, int n, int mr) {
-  __shared__ double Qs[NB * NB], CQ[NB * NB], S[NB * NB], T[NB * NB];
-  __shared__ double w1[NB], r[NB];
+  constexpr int L = Ld<NB>::L;
+  __shared__ double Qs[Ld<NB>::SZ], CQ[Ld<NB>::SZ], S[Ld<NB>::SZ], T[Ld<NB>::SZ];
+  __shared__ double w1[NB], r[NB], rc[NB];
   __shared__ int piv[NB];
-  const int64_t g = blockIdx.x;
-  const AsProb pr = probs[elem_prob[g]];
-  const int64_t t = g - pr.e0 + pr.p + 1;
+  __shared__ GJScratch<NB> g;
+  const int64_t ge = blockIdx.x;
+  const AsProb pr = probs[elem_prob[ge]];
+  const int64_t t = ge - pr.e0 + pr.p + 1;
   const int64_t s = pr.s0 + t;
   const bool firstel = t == pr.p + 1;
   const int64_t nn = (int64_t)n * n;
   const double* F = md.F + s * nn;
   const double* c = md.c + s * n;
-  double* A = el.A + g * nn;
-  double* b = el.b + g * n;
-  double* Z = el.Z + g * nn;
-  double* e = el.e + g * n;
-  double* J = el.J + g * nn;
+  double* A = el.A + ge * nn;
+  double* b = el.b + ge * n;
+  double* Z = el.Z + ge * nn;
+  double* e = el.e + ge * n;
+  double* J = el.J + ge * nn;
   const int64_t spv = pr.s0 + pr.p;
 
-  copy_mat(Qs, n, md.Q + s * nn, n, false, n, n);
-  for (int i = threadIdx.x; i < n; i += AT) w1[i] = c[i];
-  __syncthreads();
+  copy_mat<NB>(Qs, L, md.Q + s * nn, n, false, n, n);                       /* Qs = Q */
+  copy_vec(w1, c, n);
   if (firstel) { /* Q1 = Q + F P F^T, m1 = F x + c */
-    mm<NB>(T, n, F, n, false, md.Pf + spv * nn, n, false, n, n, n, 1.0, 0.0);
-    mm<NB>(Qs, n, T, n, false, F, n, true, n, n, n, 1.0, 1.0);
+    mm<NB>(T, L, F, n, false, md.Pf + spv * nn, n, false, n, n, n, 1.0, 0.0);
+    mm<NB>(Qs, L, T, L, false, F, n, true, n, n, n, 1.0, 1.0);
     mv(w1, F, n, false, md.xf + spv * n, n, n, 1.0, 1.0);
   }
   const int m = m_obs[s];
   if (m == 0) {
-    for (int idx = threadIdx.x; idx < n * n; idx += AT) {
-      A[idx] = firstel ? 0.0 : F[idx];
-      Z[idx] = Qs[idx];
-      J[idx] = 0.0;
+    for (int idx = threadIdx.x; idx < NB * NB; idx += AT) {
+      const int i = idx % NB, j = idx / NB;
+      if (i < n && j < n) {
+        A[i + j * n] = firstel ? 0.0 : F[i + j * n];
+        Z[i + j * n] = Qs[i + j * L];
+        J[i + j * n] = 0.0;
+      }
     }
     for (int i = threadIdx.x; i < n; i += AT) {
       b[i] = w1[i];
@@ -394,100 +583,137 @@ __global__ void __launch_bounds__(AT) ka_felem(const AsProb* probs, const int32_
     }
     return;
   }
-  const double* Cw = wch.C + s * nn; /* m x n, ld mr */
   const double* ow = wch.o + s * n;
-  mm<NB>(CQ, m, Cw, mr, false, Qs, n, false, m, n, n, 1.0, 0.0);             /* CQ = C Q */
-  set_identity(S, m, m, 1.0);
-  mm<NB>(S, m, CQ, m, false, Cw, mr, true, m, m, n, 1.0, 1.0);               /* S = C Q C^T + I */
-  lu(S, m, m, piv);
-  copy_mat(T, m, CQ, m, false, m, n);
-  lu_solve(S, m, piv, m, T, m, n, false);                                    /* T = S^{-1} C Q = K^T */
-  mv(r, Cw, mr, false, w1, m, n, -1.0, 0.0);
+  copy_mat<NB>(T, L, wch.C + s * nn, mr, false, m, n);                       /* T = C (m x n) */
+  mm<NB>(CQ, L, T, L, false, Qs, L, false, m, n, n, 1.0, 0.0);               /* CQ = C Q */
+  set_identity<NB>(S, L, m, 1.0);
+  mm<NB>(S, L, CQ, L, false, T, L, true, m, m, n, 1.0, 1.0);                 /* S = C Q C^T + I */
+  mv(r, T, L, false, w1, m, n, -1.0, 0.0);
   axpy(r, ow, m, 1.0);                                                       /* r = o - C w1 */
-  mv(b, T, m, true, r, n, m, 1.0, 0.0);
-  axpy(b, w1, n, 1.0);                                                       /* b = w1 + K r */
-  copy_mat(Z, n, Qs, n, false, n, n);
-  mm<NB>(Z, n, T, m, true, CQ, m, false, n, n, m, -1.0, 1.0);                /* Z = Q - K C Q */
-  mm<NB>(CQ, m, Cw, mr, false, F, n, false, m, n, n, 1.0, 0.0);              /* CQ <- C F */
+  mv(rc, T, L, false, c, m, n, -1.0, 0.0);
+  axpy(rc, ow, m, 1.0);                                                      /* rc = o - C c */
+  mm<NB>(J, n, T, L, false, F, n, false, m, n, n, 1.0, 0.0);                 /* C F, parked in J's slot */
+  copy_mat<NB>(Z, n, Qs, L, false, n, n);                                    /* Z = Q (Qs is free now) */
+  /* S can be ill-conditioned on the first element (Q1 holds the prior
+   * covariance): LU solves rather than an explicit inverse */
+  lu<NB>(S, L, m, piv, g);
+  copy_mat<NB>(T, L, CQ, L, false, m, n);
+  lu_solve(S, L, piv, m, T, L, n, false);                                    /* T = S^{-1} C Q (the gain^T) */
+  mv(b, T, L, true, r, n, m, 1.0, 0.0);
+  axpy(b, w1, n, 1.0);                                                       /* b = w1 + gain r */
+  mm<NB>(Z, n, T, L, true, CQ, L, false, n, n, m, -1.0, 1.0);                /* Z = Q - gain C Q */
+  copy_mat<NB>(CQ, L, J, n, false, m, n);                                    /* CQ <- C F */
   if (firstel) {
-    for (int idx = threadIdx.x; idx < n * n; idx += AT) A[idx] = 0.0;
+    for (int idx = threadIdx.x; idx < NB * NB; idx += AT) {
+      const int i = idx % NB, j = idx / NB;
+      if (i < n && j < n) A[i + j * n] = 0.0;
+    }
     __syncthreads();
   } else {
-    copy_mat(A, n, F, n, false, n, n);
-    mm<NB>(A, n, T, m, true, CQ, m, false, n, n, m, -1.0, 1.0);              /* A = F - K C F */
+    copy_mat<NB>(A, n, F, n, false, n, n);
+    mm<NB>(A, n, T, L, true, CQ, L, false, n, n, m, -1.0, 1.0);              /* A = F - gain C F */
   }
-  mv(r, Cw, mr, false, c, m, n, -1.0, 0.0);
-  axpy(r, ow, m, 1.0);                                                       /* r = o - C c */
-  lu_solve(S, m, piv, m, r, m, 1, false);
-  mv(e, CQ, m, true, r, n, m, 1.0, 0.0);                                     /* e = (C F)^T S^{-1} r */
-  copy_mat(T, m, CQ, m, false, m, n);
-  lu_solve(S, m, piv, m, T, m, n, false);                                    /* T = S^{-1} C F */
-  mm<NB>(J, n, CQ, m, true, T, m, false, n, n, m, 1.0, 0.0);                 /* J = (C F)^T S^{-1} C F */
+  lu_solve(S, L, piv, m, rc, NB, 1, false);                                  /* rc = S^{-1} (o - C c) */
+  mv(e, CQ, L, true, rc, n, m, 1.0, 0.0);                                    /* e = (C F)^T S^{-1} rc */
+  copy_mat<NB>(T, L, CQ, L, false, m, n);
+  lu_solve(S, L, piv, m, T, L, n, false);                                    /* T = S^{-1} C F */
+  mm<NB>(J, n, CQ, L, true, T, L, false, n, n, m, 1.0, 0.0);                 /* J = (C F)^T S^{-1} C F */
 }
 
 /* ------------------------------------------------------------------------ */
 /* combines (c :889-1016, matlab :240-262)                                   */
 /* ------------------------------------------------------------------------ */
 
-/* blocks of an accumulator or of the work area, in LDS or global memory */
+/* blocks of an accumulator or of the work area (LDS, or global for the
+ * filtering work of the 64 bucket), leading dimension Ld<NB>::L */
 struct Blk {
   double *A, *b, *Z, *e, *J;
 };
+template <int NB>
 struct Work {
-  double *M, *P, *T, *v1, *v2;
-  int* piv;
+  double *M, *W1, *W2, *R, *T, *v1, *v2;
+  GJScratch<NB>* g;
 };
 
 /* a <- a (x) element g, filtering:
- *   M = I + Z_i J_j,  Y^T = M^{-1} A_i,  X^T = M^{-T} A_j^T,
+ *   M = I + Z_i J_j,  Y^T = M^{-1} A_i,  X = A_j M^{-1},
  *   e = Y (e_j - J_j b_i) + e_i,  J = Y J_j A_i + J_i,
- *   b = X (b_i + Z_i e_j) + b_j,  Z = X Z_i A_j^T + Z_j,  A = X A_i. */
+ *   b = X (b_i + Z_i e_j) + b_j,  Z = X Z_i A_j^T + Z_j,  A = X A_i.
+ * Element g's blocks are read from HBM into registers first (all loads in
+ * flight together) and written into LDS where they are needed. */
 template <int NB>
-__device__ void filt_combine(const Blk& a, const Work& w, const AsElems& el, int64_t g, int n) {
+__device__ void filt_combine(const Blk& a, const Work<NB>& w, const AsElems& el, int64_t g, int n) {
+  constexpr int L = Ld<NB>::L;
   const int64_t nn = (int64_t)n * n;
-  const double *Aj = el.A + g * nn, *bj = el.b + g * n, *Zj = el.Z + g * nn, *ej = el.e + g * n, *Jj = el.J + g * nn;
-  set_identity(w.M, n, n, 1.0);
-  mm<NB>(w.M, n, a.Z, n, false, Jj, n, false, n, n, n, 1.0, 1.0);
-  lu(w.M, n, n, w.piv);
-  copy_mat(w.P, n, a.A, n, false, n, n);
-  lu_solve(w.M, n, w.piv, n, w.P, n, n, false);                              /* P = Y^T */
-  mv(w.v2, Jj, n, false, a.b, n, n, -1.0, 0.0);
-  axpy(w.v2, ej, n, 1.0);
-  mv(a.e, w.P, n, true, w.v2, n, n, 1.0, 1.0);                               /* e */
-  mm<NB>(w.T, n, Jj, n, false, a.A, n, false, n, n, n, 1.0, 0.0);
-  mm<NB>(a.J, n, w.P, n, true, w.T, n, false, n, n, n, 1.0, 1.0);            /* J */
-  copy_mat(w.P, n, Aj, n, true, n, n);
-  lu_solve(w.M, n, w.piv, n, w.P, n, n, true);                               /* P = X^T */
-  mv(w.v1, a.Z, n, false, ej, n, n, 1.0, 0.0);
+  const double *bj = el.b + g * n, *ej = el.e + g * n;
+  /* Up to NB = 48 all three blocks are requested at once; the 64 bucket
+   * fetches A_j and Z_j where they are used, to stay within its registers. */
+  constexpr bool EARLY = NB <= 48;
+  RegBlk<NB> rA, rZ, rJ;
+  rJ.load(el.J + g * nn, n);
+  if (EARLY) {
+    rA.load(el.A + g * nn, n);
+    rZ.load(el.Z + g * nn, n);
+  }
+  rJ.store(w.T, L, n, false);                                                /* T = J_j */
+  set_identity<NB>(w.M, L, n, 1.0);
+  mm<NB>(w.M, L, a.Z, L, false, w.T, L, false, n, n, n, 1.0, 1.0);           /* M = I + Z_i J_j */
+  mv(w.v2, w.T, L, false, a.b, n, n, -1.0, 0.0);
+  axpy(w.v2, ej, n, 1.0);                                                    /* v2 = e_j - J_j b_i */
+  mm<NB>(w.T, L, w.T, L, false, a.A, L, false, n, n, n, 1.0, 0.0);           /* T = J_j A_i */
+  double* Mi = inverse<NB>(w.M, w.W1, w.W2, L, n, *w.g);                     /* M^{-1} */
+  double* F = w.M;
+  mm<NB>(F, L, Mi, L, false, a.A, L, false, n, n, n, 1.0, 0.0);              /* F = Y^T */
+  mv(a.e, F, L, true, w.v2, n, n, 1.0, 1.0);                                 /* e */
+  mm<NB>(a.J, L, F, L, true, w.T, L, false, n, n, n, 1.0, 1.0);              /* J */
+  if (!EARLY) rA.load(el.A + g * nn, n);
+  rA.store(w.R, L, n, false);                                                /* R = A_j */
+  __syncthreads();
+  mm<NB>(w.T, L, w.R, L, false, Mi, L, false, n, n, n, 1.0, 0.0);            /* T = X */
+  mv(w.v1, a.Z, L, false, ej, n, n, 1.0, 0.0);
   axpy(w.v1, a.b, n, 1.0);
-  mv(a.b, w.P, n, true, w.v1, n, n, 1.0, 0.0);
+  mv(a.b, w.T, L, false, w.v1, n, n, 1.0, 0.0);
   axpy(a.b, bj, n, 1.0);                                                     /* b */
-  mm<NB>(w.T, n, w.P, n, true, a.Z, n, false, n, n, n, 1.0, 0.0);
-  copy_mat(a.Z, n, Zj, n, false, n, n);
-  mm<NB>(a.Z, n, w.T, n, false, Aj, n, true, n, n, n, 1.0, 1.0);             /* Z */
-  mm<NB>(a.A, n, w.P, n, true, a.A, n, false, n, n, n, 1.0, 0.0);            /* A */
+  mm<NB>(F, L, w.T, L, false, a.Z, L, false, n, n, n, 1.0, 0.0);             /* F = X Z_i */
+  if (!EARLY) rZ.load(el.Z + g * nn, n);
+  rZ.store(a.Z, L, n, false);                                                /* Z = Z_j */
+  __syncthreads();
+  mm<NB>(a.Z, L, F, L, false, w.R, L, true, n, n, n, 1.0, 1.0);              /* Z */
+  mm<NB>(a.A, L, w.T, L, false, a.A, L, false, n, n, n, 1.0, 0.0);           /* A */
 }
 
 /* a <- a (x) element g, smoothing; a is the running suffix and g the next
  * earlier step: E = E_j E_i, g = E_j g_i + g_j, L = E_j L_i E_j^T + L_j. */
 template <int NB>
-__device__ void smooth_combine(const Blk& a, const Work& w, const AsElems& el, int64_t g, int n) {
+__device__ void smooth_combine(const Blk& a, const Work<NB>& w, const AsElems& el, int64_t g, int n) {
+  constexpr int L = Ld<NB>::L;
   const int64_t nn = (int64_t)n * n;
-  const double *Ej = el.A + g * nn, *gj = el.b + g * n, *Lj = el.Z + g * nn;
-  mm<NB>(a.A, n, Ej, n, false, a.A, n, false, n, n, n, 1.0, 0.0);
-  mv(a.b, Ej, n, false, a.b, n, n, 1.0, 0.0);
+  const double* gj = el.b + g * n;
+  RegBlk<NB> rE, rL;
+  rE.load(el.A + g * nn, n);
+  rL.load(el.Z + g * nn, n);
+  rE.store(w.W1, L, n, false);                                               /* W1 = E_j */
+  __syncthreads();
+  mm<NB>(a.A, L, w.W1, L, false, a.A, L, false, n, n, n, 1.0, 0.0);
+  mv(a.b, w.W1, L, false, a.b, n, n, 1.0, 0.0);
   axpy(a.b, gj, n, 1.0);
-  mm<NB>(w.T, n, Ej, n, false, a.Z, n, false, n, n, n, 1.0, 0.0);
-  copy_mat(a.Z, n, Lj, n, false, n, n);
-  mm<NB>(a.Z, n, w.T, n, false, Ej, n, true, n, n, n, 1.0, 1.0);
+  mm<NB>(w.T, L, w.W1, L, false, a.Z, L, false, n, n, n, 1.0, 0.0);
+  rL.store(a.Z, L, n, false);
+  __syncthreads();
+  mm<NB>(a.Z, L, w.T, L, false, w.W1, L, true, n, n, n, 1.0, 1.0);
 }
 
-__device__ void blk_load(const Blk& a, const AsElems& el, int64_t g, int n, bool filt) {
+template <int NB>
+__device__ void blk_load(const Blk& a, int ld, const AsElems& el, int64_t g, int n, bool filt) {
   const int64_t nn = (int64_t)n * n;
-  for (int idx = threadIdx.x; idx < n * n; idx += AT) {
-    a.A[idx] = el.A[g * nn + idx];
-    a.Z[idx] = el.Z[g * nn + idx];
-    if (filt) a.J[idx] = el.J[g * nn + idx];
+  RegBlk<NB> r;
+  r.load(el.A + g * nn, n);
+  r.store(a.A, ld, n, false);
+  r.load(el.Z + g * nn, n);
+  r.store(a.Z, ld, n, false);
+  if (filt) {
+    r.load(el.J + g * nn, n);
+    r.store(a.J, ld, n, false);
   }
   for (int i = threadIdx.x; i < n; i += AT) {
     a.b[i] = el.b[g * n + i];
@@ -496,12 +722,16 @@ __device__ void blk_load(const Blk& a, const AsElems& el, int64_t g, int n, bool
   __syncthreads();
 }
 
-__device__ void blk_store(const Blk& a, const AsElems& el, int64_t g, int n, bool filt) {
+template <int NB>
+__device__ void blk_store(const Blk& a, int ld, const AsElems& el, int64_t g, int n, bool filt) {
   const int64_t nn = (int64_t)n * n;
-  for (int idx = threadIdx.x; idx < n * n; idx += AT) {
-    el.A[g * nn + idx] = a.A[idx];
-    el.Z[g * nn + idx] = a.Z[idx];
-    if (filt) el.J[g * nn + idx] = a.J[idx];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += AT) {
+    const int i = idx % NB, j = idx / NB;
+    if (i < n && j < n) {
+      el.A[g * nn + i + j * n] = a.A[i + j * ld];
+      el.Z[g * nn + i + j * n] = a.Z[i + j * ld];
+      if (filt) el.J[g * nn + i + j * n] = a.J[i + j * ld];
+    }
   }
   for (int i = threadIdx.x; i < n; i += AT) {
     el.b[g * n + i] = a.b[i];
@@ -513,36 +743,40 @@ __device__ void blk_store(const Blk& a, const AsElems& el, int64_t g, int n, boo
 /* LDS plan of a scan kernel: accumulator, then the work area when it fits */
 template <int NB, bool FILT>
 struct ScanPlan {
-  static constexpr int ACC = FILT ? 3 * NB * NB + 2 * NB : 2 * NB * NB + NB;
-  static constexpr int WORK = FILT ? 3 * NB * NB + 2 * NB : NB * NB;
+  static constexpr int SZ = Ld<NB>::SZ;
+  static constexpr int ACC = FILT ? 3 * SZ + 2 * NB : 2 * SZ + NB;
+  static constexpr int WORK = FILT ? 5 * SZ + 2 * NB : 2 * SZ;             /* M, W1, W2, R, T, v1, v2 | W1, T */
   static constexpr bool WORK_GLOBAL = (ACC + WORK) * 8 + NB * 4 + 256 > LDS_BYTES;
   static constexpr int LDS = ACC + (WORK_GLOBAL ? 0 : WORK);
 };
 
 template <int NB, bool FILT>
-__device__ void scan_setup(double* lds, double* gwork, Blk& a, Work& w) {
+__device__ void scan_setup(double* lds, double* gwork, Blk& a, Work<NB>& w) {
   using SP = ScanPlan<NB, FILT>;
+  constexpr int SZ = SP::SZ;
   double* q = lds;
-  a.A = q; q += NB * NB;
-  a.Z = q; q += NB * NB;
+  a.A = q; q += SZ;
+  a.Z = q; q += SZ;
   a.J = nullptr;
   a.e = nullptr;
-  if (FILT) { a.J = q; q += NB * NB; }
+  if (FILT) { a.J = q; q += SZ; }
   a.b = q; q += NB;
   if (FILT) { a.e = q; q += NB; }
   double* p = SP::WORK_GLOBAL ? gwork + (int64_t)blockIdx.x * SP::WORK : q;
-  w.T = p; p += NB * NB;
-  w.M = w.P = w.v1 = w.v2 = nullptr;
+  w.W1 = p; p += SZ;
+  w.T = p; p += SZ;
+  w.M = w.W2 = w.R = w.v1 = w.v2 = nullptr;
   if (FILT) {
-    w.M = p; p += NB * NB;
-    w.P = p; p += NB * NB;
+    w.M = p; p += SZ;
+    w.W2 = p; p += SZ;
+    w.R = p; p += SZ;
     w.v1 = p; p += NB;
     w.v2 = p; p += NB;
   }
 }
 
 template <int NB, bool FILT>
-__device__ __forceinline__ void combine(const Blk& a, const Work& w, const AsElems& el, int64_t g, int n) {
+__device__ __forceinline__ void combine(const Blk& a, const Work<NB>& w, const AsElems& el, int64_t g, int n) {
   if (FILT) filt_combine<NB>(a, w, el, g, n);
   else smooth_combine<NB>(a, w, el, g, n);
 }
@@ -568,15 +802,15 @@ template <int NB, bool FILT>
 __global__ void __launch_bounds__(AT) ka_scan_reduce(const Chunk* ch, AsElems el, AsElems tot, int n, double* gwork) {
   using SP = ScanPlan<NB, FILT>;
   __shared__ double lds[SP::LDS];
-  __shared__ int piv[NB];
+  __shared__ GJScratch<NB> gjs;
   Blk a;
-  Work w;
+  Work<NB> w;
   scan_setup<NB, FILT>(lds, gwork, a, w);
-  w.piv = piv;
+  w.g = &gjs;
   const Chunk c = ch[blockIdx.x];
-  blk_load(a, el, c.first, n, FILT);
+  blk_load<NB>(a, Ld<NB>::L, el, c.first, n, FILT);
   for (int i = 1; i < c.len; ++i) combine<NB, FILT>(a, w, el, c.first + (int64_t)i * c.dir, n);
-  blk_store(a, tot, c.tot, n, FILT);
+  blk_store<NB>(a, Ld<NB>::L, tot, c.tot, n, FILT);
 }
 
 /* pass 2, one Hillis-Steele round over the chunk totals of every problem:
@@ -586,19 +820,19 @@ __global__ void __launch_bounds__(AT) ka_scan_round(const Chunk* ch, AsElems in,
                                                     double* gwork) {
   using SP = ScanPlan<NB, FILT>;
   __shared__ double lds[SP::LDS];
-  __shared__ int piv[NB];
+  __shared__ GJScratch<NB> gjs;
   Blk a;
-  Work w;
+  Work<NB> w;
   scan_setup<NB, FILT>(lds, gwork, a, w);
-  w.piv = piv;
+  w.g = &gjs;
   const Chunk c = ch[blockIdx.x];
   if (c.rank >= d) {
-    blk_load(a, in, c.tot - d, n, FILT);
+    blk_load<NB>(a, Ld<NB>::L, in, c.tot - d, n, FILT);
     combine<NB, FILT>(a, w, in, c.tot, n);
   } else {
-    blk_load(a, in, c.tot, n, FILT);
+    blk_load<NB>(a, Ld<NB>::L, in, c.tot, n, FILT);
   }
-  blk_store(a, out, c.tot, n, FILT);
+  blk_store<NB>(a, Ld<NB>::L, out, c.tot, n, FILT);
 }
 
 /* pass 3: every prefix within a chunk, seeded with the inclusive total of
@@ -607,23 +841,23 @@ template <int NB, bool FILT>
 __global__ void __launch_bounds__(AT) ka_scan_down(const Chunk* ch, AsElems el, AsElems tot, int n, double* gwork) {
   using SP = ScanPlan<NB, FILT>;
   __shared__ double lds[SP::LDS];
-  __shared__ int piv[NB];
+  __shared__ GJScratch<NB> gjs;
   Blk a;
-  Work w;
+  Work<NB> w;
   scan_setup<NB, FILT>(lds, gwork, a, w);
-  w.piv = piv;
+  w.g = &gjs;
   const Chunk c = ch[blockIdx.x];
   int i0 = 0;
   if (c.rank == 0) {
-    blk_load(a, el, c.first, n, FILT);
+    blk_load<NB>(a, Ld<NB>::L, el, c.first, n, FILT);
     i0 = 1;
   } else {
-    blk_load(a, tot, c.tot - 1, n, FILT);
+    blk_load<NB>(a, Ld<NB>::L, tot, c.tot - 1, n, FILT);
   }
   for (int i = i0; i < c.len; ++i) {
     const int64_t g = c.first + (int64_t)i * c.dir;
     combine<NB, FILT>(a, w, el, g, n);
-    blk_store(a, el, g, n, FILT);
+    blk_store<NB>(a, Ld<NB>::L, el, g, n, FILT);
   }
 }
 
@@ -653,20 +887,22 @@ __global__ void __launch_bounds__(AT) ka_filtered(const AsProb* probs, const int
 template <int NB>
 __global__ void __launch_bounds__(AT) ka_selem(const AsProb* probs, const int32_t* step_prob, AsModel md, AsElems el,
                                                int n) {
-  __shared__ double M[NB * NB], T[NB * NB], U[NB * NB];
+  constexpr int L = Ld<NB>::L;
+  __shared__ double M[Ld<NB>::SZ], T[Ld<NB>::SZ], U[Ld<NB>::SZ];
   __shared__ double v1[NB], v2[NB];
   __shared__ int piv[NB];
+  __shared__ GJScratch<NB> g;
   const int64_t s = blockIdx.x;
   const AsProb pr = probs[step_prob[s]];
   const int64_t t = s - pr.s0;
   const int64_t nn = (int64_t)n * n;
   double* E = el.A + s * nn;
   double* gv = el.b + s * n;
-  double* L = el.Z + s * nn;
+  double* Lo = el.Z + s * nn;
   if (t == pr.k - 1) {
     for (int idx = threadIdx.x; idx < n * n; idx += AT) {
       E[idx] = 0.0;
-      L[idx] = md.Pf[s * nn + idx];
+      Lo[idx] = md.Pf[s * nn + idx];
     }
     for (int i = threadIdx.x; i < n; i += AT) gv[i] = md.xf[s * n + i];
     return;
@@ -677,38 +913,44 @@ __global__ void __launch_bounds__(AT) ka_selem(const AsProb* probs, const int32_
   if (t >= pr.p) {
     const double* P = md.Pf + s * nn;
     const double* x = md.xf + s * n;
-    mm<NB>(T, n, P, n, false, F, n, true, n, n, n, 1.0, 0.0);                 /* T = P F^T */
-    copy_mat(M, n, Q, n, false, n, n);
-    mm<NB>(M, n, F, n, false, T, n, false, n, n, n, 1.0, 1.0);                /* M = F P F^T + Q */
-    lu(M, n, n, piv);
-    copy_mat(U, n, T, n, true, n, n);                                         /* U = F P */
-    lu_solve(M, n, piv, n, U, n, n, true);                                    /* U = E^T */
-    copy_mat(E, n, U, n, true, n, n);
-    mv(v1, F, n, false, x, n, n, 1.0, 0.0);
-    axpy(v1, c, n, 1.0);                                                      /* v1 = F x + c */
-    mv(gv, U, n, true, v1, n, n, -1.0, 0.0);
-    axpy(gv, x, n, 1.0);                                                      /* g = x - E v1 */
-    copy_mat(M, n, T, n, true, n, n);                                         /* M = F P */
-    copy_mat(L, n, P, n, false, n, n);
-    mm<NB>(L, n, U, n, true, M, n, false, n, n, n, -1.0, 1.0);                /* L = P - E F P */
-  } else {
-    copy_mat(M, n, Q, n, false, n, n);
-    lu(M, n, n, piv);
-    copy_mat(T, n, F, n, false, n, n);
-    lu_solve(M, n, piv, n, T, n, n, false);                                   /* T = Q^{-1} F */
-    for (int i = threadIdx.x; i < n; i += AT) v1[i] = c[i];
+    copy_mat<NB>(U, L, P, n, false, n, n);                                   /* U = P */
+    copy_mat<NB>(M, L, F, n, false, n, n);                                   /* M = F */
+    mm<NB>(T, L, U, L, false, M, L, true, n, n, n, 1.0, 0.0);                /* T = P F^T */
+    mv(v1, M, L, false, x, n, n, 1.0, 0.0);
+    axpy(v1, c, n, 1.0);                                                     /* v1 = F x + c */
+    mm<NB>(U, L, M, L, false, T, L, false, n, n, n, 1.0, 0.0);               /* U = F P F^T */
+    copy_mat<NB>(M, L, Q, n, false, n, n);
+    for (int idx = threadIdx.x; idx < NB * NB; idx += AT) {
+      const int i = idx % NB, j = idx / NB;
+      if (i < n && j < n) M[i + j * L] += U[i + j * L];                      /* M = F P F^T + Q */
+    }
     __syncthreads();
-    lu_solve(M, n, piv, n, v1, n, 1, false);                                  /* v1 = Q^{-1} c */
-    copy_mat(U, n, md.Lam + s * nn, n, false, n, n);
-    mm<NB>(U, n, F, n, true, T, n, false, n, n, n, 1.0, 1.0);                 /* U = Lam + F^T Q^{-1} F */
-    lu(U, n, n, piv);
-    set_identity(M, n, n, 1.0);
-    lu_solve(U, n, piv, n, M, n, n, false);                                   /* M = L */
-    copy_mat(L, n, M, n, false, n, n);
-    mm<NB>(E, n, M, n, false, T, n, true, n, n, n, 1.0, 0.0);                 /* E = L (Q^{-1} F)^T */
+    lu<NB>(M, L, n, piv, g);
+    copy_mat<NB>(U, L, T, L, true, n, n);                                    /* U = F P */
+    lu_solve(M, L, piv, n, U, L, n, true);                                   /* U = E^T */
+    copy_mat<NB>(E, n, U, L, true, n, n);
+    mv(gv, U, L, true, v1, n, n, -1.0, 0.0);
+    axpy(gv, x, n, 1.0);                                                     /* g = x - E (F x + c) */
+    copy_mat<NB>(M, L, T, L, true, n, n);                                    /* M = F P */
+    copy_mat<NB>(Lo, n, P, n, false, n, n);
+    mm<NB>(Lo, n, U, L, true, M, L, false, n, n, n, -1.0, 1.0);              /* L = P - E F P */
+  } else {
+    copy_mat<NB>(M, L, Q, n, false, n, n);
+    lu<NB>(M, L, n, piv, g);
+    copy_mat<NB>(T, L, F, n, false, n, n);
+    lu_solve(M, L, piv, n, T, L, n, false);                                  /* T = Q^{-1} F */
+    copy_vec(v1, c, n);
+    lu_solve(M, L, piv, n, v1, NB, 1, false);                                /* v1 = Q^{-1} c */
+    copy_mat<NB>(U, L, md.Lam + s * nn, n, false, n, n);
+    mm<NB>(U, L, F, n, true, T, L, false, n, n, n, 1.0, 1.0);                /* U = Lam + F^T Q^{-1} F */
+    lu<NB>(U, L, n, piv, g);
+    set_identity<NB>(M, L, n, 1.0);
+    lu_solve(U, L, piv, n, M, L, n, false);                                  /* M = L */
+    copy_mat<NB>(Lo, n, M, L, false, n, n);
+    mm<NB>(E, n, M, L, false, T, L, true, n, n, n, 1.0, 0.0);                /* E = L (Q^{-1} F)^T */
     mv(v2, F, n, true, v1, n, n, -1.0, 0.0);
     axpy(v2, md.eta + s * n, n, 1.0);
-    mv(gv, M, n, false, v2, n, n, 1.0, 0.0);                                  /* g = L (eta - F^T Q^{-1} c) */
+    mv(gv, M, L, false, v2, n, n, 1.0, 0.0);                                 /* g = L (eta - F^T Q^{-1} c) */
   }
 }
 
@@ -722,7 +964,6 @@ __global__ void __launch_bounds__(AT) ka_results(AsElems el, double* xout, doubl
 }
 
 } /* namespace */
-
 /* ------------------------------------------------------------------------ */
 /* host driver                                                               */
 /* ------------------------------------------------------------------------ */
@@ -759,8 +1000,9 @@ int64_t make_chunks(std::vector<Chunk>& out, int64_t base, int64_t len, int dir,
   return nch;
 }
 
-/* chunk length: about CHUNKS_TARGET chunks over the batch, at least 4 */
-constexpr int64_t CHUNKS_TARGET = 2048;
+/* chunk length: about CHUNKS_TARGET chunks over the batch (two workgroups per
+ * compute unit at one workgroup per unit), at least 4 */
+constexpr int64_t CHUNKS_TARGET = 512;
 
 template <int NB>
 int run_assoc(ukhip_batch* b) {
