@@ -66,6 +66,13 @@ void ukhip_use_graphs(int enable);
 int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options);
 int ukhip_synchronize(void);
 
+/* Mixed precision: the odd-even elimination in FP32, then `refinements`
+ * steps of FP64 iterative refinement of the estimates (corrected
+ * semi-normal equations with the FP32 block factor; each step multiplies
+ * the error by about cond^2 * 6e-8). Estimates only: the covariances of the
+ * batch are left as they were. Enqueued on the library stream. */
+int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements);
+
 /* Copies problem p's estimates (k*n) and covariances (k*n*n, may be NULL) to the host. */
 int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covariances);
 

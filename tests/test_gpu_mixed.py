@@ -1,0 +1,91 @@
+"""The mixed-precision odd-even path (FP32 elimination, FP64 iterative
+refinement of the estimates, ultimate-kalman_amd/csrc/refine.hip) against
+the FP64 odd-even path and the oracle.
+
+Bar: after the refinement steps, per-step normwise relative error of the
+estimates <= 1e-9 against the FP64 results (which match the oracle to
+~1e-13); each refinement step must shrink the error until it reaches the
+FP64 rounding level.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from checks import max_rel_err
+from ukhip import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from ukhip import device
+    device.lib()
+    return device
+
+
+def _errors(dev, probs, iters):
+    b = dev.Batch(probs)
+    try:
+        b.smooth("oddeven")
+        ref = [b.download(i, want_cov=False)[0] for i in range(len(probs))]
+        errs = []
+        for it in iters:
+            b.smooth_mixed(it)
+            e = max(max_rel_err(list(b.download(i, want_cov=False)[0]), list(ref[i])) for i in range(len(probs)))
+            errs.append(e)
+            say(f"  {it} refinement steps: max error {e:.2e}")
+        return errs
+    finally:
+        b.free()
+
+
+@pytest.mark.parametrize("case", ["rot6", "n48m24", "n64m32"])
+def test_refinement_reaches_fp64(dev, case):
+    if case == "rot6":
+        F = P.random_orthogonal(P.rng_for(21), 6)
+        pr = P.iid_problem(4097, 6, 6, seed=22, F=F, K_kind="W", C_kind="W", name="rot6_k4097")
+        iters, final = (0, 1, 2), 2
+    elif case == "n48m24":
+        F = 0.99 * P.random_orthogonal(P.rng_for(23), 48)
+        pr = P.iid_problem(1025, 48, 24, seed=24, F=F, G_scale=1 / np.sqrt(48.0), Cvar=0.1,
+                           unobserved=lambda i: i % 10 == 9, name="n48m24_k1025")
+        iters, final = (0, 1, 2, 3), 3
+    else:  # ill-conditioned: F = Q diag(logspace(0, -3, 64)), K = 1e-2 I
+        F = P.random_orthogonal(P.rng_for(25), 64) @ np.diag(np.logspace(0, -3, 64))
+        pr = P.iid_problem(257, 64, 32, seed=26, F=F, G_scale=1 / 8.0, Kvar=1e-2, name="n64m32_k257")
+        iters, final = (0, 2, 4, 6, 8), 8
+    say(f"mixed precision {pr.name}")
+    errs = _errors(dev, [pr], iters)
+    assert errs[0] > 1e-9  # FP32 alone is not FP64-accurate
+    for a, b in zip(errs, errs[1:]):
+        assert b < a or b < 1e-13, errs
+    assert errs[iters.index(final)] < TOL, errs
+
+
+def test_mixed_batch_of_dimensions(dev, oracle):
+    specs = P.batch_specs(count=24, seed=5, lmin=8, lmax=300)
+    probs = [P.batch_problem(s) for s in specs]
+    say(f"mixed-precision batch: {len(probs)} problems")
+    errs = _errors(dev, probs, (3,))
+    assert errs[0] < TOL
+    b = dev.Batch(probs)
+    b.smooth_mixed(3)
+    oracle.lib.oracle_set_threads(min(16, os.cpu_count() or 1))
+    try:
+        for i, pr in enumerate(probs[:6]):
+            x, _ = b.download(i, want_cov=False)
+            ox, _ = oracle.oddeven(pr)
+            assert max_rel_err(list(x), ox) < TOL, pr.name
+    finally:
+        oracle.lib.oracle_set_threads(1)
+        b.free()

@@ -46,12 +46,42 @@ __device__ __forceinline__ double dpp_xor2(double v) {
   hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-template <int SL>
-__device__ __forceinline__ double group_sum(double v) {
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+template <int SL, class T>
+__device__ __forceinline__ T group_sum(T v) {
   v += dpp_xor1(v);
   if (SL == 4) v += dpp_xor2(v);
   return v;
 }
+
+/* Device views of a whitened chain and of the back-substitution factors in
+ * the working precision T (double: the layouts of uk_chain_t and
+ * uk_factors_t; float: the same layouts in single precision). */
+template <class T>
+struct ChainT {
+  T *B, *D, *c, *C, *o;
+  int32_t mr;
+};
+template <class T>
+struct FactorsT {
+  T *R, *Bt, *Y, *cf;
+};
+/* 2-vector of T for paired LDS reads */
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
 
 __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
   int lo = 0, hi = nprob - 1;
@@ -65,24 +95,24 @@ __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob,
 
 /* One matrix column as held by one of its SL lanes: rows r = SL*l + sub of
  * block 0 in locals [0,H) and of block 1 in [H,2H), H = NB/SL. */
-template <int NB, int SL>
+template <int NB, int SL, class T = double>
 struct Col {
   static constexpr int H = NB / SL;
-  double a[2 * H];
+  T a[2 * H];
 
   __device__ __forceinline__ void zero_block(int blk) {
 #pragma unroll
-    for (int l = 0; l < H; ++l) a[blk * H + l] = 0.0;
+    for (int l = 0; l < H; ++l) a[blk * H + l] = T(0);
   }
   /* block blk <- rows [0, rows) of a column at src (unit stride), zero elsewhere */
-  __device__ __forceinline__ void load_block(int blk, const double* __restrict__ src, int rows, int sub) {
+  __device__ __forceinline__ void load_block(int blk, const T* __restrict__ src, int rows, int sub) {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
-      a[blk * H + l] = (r < rows) ? src[r] : 0.0;
+      a[blk * H + l] = (r < rows) ? src[r] : T(0);
     }
   }
-  __device__ __forceinline__ void store_block(int blk, double* __restrict__ dst, int rows, int sub) const {
+  __device__ __forceinline__ void store_block(int blk, T* __restrict__ dst, int rows, int sub) const {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
@@ -90,11 +120,11 @@ struct Col {
     }
   }
   /* upper-triangular store of block 0: rows > col are written as zero */
-  __device__ __forceinline__ void store_triu(double* __restrict__ dst, int rows, int col, int sub) const {
+  __device__ __forceinline__ void store_triu(T* __restrict__ dst, int rows, int col, int sub) const {
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
-      if (r < rows) dst[r] = (r <= col) ? a[l] : 0.0;
+      if (r < rows) dst[r] = (r <= col) ? a[l] : T(0);
     }
   }
   __device__ __forceinline__ void move_0_to_1() {
@@ -105,11 +135,11 @@ struct Col {
 
 /* LDS reflector buffers: [buffer][sub][2H + 4] (the +4 skews the sub-lane
  * arrays onto different banks), then [buffer][2] scalars. */
-template <int NB, int SL>
+template <int NB, int SL, class T = double>
 struct Refl {
   static constexpr int SUBSTRIDE = 2 * (NB / SL) + 4;
-  double v[2][SL * SUBSTRIDE];
-  double sc[2][2];
+  T v[2][SL * SUBSTRIDE];
+  T sc[2][2];
 };
 
 /*
@@ -121,19 +151,19 @@ struct Refl {
  * without branches (entries outside the active rows are exact zeros), so the
  * LDS reads of v issue back to back. One barrier.
  */
-template <int NB, int SL>
-__device__ __forceinline__ void house_step(Col<NB, SL>& col, Refl<NB, SL>& R, int buf, int sub, bool is_piv, bool upd,
+template <int NB, int SL, class T>
+__device__ __forceinline__ void house_step(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, bool is_piv, bool upd,
                                            int k) {
   constexpr int H = NB / SL;
-  double* vv = &R.v[buf][sub * Refl<NB, SL>::SUBSTRIDE];
+  T* vv = &R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE];
   if (is_piv) {
-    double s0 = 0.0, s1 = 0.0, alpha = 0.0;
+    T s0 = T(0), s1 = T(0), alpha = T(0);
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
-      const double x = col.a[l];
+      const T x = col.a[l];
       alpha = (r == k) ? x : alpha;
-      const double xm = (r > k) ? x : 0.0;
+      const T xm = (r > k) ? x : T(0);
       if (l & 1) s1 = fma(xm, xm, s1);
       else s0 = fma(xm, xm, s0);
     }
@@ -142,18 +172,18 @@ __device__ __forceinline__ void house_step(Col<NB, SL>& col, Refl<NB, SL>& R, in
       if (l & 1) s1 = fma(col.a[l], col.a[l], s1);
       else s0 = fma(col.a[l], col.a[l], s0);
     }
-    const double s = group_sum<SL>(s0 + s1);
+    const T s = group_sum<SL>(s0 + s1);
     alpha = group_sum<SL>(alpha);
-    double tau = 0.0, beta = alpha, scale = 0.0;
-    if (s != 0.0) {
+    T tau = T(0), beta = alpha, scale = T(0);
+    if (s != T(0)) {
       beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
       tau = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
+      scale = T(1) / (alpha - beta);
     }
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
-      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? 1.0 : 0.0);
+      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
     }
 #pragma unroll
     for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l] * scale;
@@ -164,19 +194,20 @@ __device__ __forceinline__ void house_step(Col<NB, SL>& col, Refl<NB, SL>& R, in
   }
   __syncthreads();
   if (upd) {
-    const double tau = R.sc[buf][0];
-    if (tau != 0.0) {
+    const T tau = R.sc[buf][0];
+    if (tau != T(0)) {
       /* v is read from LDS in chunks of CH locals issued back to back */
       constexpr int NL = 2 * H;
       constexpr int CH = NL <= 24 ? NL : 16;
-      const double2* v2 = reinterpret_cast<const double2*>(vv);
-      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+      using V2 = typename Vec2<T>::type;
+      const V2* v2 = reinterpret_cast<const V2*>(vv);
+      T d0 = T(0), d1 = T(0), d2 = T(0), d3 = T(0);
 #pragma unroll
       for (int c0 = 0; c0 < NL; c0 += CH) {
-        double vr[CH];
+        T vr[CH];
 #pragma unroll
         for (int l = 0; l < CH; l += 2) {
-          const double2 t = v2[(c0 + l) / 2];
+          const V2 t = v2[(c0 + l) / 2];
           vr[l] = t.x;
           vr[l + 1] = t.y;
         }
@@ -188,21 +219,21 @@ __device__ __forceinline__ void house_step(Col<NB, SL>& col, Refl<NB, SL>& R, in
           if (l + 3 < CH) d3 = fma(vr[l + 3], col.a[c0 + l + 3], d3);
         }
         if (CH == NL) { /* whole reflector in registers: update without re-reading */
-          const double d = group_sum<SL>((d0 + d1) + (d2 + d3));
-          const double tw = tau * d;
+          const T d = group_sum<SL>((d0 + d1) + (d2 + d3));
+          const T tw = tau * d;
 #pragma unroll
           for (int l = 0; l < NL; ++l) col.a[l] = fma(-tw, vr[l], col.a[l]);
           return;
         }
       }
-      const double d = group_sum<SL>((d0 + d1) + (d2 + d3));
-      const double tw = tau * d;
+      const T d = group_sum<SL>((d0 + d1) + (d2 + d3));
+      const T tw = tau * d;
 #pragma unroll
       for (int c0 = 0; c0 < NL; c0 += CH) {
-        double vr[CH];
+        T vr[CH];
 #pragma unroll
         for (int l = 0; l < CH; l += 2) {
-          const double2 t = v2[(c0 + l) / 2];
+          const V2 t = v2[(c0 + l) / 2];
           vr[l] = t.x;
           vr[l + 1] = t.y;
         }
@@ -222,16 +253,16 @@ struct QRGeom {
   static constexpr int NT = ((SL * COLS + 63) / 64) * 64;
 };
 
-template <int NB, int SL>
-__device__ __forceinline__ void house_qr(Col<NB, SL>& col, Refl<NB, SL>& R, int mycol, int sub, int piv0, int n,
+template <int NB, int SL, class T>
+__device__ __forceinline__ void house_qr(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int mycol, int sub, int piv0, int n,
                                          bool upd) {
-  for (int k = 0; k < n; ++k) house_step<NB, SL>(col, R, k & 1, sub, mycol == piv0 + k, upd, k);
+  for (int k = 0; k < n; ++k) house_step<NB, SL, T>(col, R, k & 1, sub, mycol == piv0 + k, upd, k);
   __syncthreads();
 }
 
-template <int NB, int SL, int NT>
-__device__ __forceinline__ void refl_clear(Refl<NB, SL>& R, int tid) {
-  for (int i = tid; i < (int)(sizeof(R) / sizeof(double)); i += NT) reinterpret_cast<double*>(&R)[i] = 0.0;
+template <int NB, int SL, int NT, class T>
+__device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
+  for (int i = tid; i < (int)(sizeof(R) / sizeof(T)); i += NT) reinterpret_cast<T*>(&R)[i] = T(0);
   __syncthreads();
 }
 
@@ -254,12 +285,12 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL>& R, int tid) {
  * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
-template <int NB>
+template <int NB, class T>
 __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
-    k_phase1(uk_level_t lv, uk_chain_t in, uk_chain_t out, uk_factors_t fac, int n) {
+    k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
   constexpr int SL = QRGeom<NB, 2 * NB + 1>::SL;
   constexpr int NT = QRGeom<NB, 2 * NB + 1>::NT;
-  __shared__ Refl<NB, SL> refl;
+  __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
@@ -273,9 +304,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
   const bool lone = j + 1 == L;
   const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
   const int qc = isP ? c : (isQ ? c - n : 0);
-  refl_clear<NB, SL, NT>(refl, tid);
+  refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL> col;
+  Col<NB, SL, T> col;
   const int64_t sj = s0 + j, sp = s0 + j + 1;
   if (isP) {
     if (lone) col.zero_block(0);
@@ -291,12 +322,12 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
     else col.load_block(0, in.c + sp * n, n, sub);
     col.load_block(1, in.o + sj * n, mr, sub);
   }
-  house_qr<NB, SL>(col, refl, c, sub, 0, n, isP || isQ || isR1);
+  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isP || isQ || isR1);
   if (isP) {
     col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
     for (int r = sub; r < n; r += SL) {
-      if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = 0.0;
-      if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = 0.0;
+      if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = T(0);
+      if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = T(0);
     }
   }
   if (isQ) {
@@ -309,12 +340,12 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
   }
 }
 
-template <int NB>
+template <int NB, class T>
 __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
-    k_phase2(uk_level_t lv, uk_chain_t in, uk_chain_t out, uk_factors_t fac, double* __restrict__ fix, int n) {
+    k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
   constexpr int SL = QRGeom<NB, 3 * NB + 1>::SL;
   constexpr int NT = QRGeom<NB, 3 * NB + 1>::NT;
-  __shared__ Refl<NB, SL> refl;
+  __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
@@ -327,9 +358,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
   const bool lone = j + 1 == L;
   const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
   const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
-  refl_clear<NB, SL, NT>(refl, tid);
+  refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL> col;
+  Col<NB, SL, T> col;
   const int64_t sj = lv.step_start[p] + j;
   if (isP) {
     col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
@@ -347,8 +378,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
     col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
     col.zero_block(1);
   }
-  house_qr<NB, SL>(col, refl, c, sub, 0, n, isP || isQ || isR1 || isBl);
-  double* fz = fix + (int64_t)p * (nn + n);
+  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isP || isQ || isR1 || isBl);
+  T* fz = fix + (int64_t)p * (nn + n);
   if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
   if (isBl) {
     col.store_block(0, fac.Bt + e * nn + (int64_t)qc * n, n, sub);
@@ -364,12 +395,12 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
   }
 }
 
-template <int NB>
+template <int NB, class T>
 __global__ void __launch_bounds__((QRGeom<NB, NB + 1>::NT))
-    k_phase3(uk_level_t lv, uk_chain_t in, uk_chain_t out, const double* __restrict__ fix, int n) {
+    k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
   constexpr int SL = QRGeom<NB, NB + 1>::SL;
   constexpr int NT = QRGeom<NB, NB + 1>::NT;
-  __shared__ Refl<NB, SL> refl;
+  __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t unit = blockIdx.x;
   const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
@@ -383,9 +414,9 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1>::NT))
   const bool do_fix = (L & 1) && (u == L / 2 - 1);
   const bool isQ = c < n, isR1 = c == n; /* Q columns numbered from 0 here */
   const int qc = isQ ? c : 0;
-  refl_clear<NB, SL, NT>(refl, tid);
+  refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL> col;
+  Col<NB, SL, T> col;
   const int64_t sp = lv.step_start[p] + j + 1;
   if (isQ) {
     col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
@@ -395,12 +426,12 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1>::NT))
     col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
     col.load_block(1, in.o + sp * n, mr, sub);
   }
-  house_qr<NB, SL>(col, refl, c, sub, 0, n, isQ || isR1);
+  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isQ || isR1);
   if (do_fix) {
-    const double* fz = fix + (int64_t)p * (nn + n);
+    const T* fz = fix + (int64_t)p * (nn + n);
     if (isQ) col.load_block(1, fz + (int64_t)qc * n, n, sub);
     if (isR1) col.load_block(1, fz + nn, n, sub);
-    house_qr<NB, SL>(col, refl, c, sub, 0, n, isQ || isR1);
+    house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isQ || isR1);
   }
   if (isQ) col.store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
   if (isR1) col.store_block(0, out.o + t * n, n, sub);
@@ -798,22 +829,22 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
   return hipGetLastError();
 }
 
-/* ev (NULL or 4 events) is recorded before phase 1 and after each phase. */
-extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
-                                          const uk_factors_t* fac, double* fix, int n, hipStream_t st,
-                                          hipEvent_t* ev) {
-  const int nb = uk_bucket(n > in->mr ? n : in->mr);
+namespace {
+template <class T>
+hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac,
+                            T* fix, int n, hipStream_t st, hipEvent_t* ev) {
+  const int nb = uk_bucket(n > in.mr ? n : in.mr);
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
 #define L_(NBV)                                                                                                      \
   mark(0);                                                                                                           \
-  k_phase1<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, n); \
+  k_phase1<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1>::NT), 0, st>>>(*lv, in, out, fac, n);  \
   mark(1);                                                                                                           \
-  k_phase2<NBV><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1>::NT), 0, st>>>(*lv, *in, *out, *fac, fix, \
-                                                                                        n);                        \
+  k_phase2<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1>::NT), 0, st>>>(*lv, in, out, fac, fix, \
+                                                                                           n);                     \
   mark(2);                                                                                                           \
-  k_phase3<NBV><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, *in, *out, fix, n);      \
+  k_phase3<NBV, T><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, in, out, fix, n);      \
   mark(3)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
@@ -821,6 +852,26 @@ extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t
   }
 #undef L_
   return hipGetLastError();
+}
+}  // namespace
+
+/* ev (NULL or 4 events) is recorded before phase 1 and after each phase. */
+extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
+                                          const uk_factors_t* fac, double* fix, int n, hipStream_t st,
+                                          hipEvent_t* ev) {
+  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr};
+  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr};
+  const FactorsT<double> f{fac->R, fac->Bt, fac->Y, fac->cf};
+  return launch_eliminate<double>(lv, ci, co, f, fix, n, st, ev);
+}
+
+/* The same elimination in single precision (chains, factors and arithmetic). */
+extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
+                                            const uk_factors32_t* fac, float* fix, int n, hipStream_t st) {
+  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr};
+  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr};
+  const FactorsT<float> f{fac->R, fac->Bt, fac->Y, fac->cf};
+  return launch_eliminate<float>(lv, ci, co, f, fix, n, st, nullptr);
 }
 
 extern "C" hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,

@@ -1,0 +1,374 @@
+/*
+ * refine.hip -- the mixed-precision odd-even smoother: the elimination runs in
+ * FP32 (k_phase1..3<NB, float>), and FP64 accuracy of the estimates is
+ * recovered by iterative refinement on the least-squares problem
+ * min || A u - b || of the whitened step equations (the system the
+ * reference's odd-even smoother solves, c/kalman_oddeven_smoother.c:379-975).
+ *
+ * The odd-even elimination is an orthogonal factorization of the permuted
+ * block matrix, A P = Q Rs, Rs block upper triangular with one block row
+ * per even step j of every level l:  R_j u_j + B~_j u_{j-1} + Y_j u_{j+1}
+ * (u_{j-1}, u_{j+1} are the odd steps, eliminated at later levels). So
+ * Rs^T Rs = P^T A^T A P, and with the FP32 factors one refinement step is a
+ * corrected semi-normal-equations step:
+ *   r = b - A x                (FP64, whitened chain kept in FP64)
+ *   g = A^T r                  (k_normal)
+ *   Rs^T y = g                 (forward, level 0 -> top: k_fwd_even, k_fwd_odd)
+ *   Rs d = y                   (backward, top -> level 0: k_solve_est)
+ *   x <- x + d                 (k_add)
+ * The sweeps compute in FP64 and read the FP32 factors. The first x comes
+ * from the FP32 transformed right-hand side (k_solve_est on cf).
+ * Each step multiplies the error by about cond(A)^2 eps_32.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "ukhip_internal.h"
+
+namespace {
+
+template <class T>
+struct FactorPtrs {
+  const T* R;
+  const T* Bt;
+  const T* Y;
+};
+
+__device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+/* dst[i] = (float) src[i] */
+__global__ void __launch_bounds__(256) k_to32(float* __restrict__ dst, const double* __restrict__ src, int64_t count) {
+  for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (float)src[i];
+}
+
+/* x[i] += d[i] */
+__global__ void __launch_bounds__(256) k_add(double* __restrict__ x, const double* __restrict__ d, int64_t count) {
+  for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) x[i] += d[i];
+}
+
+/*
+ * Residual and normal-equation right-hand side of step s (one workgroup):
+ *   re_s = c_s - B_s x_{s-1} - D_s x_s   (not on a problem's first step)
+ *   ro_s = o_s - C_s x_s
+ *   gA_s = D_s^T re_s + C_s^T ro_s,   gB_s = B_s^T re_s (belongs to g_{s-1}).
+ * Blocks are staged through LDS (odd leading dimension) so that HBM is read
+ * along columns and the transposed products read LDS.
+ */
+template <int NB>
+__global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* first, const int32_t* m_obs,
+                                                const double* __restrict__ x, double* __restrict__ gA,
+                                                double* __restrict__ gB, int n) {
+  constexpr int L = NB + 1;
+  __shared__ double M1[NB * L], M2[NB * L];
+  __shared__ double xs[NB], xm[NB], re[NB], ro[NB];
+  const int tid = threadIdx.x;
+  const int64_t s = blockIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  const int mr = ch.mr;
+  const bool evo = first[s] == 0;
+  const int m = m_obs[s];
+  for (int i = tid; i < n; i += 256) {
+    xs[i] = x[s * n + i];
+    xm[i] = evo ? x[(s - 1) * n + i] : 0.0;
+  }
+  if (evo) {
+    for (int idx = tid; idx < n * n; idx += 256) {
+      const int i = idx % n, j = idx / n;
+      M1[i + j * L] = ch.D[s * nn + idx];
+      M2[i + j * L] = ch.B[s * nn + idx];
+    }
+  }
+  __syncthreads();
+  if (tid < n) {
+    double r = 0.0;
+    if (evo) {
+      r = ch.c[s * n + tid];
+      for (int j = 0; j < n; ++j) r -= M1[tid + j * L] * xs[j] + M2[tid + j * L] * xm[j];
+    }
+    re[tid] = r;
+  }
+  __syncthreads();
+  double ga = 0.0, gb = 0.0;
+  if (tid < n && evo)
+    for (int r = 0; r < n; ++r) {
+      ga = fma(M1[r + tid * L], re[r], ga);
+      gb = fma(M2[r + tid * L], re[r], gb);
+    }
+  __syncthreads();
+  if (m > 0) {
+    for (int idx = tid; idx < mr * n; idx += 256) {
+      const int i = idx % mr, j = idx / mr;
+      M1[i + j * L] = ch.C[s * nn + idx]; /* C is mr x n, ld mr; rows >= m are zero */
+    }
+    __syncthreads();
+    if (tid < mr) {
+      double r = ch.o[s * n + tid];
+      for (int j = 0; j < n; ++j) r -= M1[tid + j * L] * xs[j];
+      ro[tid] = tid < m ? r : 0.0;
+    }
+    __syncthreads();
+    if (tid < n)
+      for (int r = 0; r < mr; ++r) ga = fma(M1[r + tid * L], ro[r], ga);
+  }
+  if (tid < n) {
+    gA[s * n + tid] = ga;
+    gB[s * n + tid] = gb;
+  }
+}
+
+/* g_s = gA_s + gB_{s+1} (when step s+1 belongs to the same problem) */
+__global__ void __launch_bounds__(256) k_gather(const double* __restrict__ gA, const double* __restrict__ gB,
+                                                const int8_t* first, double* __restrict__ g, int64_t steps, int n) {
+  const int64_t total = steps * n;
+  for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256) {
+    const int64_t s = i / n;
+    const bool nxt = s + 1 < steps && first[s + 1] == 0;
+    g[i] = gA[i] + (nxt ? gB[i + n] : 0.0);
+  }
+}
+
+/*
+ * Forward sweep, even steps of a level (one wave per even step):
+ * y_j = R_j^{-T} g_j. R is staged through LDS; lane i holds row i.
+ */
+template <int NB, class T>
+__global__ void __launch_bounds__(64) k_fwd_even(uk_level_t lv, const T* __restrict__ R, const double* __restrict__ g,
+                                                 double* __restrict__ y, int n) {
+  constexpr int L = NB + 1;
+  __shared__ double Rs[NB * L];
+  const int lane = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t j = 2 * (e - lv.even_start[p]);
+  const int64_t sj = lv.step_start[p] + j;
+  const int64_t nn = (int64_t)n * n;
+  {
+    constexpr int PER = (NB * NB + 63) / 64;
+    T rr[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int idx = lane + 64 * t;
+      rr[t] = idx < n * n ? R[e * nn + idx] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int idx = lane + 64 * t;
+      if (idx < n * n) Rs[idx % n + (idx / n) * L] = (double)rr[t];
+    }
+  }
+  double v = lane < n ? g[sj * n + lane] : 0.0;
+  __syncthreads();
+  /* R^T is lower triangular: (R^T)_{ik} = R_{ki} */
+  for (int k = 0; k < n; ++k) {
+    const double yk = __shfl(v, k) / Rs[k + k * L];
+    if (lane == k) v = yk;
+    else if (lane > k && lane < n) v = fma(-Rs[k + lane * L], yk, v);
+  }
+  if (lane < n) y[e * n + lane] = v;
+}
+
+/*
+ * Forward sweep, odd steps (one wave per pair u, odd step j + 1 = 2u + 1):
+ * the next level's right-hand side  g'_{j+1} = g_{j+1} - Y_j^T y_j - B~_{j+2}^T y_{j+2}.
+ * Y_j and B~_{j+2} are staged through LDS with coalesced loads (all in
+ * flight together); lane i then forms entry i from column i.
+ */
+template <int NB, class T>
+__global__ void __launch_bounds__(64) k_fwd_odd(uk_level_t lv, const T* __restrict__ Bt, const T* __restrict__ Y,
+                                                const double* __restrict__ g, const double* __restrict__ y,
+                                                double* __restrict__ gnext, int n) {
+  constexpr int L = NB + 1;
+  constexpr int PER = (NB * NB + 63) / 64;
+  __shared__ T Ys[NB * L], Bs[NB * L];
+  __shared__ double yj[NB], yn[NB];
+  const int lane = threadIdx.x;
+  const int64_t unit = blockIdx.x;
+  const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
+  const int64_t Lp = lv.L[p];
+  if (Lp == 1) return;
+  const int64_t u = unit - lv.unit_start[p];
+  const int64_t j = 2 * u;
+  const int64_t e = lv.even_start[p] + u; /* slot of even step j */
+  const bool has_n = j + 2 < Lp;          /* even step j + 2 exists */
+  const int64_t nn = (int64_t)n * n;
+  T ry[PER], rb[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int idx = lane + 64 * t;
+    const bool in = idx < n * n;
+    ry[t] = in ? Y[e * nn + idx] : T(0);
+    rb[t] = (in && has_n) ? Bt[(e + 1) * nn + idx] : T(0);
+  }
+  if (lane < n) {
+    yj[lane] = y[e * n + lane];
+    yn[lane] = has_n ? y[(e + 1) * n + lane] : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int idx = lane + 64 * t;
+    if (idx < n * n) {
+      const int r = idx % n, c = idx / n;
+      Ys[r + c * L] = ry[t];
+      Bs[r + c * L] = rb[t];
+    }
+  }
+  __syncthreads();
+  if (lane < n) {
+    double v0 = g[(lv.step_start[p] + j + 1) * n + lane], v1 = 0.0;
+    for (int r = 0; r < n; ++r) {
+      v0 = fma(-(double)Ys[r + lane * L], yj[r], v0);
+      v1 = fma(-(double)Bs[r + lane * L], yn[r], v1);
+    }
+    gnext[(lv.next_start[p] + u) * n + lane] = v0 + v1;
+  }
+}
+
+/*
+ * Backward sweep (one wave per even step): x_j = R^{-1} (y_j - B~ x_{j-1} - Y x_{j+1})
+ * with the neighbours' x from the levels above, in the original step order.
+ * The three blocks are staged through LDS with coalesced loads first.
+ */
+template <int NB, class T, class TR>
+__global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> fac, const TR* __restrict__ rhs,
+                                                  double* __restrict__ x, int n) {
+  constexpr int L = NB + 1;
+  constexpr int PER = (NB * NB + 63) / 64;
+  __shared__ T Rs[NB * L], Bs[NB * L], Ys[NB * L];
+  __shared__ double xm[NB], xp[NB];
+  const int lane = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t j = 2 * (e - lv.even_start[p]);
+  const int64_t Lp = lv.L[p];
+  const bool has_m = j > 0, has_p = j + 1 < Lp;
+  const int64_t nn = (int64_t)n * n;
+  const int64_t ob = lv.orig_start[lv.prob[p]];
+  const int64_t stride = (int64_t)1 << lv.level;
+  const int64_t oj = ob + (j + 1) * stride - 1;
+  const int64_t om = ob + j * stride - 1;
+  const int64_t op = ob + (j + 2) * stride - 1;
+  {
+    T rr[PER], rb[PER], ry[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int idx = lane + 64 * t;
+      const bool in = idx < n * n;
+      rr[t] = in ? fac.R[e * nn + idx] : T(0);
+      rb[t] = (in && has_m) ? fac.Bt[e * nn + idx] : T(0);
+      ry[t] = (in && has_p) ? fac.Y[e * nn + idx] : T(0);
+    }
+    if (lane < n) {
+      xm[lane] = has_m ? x[om * n + lane] : 0.0;
+      xp[lane] = has_p ? x[op * n + lane] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int idx = lane + 64 * t;
+      if (idx < n * n) {
+        const int r = idx % n, c = idx / n;
+        Rs[r + c * L] = rr[t];
+        Bs[r + c * L] = rb[t];
+        Ys[r + c * L] = ry[t];
+      }
+    }
+  }
+  __syncthreads();
+  double v = 0.0;
+  if (lane < n) {
+    double v1 = 0.0;
+    v = (double)rhs[e * n + lane];
+    for (int q = 0; q < n; ++q) {
+      v = fma(-(double)Bs[lane + q * L], xm[q], v);
+      v1 = fma(-(double)Ys[lane + q * L], xp[q], v1);
+    }
+    v += v1;
+  }
+  for (int q = n - 1; q >= 0; --q) {
+    const double xq = __shfl(v, q) / (double)Rs[q + q * L];
+    if (lane == q) v = xq;
+    else if (lane < q) v = fma(-(double)Rs[lane + q * L], xq, v);
+  }
+  if (lane < n) x[oj * n + lane] = v;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------------ */
+/* launchers                                                                  */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int uk_bucket(int n);
+
+#define UK_DISPATCH(NBV, CALL) \
+  case NBV: { CALL(NBV); } break;
+
+extern "C" hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 65536);
+  k_to32<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(dst, src, count);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t uk_launch_add(double* x, const double* d, int64_t count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 65536);
+  k_add<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(x, d, count);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first, const int32_t* m_obs,
+                                       const double* x, double* gA, double* gB, double* g, int64_t steps, int n,
+                                       hipStream_t st) {
+  const int nb = uk_bucket(n > ch->mr ? n : ch->mr);
+#define L_(NBV) k_normal<NBV><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n)
+  switch (nb) {
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    default: return hipErrorInvalidValue;
+  }
+#undef L_
+  const int64_t blocks = std::min<int64_t>((steps * n + 255) / 256, 65536);
+  k_gather<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(gA, gB, first, g, steps, n);
+  return hipGetLastError();
+}
+
+/* forward sweep of one level: y (per even slot of the level) and the next
+ * level's right-hand side */
+extern "C" hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* fac, const double* g, double* y,
+                                    double* gnext, int n, hipStream_t st) {
+  const int nb = uk_bucket(n);
+#define L_(NBV)                                                                                          \
+  k_fwd_even<NBV, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, fac->R, g, y, n);          \
+  k_fwd_odd<NBV, float><<<dim3((unsigned)lv->units), dim3(64), 0, st>>>(*lv, fac->Bt, fac->Y, g, y, gnext, n)
+  switch (nb) {
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    default: return hipErrorInvalidValue;
+  }
+#undef L_
+  return hipGetLastError();
+}
+
+/* backward sweep of one level; rhs32 (the FP32 transformed right-hand side)
+ * or rhs64 (the forward sweep's y) */
+extern "C" hipError_t uk_launch_solve_est(const uk_level_t* lv, const uk_factors32_t* fac, const float* rhs32,
+                                          const double* rhs64, double* x, int n, hipStream_t st) {
+  const int nb = uk_bucket(n);
+  const FactorPtrs<float> f{fac->R, fac->Bt, fac->Y};
+#define L_(NBV)                                                                                            \
+  if (rhs32) k_solve_est<NBV, float, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
+  else k_solve_est<NBV, float, double><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
+  switch (nb) {
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    default: return hipErrorInvalidValue;
+  }
+#undef L_
+  return hipGetLastError();
+}

@@ -32,6 +32,16 @@ hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, doub
                              hipStream_t st);
 int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options);
 void uk_set_error(const char* fmt, ...);
+hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
+                                 const uk_factors32_t* fac, float* fix, int n, hipStream_t st);
+hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st);
+hipError_t uk_launch_add(double* x, const double* d, int64_t count, hipStream_t st);
+hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first, const int32_t* m_obs, const double* x,
+                            double* gA, double* gB, double* g, int64_t steps, int n, hipStream_t st);
+hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* fac, const double* g, double* y, double* gnext,
+                         int n, hipStream_t st);
+hipError_t uk_launch_solve_est(const uk_level_t* lv, const uk_factors32_t* fac, const float* rhs32,
+                               const double* rhs64, double* x, int n, hipStream_t st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -125,6 +135,13 @@ struct ukhip_batch {
   std::vector<hipStream_t> part_streams;
   /* captured odd-even smoothing, [0] without and [1] with covariances */
   hipGraphExec_t graph[2]{};
+  /* mixed-precision workspace, allocated on first use: FP32 chains and
+   * factors, FP64 right-hand sides and corrections */
+  void* d_mixed = nullptr;
+  uk_chain32_t chain32[2]{};
+  uk_factors32_t fac32{};
+  float* fix32 = nullptr;
+  double *gA = nullptr, *gB = nullptr, *gbuf[2]{}, *ys = nullptr, *dx = nullptr;
 };
 
 static int cov_code(char t) {
@@ -236,6 +253,7 @@ static void batch_release(ukhip_batch* b) {
   if (b->d_work) (void)hipFree(b->d_work);
   if (b->d_tab) (void)hipFree(b->d_tab);
   if (b->d_assoc) (void)hipFree(b->d_assoc);
+  if (b->d_mixed) (void)hipFree(b->d_mixed);
   delete b;
 }
 
@@ -809,6 +827,135 @@ extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t optio
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   return ok ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mixed precision: FP32 elimination + FP64 refinement (refine.hip)          */
+/* ------------------------------------------------------------------------ */
+
+static bool mixed_workspace(ukhip_batch* b) {
+  if (b->d_mixed) return true;
+  const size_t n = b->n, nn = n * n, S = (size_t)b->steps;
+  const size_t L1 = b->levels.size() > 1 ? (size_t)b->levels[1].steps : 1;
+  const size_t E = (size_t)b->total_evens;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t ch[2][5];
+  for (int cb = 0; cb < 2; ++cb) {
+    const size_t cs = cb == 0 ? S : std::max<size_t>(L1, 1);
+    ch[cb][0] = take(cs * nn * 4);
+    ch[cb][1] = take(cs * nn * 4);
+    ch[cb][2] = take(cs * n * 4);
+    ch[cb][3] = take(cs * nn * 4);
+    ch[cb][4] = take(cs * n * 4);
+  }
+  const size_t fR = take(E * nn * 4), fB = take(E * nn * 4), fY = take(E * nn * 4), fc = take(E * n * 4);
+  const size_t ofx = take((size_t)b->nprob * (nn + n) * 4);
+  const size_t ogA = take(S * n * 8), ogB = take(S * n * 8), og0 = take(S * n * 8),
+               og1 = take(std::max<size_t>(L1, 1) * n * 8), oys = take(E * n * 8), odx = take(S * n * 8);
+  if (!hip_ok(hipMalloc(&b->d_mixed, o), "hipMalloc(mixed-precision workspace)")) {
+    b->d_mixed = nullptr;
+    return false;
+  }
+  char* w = (char*)b->d_mixed;
+  for (int cb = 0; cb < 2; ++cb) {
+    b->chain32[cb].B = (float*)(w + ch[cb][0]);
+    b->chain32[cb].D = (float*)(w + ch[cb][1]);
+    b->chain32[cb].c = (float*)(w + ch[cb][2]);
+    b->chain32[cb].C = (float*)(w + ch[cb][3]);
+    b->chain32[cb].o = (float*)(w + ch[cb][4]);
+  }
+  b->fac32.R = (float*)(w + fR);
+  b->fac32.Bt = (float*)(w + fB);
+  b->fac32.Y = (float*)(w + fY);
+  b->fac32.cf = (float*)(w + fc);
+  b->fix32 = (float*)(w + ofx);
+  b->gA = (double*)(w + ogA);
+  b->gB = (double*)(w + ogB);
+  b->gbuf[0] = (double*)(w + og0);
+  b->gbuf[1] = (double*)(w + og1);
+  b->ys = (double*)(w + oys);
+  b->dx = (double*)(w + odx);
+  return true;
+}
+
+static uk_factors32_t fac32_at(const ukhip_batch* b, int l) {
+  uk_factors32_t f = b->fac32;
+  const int64_t fb = b->levels[l].fac_base;
+  const size_t nn = (size_t)b->n * b->n;
+  f.R += fb * nn;
+  f.Bt += fb * nn;
+  f.Y += fb * nn;
+  f.cf += fb * b->n;
+  return f;
+}
+
+/* Estimates only, in the original step order of xout. */
+static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
+  if (!mixed_workspace(b)) return false;
+  const int n = b->n, nl = (int)b->levels.size();
+  const size_t nn = (size_t)n * n, S = (size_t)b->steps;
+  if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
+  /* the level-0 chain in FP32 (the FP64 chain stays for the residuals) */
+  const uk_chain_t& c64 = b->chain[0];
+  uk_chain32_t c32 = b->chain32[0];
+  if (!hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
+      !hip_ok(uk_launch_to32(c32.D, c64.D, S * nn, st), "to32") ||
+      !hip_ok(uk_launch_to32(c32.c, c64.c, S * n, st), "to32") ||
+      !hip_ok(uk_launch_to32(c32.C, c64.C, S * nn, st), "to32") ||
+      !hip_ok(uk_launch_to32(c32.o, c64.o, S * n, st), "to32"))
+    return false;
+  for (int l = 0; l < nl; ++l) {
+    uk_chain32_t in = b->chain32[l & 1];
+    in.mr = l == 0 ? b->mr : n;
+    uk_chain32_t out = b->chain32[(l + 1) & 1];
+    out.mr = n;
+    const uk_factors32_t f = fac32_at(b, l);
+    uk_level_t lv = b->levels[l].dev;
+    if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;
+  }
+  /* x0 from the FP32 transformed right-hand side */
+  for (int l = nl - 1; l >= 0; --l) {
+    const uk_factors32_t f = fac32_at(b, l);
+    uk_level_t lv = b->levels[l].dev;
+    if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
+  }
+  uk_chain_t r64 = b->chain[0];
+  r64.mr = b->mr;
+  for (int it = 0; it < refinements; ++it) {
+    if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->raw.m, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),
+                "residual launch"))
+      return false;
+    for (int l = 0; l < nl; ++l) {
+      const uk_factors32_t f = fac32_at(b, l);
+      uk_level_t lv = b->levels[l].dev;
+      double* y = b->ys + b->levels[l].fac_base * n;
+      if (!hip_ok(uk_launch_fwd(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st), "forward sweep launch"))
+        return false;
+    }
+    for (int l = nl - 1; l >= 0; --l) {
+      const uk_factors32_t f = fac32_at(b, l);
+      uk_level_t lv = b->levels[l].dev;
+      const double* y = b->ys + b->levels[l].fac_base * n;
+      if (!hip_ok(uk_launch_solve_est(&lv, &f, nullptr, y, b->dx, n, st), "backward sweep launch")) return false;
+    }
+    if (!hip_ok(uk_launch_add(b->xout, b->dx, (int64_t)(S * n), st), "update launch")) return false;
+  }
+  return true;
+}
+
+extern "C" int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements) {
+  if (!b) { uk_set_error("NULL batch"); return -1; }
+  if (!ensure_stream()) return -1;
+  if (refinements < 0 || refinements > 64) { uk_set_error("refinements must be in 0..64"); return -1; }
+  if (b->parts.empty()) return run_mixed(b, refinements, g_stream) ? 0 : -1;
+  for (auto* q : b->parts)
+    if (!run_mixed(q, refinements, g_stream)) return -1;
+  return 0;
 }
 
 extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covs) {

@@ -51,6 +51,16 @@ typedef struct {
   int32_t mr;
 } uk_chain_t;
 
+/* The same chain in single precision (the FP32 elimination). */
+typedef struct {
+  float* B;
+  float* D;
+  float* c;
+  float* C;
+  float* o;
+  int32_t mr;
+} uk_chain32_t;
+
 /* Per-level factors kept for the back substitution (one slot per even step):
  *   R_j u_j + Bt_j u_{j-1} + Y_j u_{j+1} = cf_j */
 typedef struct {
@@ -59,6 +69,13 @@ typedef struct {
   double* Y;
   double* cf;
 } uk_factors_t;
+
+typedef struct {
+  float* R;
+  float* Bt;
+  float* Y;
+  float* cf;
+} uk_factors32_t;
 
 /* Level plan: problem p of the batch has L[p] steps at this level. Work units
  * of the elimination are numbered unit_start[p] .. unit_start[p+1]-1; even

@@ -54,6 +54,8 @@ def lib():
         L.ukhip_synchronize.restype = C.c_int
         L.ukhip_stream.restype = C.c_void_p
         L.ukhip_use_graphs.argtypes = [C.c_int]
+        L.ukhip_batch_smooth_oddeven_mixed.argtypes = [C.c_void_p, C.c_int32]
+        L.ukhip_batch_smooth_oddeven_mixed.restype = C.c_int
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
@@ -146,6 +148,14 @@ class Batch:
         L = lib()
         fn = L.ukhip_batch_smooth_oddeven if algorithm == "oddeven" else L.ukhip_batch_smooth_associative
         _check(fn(self.h, options), f"smooth_{algorithm}")
+        if sync:
+            _check(L.ukhip_synchronize(), "synchronize")
+
+    def smooth_mixed(self, refinements=3, sync=True):
+        """Estimates by FP32 elimination plus `refinements` FP64 refinement
+        steps (covariances untouched)."""
+        L = lib()
+        _check(L.ukhip_batch_smooth_oddeven_mixed(self.h, refinements), "smooth_mixed")
         if sync:
             _check(L.ukhip_synchronize(), "synchronize")
 
