@@ -87,6 +87,34 @@ enum { UKHIP_OE_WEIGH = 0, UKHIP_OE_PHASE1, UKHIP_OE_PHASE2, UKHIP_OE_PHASE3, UK
 int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
                                      int32_t launches[UKHIP_OE_SLOTS]);
 
+/* ---- serving: continuous batching of smoothing requests -------------------
+ * A server owns a device memory arena (the HBM budget) and a scheduler
+ * thread. Submitted requests (their inputs are copied) wait in arrival
+ * order; whenever a stream slot and arena memory are free, the scheduler
+ * takes the oldest request and every queued request of the same state
+ * dimension that still fits, uploads them as one batch and runs it, while
+ * earlier batches are still on the device. A result depends only on its own
+ * request: it is the same alone and in any mix. */
+typedef struct ukhip_server ukhip_server_t;
+typedef struct {
+  size_t hbm_budget_bytes;   /* device arena; 0: 40% of the free device memory */
+  int32_t max_batch;         /* requests per batch; 0: 256 */
+  int32_t max_inflight;      /* batches on the device at once (one stream each); 0: 4 */
+  int32_t mixed_refinements; /* < 0: FP64 odd-even with covariances; >= 0: FP32 elimination plus that many
+                                FP64 refinement steps, estimates only (covariances read as NaN) */
+} ukhip_server_options_t;
+ukhip_server_t* ukhip_server_create(const ukhip_server_options_t* options);
+/* Returns the request id (>= 0), or -1. */
+int64_t ukhip_server_submit(ukhip_server_t* s, const ukhip_problem_t* problem);
+/* 1 done, 0 pending, -1 failed or unknown. */
+int ukhip_server_poll(ukhip_server_t* s, int64_t id);
+/* Blocks until request id is done; copies its estimates (k*n) and
+ * covariances (k*n*n, may be NULL) out and gives the time from submission to
+ * completion; forgets the request. 0 or -1 (ukhip_last_error). */
+int ukhip_server_wait(ukhip_server_t* s, int64_t id, double* states, double* covariances, double* latency_ms);
+/* Finishes every submitted request, then frees the server. */
+void ukhip_server_destroy(ukhip_server_t* s);
+
 /* The HIP stream (hipStream_t) every libukhip launch uses. */
 void* ukhip_stream(void);
 

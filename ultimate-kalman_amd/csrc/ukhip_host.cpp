@@ -11,11 +11,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kalman.h"
@@ -105,6 +111,62 @@ struct LevelHost {
   uk_level_t dev{};
 };
 
+/* A device memory arena: one allocation carved first-fit into 256-byte
+ * aligned blocks. Batches created for the server take their memory from it,
+ * so admitting and retiring batches never calls hipMalloc/hipFree (which
+ * would synchronize with the work of other batches in flight). */
+struct DevArena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_blocks; /* offset -> size */
+  std::map<size_t, size_t> used;        /* offset -> size */
+  std::mutex mu;
+
+  void* alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+      if (it->second >= bytes) {
+        const size_t off = it->first, sz = it->second;
+        free_blocks.erase(it);
+        if (sz > bytes) free_blocks[off + bytes] = sz - bytes;
+        used[off] = bytes;
+        return base + off;
+      }
+    }
+    return nullptr;
+  }
+  void release(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu);
+    const size_t off = (size_t)((char*)p - base);
+    auto u = used.find(off);
+    if (u == used.end()) return;
+    size_t start = off, sz = u->second;
+    used.erase(u);
+    auto nx = free_blocks.lower_bound(start);
+    if (nx != free_blocks.end() && nx->first == start + sz) {
+      sz += nx->second;
+      nx = free_blocks.erase(nx);
+    }
+    if (nx != free_blocks.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == start) {
+        start = pv->first;
+        sz += pv->second;
+        free_blocks.erase(pv);
+      }
+    }
+    free_blocks[start] = sz;
+  }
+  size_t in_use() {
+    std::lock_guard<std::mutex> g(mu);
+    size_t t = 0;
+    for (auto& kv : used) t += kv.second;
+    return t;
+  }
+};
+
 struct ukhip_batch {
   int n = 0, mr = 0, nprob = 0;
   int64_t steps = 0;
@@ -142,7 +204,27 @@ struct ukhip_batch {
   uk_factors32_t fac32{};
   float* fix32 = nullptr;
   double *gA = nullptr, *gB = nullptr, *gbuf[2]{}, *ys = nullptr, *dx = nullptr;
+  /* memory source (NULL: hipMalloc) and the host copy of the level tables
+   * (kept until an asynchronous upload has completed) */
+  DevArena* arena = nullptr;
+  std::vector<int64_t> tab_host;
 };
+
+static void* dev_alloc(ukhip_batch* b, size_t bytes, const char* what) {
+  if (b->arena) {
+    void* p = b->arena->alloc(bytes);
+    if (!p) uk_set_error("%s: %zu bytes do not fit in the device arena", what, bytes);
+    return p;
+  }
+  void* p = nullptr;
+  if (!hip_ok(hipMalloc(&p, bytes), what)) return nullptr;
+  return p;
+}
+static void dev_free(ukhip_batch* b, void* p) {
+  if (!p) return;
+  if (b->arena) b->arena->release(p);
+  else (void)hipFree(p);
+}
 
 static int cov_code(char t) {
   switch (t) {
@@ -156,7 +238,47 @@ static int cov_code(char t) {
 }
 
 /* Host staging of the raw inputs in the device layout of uk_raw_t. */
+/* Pinned host buffers reused by size class (powers of two from 1 MiB), so
+ * that a steady stream of batches does not allocate and free pinned memory
+ * (hipHostFree can wait for the device). */
+struct PinnedPool {
+  std::mutex mu;
+  std::map<size_t, std::vector<void*>> free_;
+  static size_t cls(size_t bytes) {
+    size_t c = (size_t)1 << 20;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  void* get(size_t bytes) {
+    const size_t c = cls(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto& v = free_[c];
+      if (!v.empty()) {
+        void* p = v.back();
+        v.pop_back();
+        return p;
+      }
+    }
+    void* p = nullptr;
+    if (!hip_ok(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc(pinned pool)")) return nullptr;
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu);
+    free_[cls(bytes)].push_back(p);
+  }
+  void clear() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& kv : free_)
+      for (void* p : kv.second) (void)hipHostFree(p);
+    free_.clear();
+  }
+};
+
 struct Staging {
+  PinnedPool* pool = nullptr; /* host buffer from this pool, or hipHostMalloc */
   int n = 0, mr = 0;
   int64_t steps = 0;
   int H_ident = 1, K_full = 0, C_full = 0;
@@ -249,17 +371,22 @@ static void batch_release(ukhip_batch* b) {
   for (auto st : b->part_streams) (void)hipStreamDestroy(st);
   for (auto g : b->graph)
     if (g) (void)hipGraphExecDestroy(g);
-  if (b->d_raw) (void)hipFree(b->d_raw);
-  if (b->d_work) (void)hipFree(b->d_work);
-  if (b->d_tab) (void)hipFree(b->d_tab);
+  dev_free(b, b->d_raw);
+  dev_free(b, b->d_work);
+  dev_free(b, b->d_tab);
   if (b->d_assoc) (void)hipFree(b->d_assoc);
-  if (b->d_mixed) (void)hipFree(b->d_mixed);
+  dev_free(b, b->d_mixed);
   delete b;
 }
 
-/* Allocates device memory, uploads staging, builds level tables. */
-static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& lengths) {
+/* Allocates device memory (from `arena`, or hipMalloc), uploads staging and
+ * builds the level tables, on stream `us`; with `sync` false the upload may
+ * still be in flight on return (st.host and b->tab_host must outlive it). */
+static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& lengths, hipStream_t us = nullptr,
+                                       DevArena* arena = nullptr, bool sync = true) {
+  if (!us) us = g_stream;
   ukhip_batch* b = new ukhip_batch();
+  b->arena = arena;
   b->n = st.n;
   b->mr = st.mr;
   b->nprob = (int)lengths.size();
@@ -276,9 +403,10 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   const int n = st.n;
   const size_t nn = (size_t)n * n, S = (size_t)st.steps;
   /* raw inputs */
-  if (!hip_ok(hipMalloc(&b->d_raw, st.bytes), "hipMalloc(raw inputs)")) { batch_release(b); return nullptr; }
+  b->d_raw = dev_alloc(b, st.bytes, "device memory for the raw inputs");
+  if (!b->d_raw) { batch_release(b); return nullptr; }
   b->raw_bytes = st.bytes;
-  if (!hip_ok(hipMemcpyAsync(b->d_raw, st.host, st.bytes, hipMemcpyHostToDevice, g_stream), "upload")) {
+  if (!hip_ok(hipMemcpyAsync(b->d_raw, st.host, st.bytes, hipMemcpyHostToDevice, us), "upload")) {
     batch_release(b);
     return nullptr;
   }
@@ -332,7 +460,8 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   const size_t xoo = take(S * n * 8), Soo = take(S * nn * 8);
   const size_t fxo = take((size_t)b->nprob * (nn + n) * 8);
   b->work_bytes = o;
-  if (!hip_ok(hipMalloc(&b->d_work, o), "hipMalloc(workspace)")) { batch_release(b); return nullptr; }
+  b->d_work = dev_alloc(b, o, "device memory for the workspace");
+  if (!b->d_work) { batch_release(b); return nullptr; }
   char* w = (char*)b->d_work;
   for (int cb = 0; cb < 2; ++cb) {
     b->chain[cb].B = (double*)(w + chain_off[cb][0]);
@@ -384,9 +513,12 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
     for (auto& lv : b->levels)
       for (int32_t v : lv.prob) p32[at++] = v;
   }
-  if (!hip_ok(hipMalloc(&b->d_tab, tab.size() * 8), "hipMalloc(level tables)")) { batch_release(b); return nullptr; }
-  if (!hip_ok(hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, g_stream), "upload tables") ||
-      !hip_ok(hipStreamSynchronize(g_stream), "upload")) {
+  b->d_tab = dev_alloc(b, tab.size() * 8, "device memory for the level tables");
+  if (!b->d_tab) { batch_release(b); return nullptr; }
+  b->tab_host.swap(tab);
+  if (!hip_ok(hipMemcpyAsync(b->d_tab, b->tab_host.data(), b->tab_host.size() * 8, hipMemcpyHostToDevice, us),
+              "upload tables") ||
+      (sync && !hip_ok(hipStreamSynchronize(us), "upload"))) {
     batch_release(b);
     return nullptr;
   }
@@ -418,10 +550,17 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
 
 static bool staging_alloc(Staging& st) {
   st.layout();
+  if (st.pool) {
+    st.host = (char*)st.pool->get(st.bytes);
+    return st.host != nullptr;
+  }
   return hip_ok(hipHostMalloc((void**)&st.host, st.bytes, hipHostMallocDefault), "hipHostMalloc(staging)");
 }
 static void staging_free(Staging& st) {
-  if (st.host) (void)hipHostFree(st.host);
+  if (st.host) {
+    if (st.pool) st.pool->put(st.host, st.bytes);
+    else (void)hipHostFree(st.host);
+  }
   st.host = nullptr;
 }
 
@@ -857,10 +996,8 @@ static bool mixed_workspace(ukhip_batch* b) {
   const size_t ofx = take((size_t)b->nprob * (nn + n) * 4);
   const size_t ogA = take(S * n * 8), ogB = take(S * n * 8), og0 = take(S * n * 8),
                og1 = take(std::max<size_t>(L1, 1) * n * 8), oys = take(E * n * 8), odx = take(S * n * 8);
-  if (!hip_ok(hipMalloc(&b->d_mixed, o), "hipMalloc(mixed-precision workspace)")) {
-    b->d_mixed = nullptr;
-    return false;
-  }
+  b->d_mixed = dev_alloc(b, o, "device memory for the mixed-precision workspace");
+  if (!b->d_mixed) return false;
   char* w = (char*)b->d_mixed;
   for (int cb = 0; cb < 2; ++cb) {
     b->chain32[cb].B = (float*)(w + ch[cb][0]);
@@ -1048,4 +1185,335 @@ extern "C" void kalman_smooth_oddeven(kalman_options_t options, kalman_step_equa
 extern "C" void kalman_smooth_associative(kalman_options_t options, kalman_step_equations_t** equations,
                                           kalman_step_index_t length) {
   smooth_equations(options, equations, length, true);
+}
+
+/* ------------------------------------------------------------------------ */
+/* serving: continuous batching of smoothing requests                        */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+/* One request: its own copy of the inputs (packed as ukhip_problem_t), its
+ * results and timestamps. */
+struct Request {
+  int64_t id = 0;
+  ukhip_problem_t prob{};
+  std::vector<double> F, H, c, K, G, o, C;
+  std::vector<int32_t> m_i;
+  std::vector<double> states, covs;
+  Clock::time_point t_submit, t_done;
+  int status = 0; /* 0 queued, 1 running, 2 done, -1 failed */
+  std::string error;
+};
+
+/* A batch on the device: its requests, stream slot, staging and results. */
+struct InFlight {
+  ukhip_batch* b = nullptr;
+  Staging st;
+  std::vector<Request*> reqs;
+  int slot = 0;
+  hipEvent_t done = nullptr;
+  double* host_x = nullptr; /* pinned: steps * n */
+  double* host_S = nullptr; /* pinned: steps * n * n (FP64 mode) */
+  size_t bytes = 0;
+  int64_t b_steps = 0;
+};
+
+/* Device bytes of a batch of requests of state dimension n (the layout of
+ * batch_from_staging and, in mixed mode, mixed_workspace), with slack. */
+size_t batch_device_bytes(int n, int mr, int64_t steps, bool mixed) {
+  const double nn = (double)n * n;
+  double per = 8.0 * (14.0 * nn + 10.0 * n + 4.0 * mr + mr * (double)n + (double)mr * mr);
+  if (mixed) per += 4.0 * (7.5 * nn + 4 * n) + 8.0 * (6.0 * n);
+  return (size_t)(per * (double)steps * 1.05) + (size_t)(4u << 20);
+}
+
+}  // namespace
+
+struct ukhip_server {
+  ukhip_server_options_t opt{};
+  DevArena arena;
+  PinnedPool pinned;
+  std::vector<hipStream_t> streams;
+  std::vector<bool> slot_busy;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::map<int64_t, Request*> all;
+  std::vector<InFlight*> flight;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  int64_t next_id = 0;
+  size_t in_flight_bytes = 0;
+};
+
+static void server_fail(ukhip_server* s, Request* r, const std::string& why) {
+  r->status = -1;
+  r->error = why;
+  r->t_done = Clock::now();
+  (void)s;
+}
+
+/* Starts one batch from the queue (lock held by the caller). Returns false
+ * when nothing could be admitted. */
+static bool server_admit(ukhip_server* s) {
+  if (s->queue.empty()) return false;
+  int slot = -1;
+  for (size_t i = 0; i < s->slot_busy.size(); ++i)
+    if (!s->slot_busy[i]) { slot = (int)i; break; }
+  if (slot < 0) return false;
+  const bool mixed = s->opt.mixed_refinements >= 0;
+  /* the oldest request fixes the state dimension; then every queued request
+   * of that dimension in arrival order, while the batch fits the budget */
+  const int n = s->queue.front()->prob.n;
+  std::vector<Request*> take;
+  int64_t steps = 0;
+  int mr = 1;
+  for (auto it = s->queue.begin(); it != s->queue.end() && (int)take.size() < s->opt.max_batch;) {
+    Request* r = *it;
+    if (r->prob.n != n) { ++it; continue; }
+    const int mr2 = std::max(mr, r->prob.m);
+    const size_t need = batch_device_bytes(n, mr2, steps + r->prob.k, mixed);
+    if (s->in_flight_bytes + need > s->arena.size) {
+      if (take.empty() && s->flight.empty()) { /* alone and still too large: reject it */
+        server_fail(s, r, "request exceeds the device memory budget of the server");
+        it = s->queue.erase(it);
+        s->cv_done.notify_all();
+        continue;
+      }
+      break;
+    }
+    take.push_back(r);
+    steps += r->prob.k;
+    mr = mr2;
+    it = s->queue.erase(it);
+  }
+  if (take.empty()) return false;
+  InFlight* f = new InFlight();
+  f->reqs = take;
+  f->slot = slot;
+  hipStream_t st = s->streams[slot];
+  std::vector<ukhip_problem_t> probs;
+  for (auto* r : take) {
+    probs.push_back(r->prob);
+    r->status = 1;
+  }
+  std::vector<int64_t> lengths;
+  f->st.pool = &s->pinned;
+  bool ok = pack_problems((int32_t)probs.size(), probs.data(), f->st, lengths);
+  if (ok) {
+    f->bytes = batch_device_bytes(n, f->st.mr, f->st.steps, mixed);
+    f->b = batch_from_staging(f->st, lengths, st, &s->arena, false);
+    ok = f->b != nullptr;
+  }
+  if (ok) ok = mixed ? run_mixed(f->b, s->opt.mixed_refinements, st) : run_oddeven(f->b, 1, nullptr, st);
+  const size_t S = (size_t)f->st.steps, nn = (size_t)n * n;
+  if (ok) ok = (f->host_x = (double*)s->pinned.get(S * n * 8)) != nullptr;
+  if (ok && !mixed) ok = (f->host_S = (double*)s->pinned.get(S * nn * 8)) != nullptr;
+  if (ok) ok = hip_ok(hipMemcpyAsync(f->host_x, f->b->xout, S * n * 8, hipMemcpyDeviceToHost, st), "download");
+  if (ok && !mixed)
+    ok = hip_ok(hipMemcpyAsync(f->host_S, f->b->Sout, S * nn * 8, hipMemcpyDeviceToHost, st), "download");
+  if (ok) ok = hip_ok(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreate") &&
+               hip_ok(hipEventRecord(f->done, st), "hipEventRecord");
+  if (!ok) {
+    const std::string why = ukhip_last_error();
+    const bool arena_full = why.find("device arena") != std::string::npos;
+    (void)hipStreamSynchronize(st);
+    if (f->b) batch_release(f->b);
+    staging_free(f->st);
+    s->pinned.put(f->host_x, S * n * 8);
+    s->pinned.put(f->host_S, S * nn * 8);
+    if (f->done) (void)hipEventDestroy(f->done);
+    delete f;
+    if (arena_full && !s->flight.empty()) { /* fragmented arena: retry after a batch retires */
+      for (auto it = take.rbegin(); it != take.rend(); ++it) {
+        (*it)->status = 0;
+        s->queue.push_front(*it);
+      }
+      return false;
+    }
+    for (auto* r : take) server_fail(s, r, why);
+    s->cv_done.notify_all();
+    return true;
+  }
+  f->b_steps = f->st.steps;
+  s->slot_busy[slot] = true;
+  s->in_flight_bytes += f->bytes;
+  s->flight.push_back(f);
+  return true;
+}
+
+/* Retires finished batches (lock held by the caller). */
+static bool server_reap(ukhip_server* s) {
+  bool any = false;
+  for (size_t i = 0; i < s->flight.size();) {
+    InFlight* f = s->flight[i];
+    const hipError_t q = hipEventQuery(f->done);
+    if (q == hipErrorNotReady) { ++i; continue; }
+    const bool ok = q == hipSuccess;
+    const int n = f->b->n;
+    const size_t nn = (size_t)n * n;
+    for (size_t p = 0; p < f->reqs.size(); ++p) {
+      Request* r = f->reqs[p];
+      if (!ok) { server_fail(s, r, hipGetErrorString(q)); continue; }
+      const int64_t o = f->b->orig_start[p], k = f->b->k[p];
+      r->states.assign(f->host_x + o * n, f->host_x + (o + k) * n);
+      if (f->host_S) r->covs.assign(f->host_S + o * nn, f->host_S + (o + k) * nn);
+      r->t_done = Clock::now();
+      r->status = 2;
+    }
+    batch_release(f->b);
+    staging_free(f->st);
+    const size_t S = (size_t)f->b_steps;
+    s->pinned.put(f->host_x, S * n * 8);
+    s->pinned.put(f->host_S, S * nn * 8);
+    (void)hipEventDestroy(f->done);
+    s->slot_busy[f->slot] = false;
+    s->in_flight_bytes -= f->bytes;
+    s->flight.erase(s->flight.begin() + i);
+    delete f;
+    any = true;
+  }
+  return any;
+}
+
+static void server_loop(ukhip_server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    const bool reaped = server_reap(s);
+    if (reaped) s->cv_done.notify_all();
+    while (server_admit(s)) {
+    }
+    if (s->stop && s->queue.empty() && s->flight.empty()) break;
+    /* in flight: poll the events at a short interval; idle: sleep until a submission */
+    if (!s->flight.empty()) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    else s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+  }
+}
+
+extern "C" ukhip_server_t* ukhip_server_create(const ukhip_server_options_t* options) {
+  if (!ensure_stream()) return nullptr;
+  ukhip_server* s = new ukhip_server();
+  if (options) s->opt = *options;
+  if (s->opt.max_batch <= 0) s->opt.max_batch = 256;
+  if (s->opt.max_inflight <= 0) s->opt.max_inflight = 4;
+  size_t budget = s->opt.hbm_budget_bytes;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (!hip_ok(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) { delete s; return nullptr; }
+    budget = fr / 5 * 2;
+  }
+  if (!hip_ok(hipMalloc((void**)&s->arena.base, budget), "hipMalloc(server arena)")) { delete s; return nullptr; }
+  s->arena.size = budget;
+  s->arena.free_blocks[0] = budget;
+  for (int i = 0; i < s->opt.max_inflight; ++i) {
+    hipStream_t st = nullptr;
+    if (!hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate")) break;
+    s->streams.push_back(st);
+  }
+  if ((int)s->streams.size() != s->opt.max_inflight) {
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    (void)hipFree(s->arena.base);
+    delete s;
+    return nullptr;
+  }
+  s->slot_busy.assign(s->streams.size(), false);
+  s->worker = std::thread(server_loop, s);
+  return s;
+}
+
+template <class T>
+static void copy_in(std::vector<T>& dst, const T* src, size_t count) {
+  if (src) dst.assign(src, src + count);
+  else dst.clear();
+}
+
+extern "C" int64_t ukhip_server_submit(ukhip_server_t* s, const ukhip_problem_t* p) {
+  if (!s || !p) { uk_set_error("NULL server or problem"); return -1; }
+  if (p->n < 1 || p->n > UKHIP_MAX_N || p->k < 1 || p->m < 0 || p->m > p->n) {
+    uk_set_error("ukhip_server_submit: need 1 <= n <= %d, k >= 1, 0 <= m <= n", UKHIP_MAX_N);
+    return -1;
+  }
+  Request* r = new Request();
+  const size_t n = p->n, m = p->m, k = p->k;
+  copy_in(r->F, p->F, (p->F_shared ? 1 : k) * n * n);
+  copy_in(r->H, p->H, k * n * n);
+  copy_in(r->c, p->c, k * n);
+  copy_in(r->K, p->K, k * (p->K_type == 'w' ? n : n * n));
+  copy_in(r->G, p->G, k * m * n);
+  copy_in(r->o, p->o, k * m);
+  copy_in(r->C, p->C, k * (p->C_type == 'w' ? m : m * m));
+  copy_in(r->m_i, p->m_i, k);
+  r->prob = *p;
+  r->prob.F = r->F.empty() ? nullptr : r->F.data();
+  r->prob.H = r->H.empty() ? nullptr : r->H.data();
+  r->prob.c = r->c.empty() ? nullptr : r->c.data();
+  r->prob.K = r->K.empty() ? nullptr : r->K.data();
+  r->prob.G = r->G.empty() ? nullptr : r->G.data();
+  r->prob.o = r->o.empty() ? nullptr : r->o.data();
+  r->prob.C = r->C.empty() ? nullptr : r->C.data();
+  r->prob.m_i = r->m_i.empty() ? nullptr : r->m_i.data();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    r->id = s->next_id++;
+    r->t_submit = Clock::now();
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+  }
+  s->cv_work.notify_one();
+  return r->id;
+}
+
+extern "C" int ukhip_server_poll(ukhip_server_t* s, int64_t id) {
+  if (!s) return -1;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) return -1;
+  return it->second->status == 2 ? 1 : (it->second->status < 0 ? -1 : 0);
+}
+
+extern "C" int ukhip_server_wait(ukhip_server_t* s, int64_t id, double* states, double* covs, double* latency_ms) {
+  if (!s) { uk_set_error("NULL server"); return -1; }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) { uk_set_error("unknown request %lld", (long long)id); return -1; }
+  Request* r = it->second;
+  s->cv_done.wait(lk, [&] { return r->status == 2 || r->status < 0; });
+  s->all.erase(it);
+  lk.unlock();
+  int rc = 0;
+  if (r->status < 0) {
+    uk_set_error("request %lld: %s", (long long)id, r->error.c_str());
+    rc = -1;
+  } else {
+    if (states) std::memcpy(states, r->states.data(), r->states.size() * 8);
+    if (covs) {
+      if (r->covs.empty()) {
+        const size_t cnt = (size_t)r->prob.k * r->prob.n * r->prob.n;
+        for (size_t i = 0; i < cnt; ++i) covs[i] = kalman_nan;
+      } else {
+        std::memcpy(covs, r->covs.data(), r->covs.size() * 8);
+      }
+    }
+  }
+  if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  delete r;
+  return rc;
+}
+
+extern "C" void ukhip_server_destroy(ukhip_server_t* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& kv : s->all) delete kv.second;
+  for (auto st : s->streams) (void)hipStreamDestroy(st);
+  (void)hipFree(s->arena.base);
+  s->pinned.clear();
+  delete s;
 }

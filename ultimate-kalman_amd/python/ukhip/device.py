@@ -56,6 +56,14 @@ def lib():
         L.ukhip_use_graphs.argtypes = [C.c_int]
         L.ukhip_batch_smooth_oddeven_mixed.argtypes = [C.c_void_p, C.c_int32]
         L.ukhip_batch_smooth_oddeven_mixed.restype = C.c_int
+        L.ukhip_server_create.restype = C.c_void_p
+        L.ukhip_server_create.argtypes = [C.c_void_p]
+        L.ukhip_server_submit.restype = C.c_int64
+        L.ukhip_server_submit.argtypes = [C.c_void_p, C.POINTER(ProblemStruct)]
+        L.ukhip_server_poll.argtypes = [C.c_void_p, C.c_int64]
+        L.ukhip_server_wait.restype = C.c_int
+        L.ukhip_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        L.ukhip_server_destroy.argtypes = [C.c_void_p]
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
@@ -89,6 +97,100 @@ def smooth_equations(problem, algorithm="oddeven", options=0):
     return states, covs
 
 
+class _Keep:
+    """Holds the numpy buffers a ProblemStruct points into."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def ptr(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.arrays.append(a)
+        return a.ctypes.data
+
+
+def problem_struct(p, keep):
+    """ukhip_problem_t of a packed Problem (buffers kept alive by `keep`)."""
+    pk = p.packed
+    n, m, k = pk["n"], pk["m"], p.k
+    s = ProblemStruct()
+    s.n, s.m, s.k = n, m, k
+    if pk.get("F_shared") is not None:
+        s.F = keep.ptr(np.asfortranarray(pk["F_shared"]).T.ravel())  # column-major n x n
+        s.F_shared = 1
+    else:
+        s.F = keep.ptr(pk["F"])
+    s.H = None
+    s.c = keep.ptr(pk["c"])
+    K = pk["K"]
+    s.K_type = pk["K_type"].encode()
+    s.K = keep.ptr(np.broadcast_to(np.asfortranarray(K).T.ravel(), (k, K.size)))
+    s.G = keep.ptr(pk["G"])
+    s.o = keep.ptr(pk["o"])
+    mi = np.where(pk["observed"], m, 0).astype(np.int32)
+    keep.arrays.append(mi)
+    s.m_i = mi.ctypes.data
+    Cm = pk["C"]
+    s.C_type = pk["C_type"].encode()
+    s.C = keep.ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
+    return s
+
+
+class ServerOptions(C.Structure):
+    """ukhip_server_options_t (include/ukhip.h)."""
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+                ("mixed_refinements", C.c_int32)]
+
+
+class Server:
+    """Continuous-batching smoothing server (ukhip_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=0, max_batch=256, max_inflight=4, mixed_refinements=-1):
+        L = lib()
+        L.ukhip_clear_error()
+        o = ServerOptions(hbm_budget_bytes, max_batch, max_inflight, mixed_refinements)
+        self.h = L.ukhip_server_create(C.byref(o))
+        if not self.h:
+            raise UkhipError("ukhip_server_create: " + last_error())
+        self.mixed = mixed_refinements >= 0
+        self._shapes = {}
+
+    def submit(self, problem):
+        keep = _Keep()
+        s = problem_struct(problem, keep)
+        rid = lib().ukhip_server_submit(self.h, C.byref(s))  # inputs are copied by the call
+        if rid < 0:
+            raise UkhipError("ukhip_server_submit: " + last_error())
+        self._shapes[rid] = (problem.k, problem.packed["n"])
+        return rid
+
+    def wait(self, rid, want_cov=True):
+        """(states k x n, covariances k x n x n or None, latency in ms)."""
+        k, n = self._shapes.pop(rid)
+        x = np.empty((k, n))
+        cv = np.empty((k, n, n)) if want_cov else None
+        lat = C.c_double()
+        rc = lib().ukhip_server_wait(self.h, rid, x.ctypes.data, cv.ctypes.data if want_cov else None, C.byref(lat))
+        if rc != 0:
+            raise UkhipError(last_error())
+        if want_cov:
+            cv = np.swapaxes(cv, 1, 2)
+        return x, cv, lat.value
+
+    def close(self):
+        if self.h:
+            lib().ukhip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Device-resident set of problems with one state dimension."""
 
@@ -104,37 +206,11 @@ class Batch:
         if not self.h:
             raise UkhipError("ukhip_batch_create: " + last_error())
 
-    def _ptr(self, a):
-        if a is None:
-            return None
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        self._keep.append(a)
-        return a.ctypes.data
-
     def _struct(self, p):
-        pk = p.packed
-        n, m, k = pk["n"], pk["m"], p.k
-        s = ProblemStruct()
-        s.n, s.m, s.k = n, m, k
-        if pk.get("F_shared") is not None:
-            s.F = self._ptr(np.asfortranarray(pk["F_shared"]).T.ravel())  # column-major n x n
-            s.F_shared = 1
-        else:
-            s.F = self._ptr(pk["F"])
-        s.H = None
-        s.c = self._ptr(pk["c"])
-        K = pk["K"]
-        s.K_type = pk["K_type"].encode()
-        s.K = self._ptr(np.broadcast_to(np.asfortranarray(K).T.ravel(), (k, K.size)))
-        s.G = self._ptr(pk["G"])
-        s.o = self._ptr(pk["o"])
-        mi = np.where(pk["observed"], m, 0).astype(np.int32)
-        self._keep.append(mi)
-        s.m_i = mi.ctypes.data
-        Cm = pk["C"]
-        s.C_type = pk["C_type"].encode()
-        s.C = self._ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
-        return s
+        if not hasattr(self, "_kp"):
+            self._kp = _Keep()
+            self._keep.append(self._kp)
+        return problem_struct(p, self._kp)
 
     @property
     def steps(self):

@@ -283,6 +283,34 @@ def batch_problem(spec, lists=True):
                        name=spec["name"], lists=lists)
 
 
+def serve_specs(count, seed=5, lmin=100, lmax=50_000, nmin=3, nmax=16, unobserved=0.05):
+    """The serving workload's request list, without the data: n uniform over
+    {nmin..nmax}, m uniform over [1, n], length log-uniform in [lmin, lmax],
+    each step unobserved with probability `unobserved`."""
+    rng = rng_for(seed)
+    specs = []
+    for p in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        m = int(rng.integers(1, n + 1))
+        k = int(np.exp(rng.uniform(np.log(lmin), np.log(lmax))))
+        specs.append(dict(n=n, m=m, k=k, seed=int(rng.integers(1 << 62)), unobserved=unobserved,
+                          name=f"req{p}_n{n}_m{m}_k{k}"))
+    return specs
+
+
+def serve_problem(spec, lists=True):
+    """One request of serve_specs: F = 0.99 Q_n, G ~ N(0, 1/n) per step, K = I,
+    C = 0.1 I, i.i.d. N(0,1) observations; the first steps are observed until
+    they determine the state (so both smoothers accept the request)."""
+    n, m, k = spec["n"], spec["m"], spec["k"]
+    rng = rng_for(spec["seed"] ^ 0x3C3C)
+    unob = rng.random(k) < spec["unobserved"]
+    unob[: (n + m - 1) // m] = False
+    F = 0.99 * random_orthogonal(rng, n)
+    return iid_problem(k, n, m, seed=spec["seed"], F=F, G_scale=1.0 / np.sqrt(n), Kvar=1.0, Cvar=0.1,
+                       unobserved=lambda i: bool(unob[i]), name=spec["name"], lists=lists)
+
+
 def input_digest(problem):
     """SHA-256 over every input matrix of every step, in step order."""
     import hashlib
