@@ -159,6 +159,129 @@ def cpu_port_allcores(name, sample_k):
             "sample": f"k={sample_k}, {dt:.1f} s"}
 
 
+def step_rel_err(x, ref):
+    """Largest per-step normwise relative error, max_i |x_i - ref_i| / |ref_i|
+    (max norms); inf when x holds a non-finite value."""
+    if not np.all(np.isfinite(x)):
+        return float("inf")
+    d = np.max(np.abs(ref), axis=1)
+    e = np.max(np.abs(x - ref), axis=1)
+    return float(np.max(np.where(d > 0, e / np.where(d > 0, d, 1.0), e)))
+
+
+def _time(fn, steps, warmup):
+    for i in range(warmup):
+        fn()
+    device.lib().ukhip_synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn()
+    device.lib().ukhip_synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def mixed_precision_leg(b, timesteps, x64, args):
+    """FP32 elimination + FP64 refinement against plain FP64 odd-even on the
+    same resident batch: estimates only (the mixed path computes no
+    covariances, so its FP64 counterpart is KALMAN_NO_COVARIANCE). Errors are
+    per-step normwise relative errors of the estimates against the FP64
+    smoother's (x64; lists of per-problem arrays)."""
+    from ukhip import capi
+    ms64 = _time(lambda: b.smooth("oddeven", options=capi.KALMAN_NO_COVARIANCE, sync=False), args.steps, 1)
+    out = {"fp64_states_only": {"ms_per_step": ms64, "value": timesteps / (ms64 * 1e-3)}}
+    for r in args.refinements:
+        b.smooth_mixed(r)
+        err = max(step_rel_err(b.download(i, want_cov=False)[0], x64[i]) for i in range(len(x64)))
+        ms = _time(lambda: b.smooth_mixed(r, sync=False), args.steps, 1)
+        out[f"refine{r}"] = {"ms_per_step": ms, "value": timesteps / (ms * 1e-3),
+                             "speedup_vs_fp64_states_only": ms64 / ms, "max_rel_err_states": err}
+        print(f"[bench] mixed, {r} refinement(s): {ms:.2f} ms/step ({ms64 / ms:.2f}x FP64 states-only), "
+              f"max rel err {err:.2e}", file=sys.stderr, flush=True)
+    return out
+
+
+def precision_workload(args):
+    """The ill-conditioned precision workload (n=64, m=32, k=200000, F = Q
+    diag(logspace(0,-3,64)), K = 1e-2 I, C = I): FP64 with covariances, FP64
+    estimates only, and FP32 + FP64 refinement."""
+    pr = make_problem("n64m32_k200000", args.k)
+    b = device.Batch([pr])
+    ms_cov = _time(lambda: b.smooth("oddeven", sync=False), args.steps, 1)
+    x64 = [b.download(0, want_cov=False)[0]]
+    res = {"workload": WORKLOADS["n64m32_k200000"]["desc"] + ("" if args.k is None else f" (k = {pr.k})"),
+           "fp64_with_covariances": {"ms_per_step": ms_cov, "value": pr.k / (ms_cov * 1e-3)}}
+    res.update(mixed_precision_leg(b, pr.k, x64, args))
+    b.free()
+    return res
+
+
+def run_serving(args):
+    """The serving workload on a bounded sample: --serve-requests requests
+    (n uniform over {3..16}, m uniform over [1,n], lengths log-uniform in
+    [100, 50000], F = 0.99 Q_n, G ~ N(0,1/n), K = I, C = 0.1 I, 5% of steps
+    unobserved), drawn in turn from a pool of --serve-pool generated requests.
+    For each mode (FP64 with covariances; FP32 + refinement, estimates) the
+    stream is first submitted at once (capacity), then as a Poisson stream at
+    --serve-load times the FP64 capacity (requests per second; the same
+    arrival times for both modes). Latency is the server's own, from submit to
+    results ready on the host."""
+    import threading
+    specs = P.serve_specs(args.serve_requests, seed=5)
+    t = time.perf_counter()
+    pool = [P.serve_problem(s, lists=False) for s in specs[:args.serve_pool]]
+    reqs = [pool[i % len(pool)] for i in range(args.serve_requests)]
+    steps = sum(p.k for p in reqs)
+    print(f"[bench] serving: {len(reqs)} requests ({steps} steps) from a pool of {len(pool)}, generated in "
+          f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+
+    def stream(refine, arrivals):
+        srv = device.Server(hbm_budget_bytes=int(args.serve_budget_gb * 2**30), max_batch=args.serve_max_batch,
+                            max_inflight=args.serve_inflight, mixed_refinements=refine)
+        lat = [0.0] * len(reqs)
+        ids = [None] * len(reqs)
+        ready = [threading.Event() for _ in reqs]
+
+        def submit():
+            t0 = time.perf_counter()
+            for i, p in enumerate(reqs):
+                if arrivals is not None:
+                    d = t0 + arrivals[i] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                ids[i] = srv.submit(p)
+                ready[i].set()
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submit)
+        th.start()
+        for i in range(len(reqs)):
+            ready[i].wait()
+            lat[i] = srv.wait(ids[i], want_cov=False, want_states=False)[2]
+        wall = time.perf_counter() - t0
+        th.join()
+        srv.close()
+        lat = np.array(lat)
+        return {"timesteps_per_s": steps / wall, "requests_per_s": len(reqs) / wall, "wall_s": wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
+
+    out = {"requests": len(reqs), "pool": len(pool), "timesteps": steps, "hbm_budget_gb": args.serve_budget_gb,
+           "max_batch": args.serve_max_batch, "max_inflight": args.serve_inflight}
+    modes = (("fp64", -1), (f"mixed_refine{args.serve_refine}", args.serve_refine))
+    for name, refine in modes:
+        out[name] = {"all_at_once": stream(refine, None)}
+        print(f"[bench] serving {name}, all at once: {out[name]['all_at_once']}", file=sys.stderr, flush=True)
+    lam = args.serve_load * out["fp64"]["all_at_once"]["requests_per_s"]
+    gaps = P.rng_for(77).exponential(1.0 / lam, len(reqs))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    out["poisson_rate_requests_per_s"] = lam
+    out["offered_timesteps_per_s"] = lam * steps / len(reqs)
+    for name, refine in modes:
+        out[name]["poisson"] = stream(refine, arrivals)
+        print(f"[bench] serving {name}, Poisson: {out[name]['poisson']}", file=sys.stderr, flush=True)
+    return out
+
+
 def device_bytes_per_step(n, m):
     """libukhip's odd-even device memory per step (inputs, two chains, factors,
     cross covariances, outputs), in bytes."""
@@ -187,6 +310,7 @@ def run_batch(args):
     total_steps = sum(sp["k"] for sp in specs)
     ms_total, flops_total = 0.0, 0.0
     per_chunk = []
+    mixed_acc = {}
     for ci, ch in enumerate(chunks):
         t = time.perf_counter()
         probs = [P.batch_problem(sp, lists=False) for sp in ch]
@@ -202,8 +326,15 @@ def run_batch(args):
             b.smooth("oddeven", sync=False)
         device.lib().ukhip_synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        b.free()
         ksum = sum(sp["k"] for sp in ch)
+        if not args.no_mixed:
+            x64 = [b.download(i, want_cov=False)[0] for i in range(len(probs))]
+            leg = mixed_precision_leg(b, ksum, x64, args)
+            for key, v in leg.items():
+                acc = mixed_acc.setdefault(key, {"ms_per_step": 0.0, "max_rel_err_states": 0.0})
+                acc["ms_per_step"] += v["ms_per_step"]
+                acc["max_rel_err_states"] = max(acc["max_rel_err_states"], v.get("max_rel_err_states", 0.0))
+        b.free()
         for sp in ch:
             flops_total += _scaled_flops(sp)
         ms_total += ms
@@ -211,6 +342,12 @@ def run_batch(args):
         print(f"[bench] chunk {ci + 1}/{len(chunks)}: {len(ch)} problems, {ksum} steps, generated {tg:.1f}s, "
               f"uploaded {tu:.1f}s, {ms:.2f} ms per smoothing", file=sys.stderr, flush=True)
     value = total_steps / (ms_total * 1e-3)
+    for key, acc in mixed_acc.items():
+        acc["value"] = total_steps / (acc["ms_per_step"] * 1e-3)
+        if key != "fp64_states_only":
+            acc["speedup_vs_fp64_states_only"] = mixed_acc["fp64_states_only"]["ms_per_step"] / acc["ms_per_step"]
+        else:
+            del acc["max_rel_err_states"]
     return {
         "metric": METRIC, "value": value, "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_total, "higher_is_better": True, "scaling": "weak",
@@ -222,6 +359,7 @@ def run_batch(args):
                    "problems": args.batch_count, "timesteps": total_steps, "chunks": per_chunk,
                    "hbm_budget_gb": args.hbm_budget_gb},
         "whole_path_tflops": flops_total / (ms_total * 1e-3) / 1e12,
+        "mixed_precision": mixed_acc or None,
     }
 
 
@@ -254,6 +392,19 @@ def main():
     ap.add_argument("--no-associative", action="store_true", help="skip the associative smoother's timing")
     ap.add_argument("--assoc-steps", type=int, default=2)
     ap.add_argument("--cpu-sample-k-assoc", type=int, default=2048)
+    ap.add_argument("--refinements", type=lambda s: [int(v) for v in s.split(",")], default=[1, 2],
+                    help="mixed-precision legs: FP64 refinement steps after the FP32 elimination")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--no-precision", action="store_true", help="skip the n=64 precision workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-pool", type=int, default=250)
+    ap.add_argument("--serve-budget-gb", type=float, default=32.0)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-refine", type=int, default=2)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson rate as a fraction of the FP64 server's all-at-once request rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB device-to-device once (a plain copy kernel of "
                          "known size for calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
@@ -307,6 +458,11 @@ def main():
     if traffic is not None and dom == "k_backsub":
         traffic["algorithmic_bytes_per_step"] = flops.oddeven_backsub_bytes(n, pr.k)
 
+    mixed = None
+    if not args.no_mixed:
+        x64 = [b.download(0, want_cov=False)[0]]
+        mixed = mixed_precision_leg(b, pr.k, x64, args)
+
     assoc = None
     if not args.no_associative:
         for i in range(max(1, args.warmup // 2)):
@@ -355,8 +511,14 @@ def main():
         "algorithmic_flops_per_timestep": fl["total"] / pr.k,
         "whole_path_tflops": fl["total"] / (kern["total"] * 1e-3) / 1e12,
     }
+    if mixed:
+        out["mixed_precision"] = mixed
     if assoc:
         out["associative"] = assoc
+    if not args.no_mixed and not args.no_precision:
+        out["precision_workload"] = precision_workload(args)
+    if not args.no_serving:
+        out["serving"] = run_serving(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_k)

@@ -166,13 +166,14 @@ class Server:
         self._shapes[rid] = (problem.k, problem.packed["n"])
         return rid
 
-    def wait(self, rid, want_cov=True):
-        """(states k x n, covariances k x n x n or None, latency in ms)."""
+    def wait(self, rid, want_cov=True, want_states=True):
+        """(states k x n or None, covariances k x n x n or None, latency in ms)."""
         k, n = self._shapes.pop(rid)
-        x = np.empty((k, n))
+        x = np.empty((k, n)) if want_states else None
         cv = np.empty((k, n, n)) if want_cov else None
         lat = C.c_double()
-        rc = lib().ukhip_server_wait(self.h, rid, x.ctypes.data, cv.ctypes.data if want_cov else None, C.byref(lat))
+        rc = lib().ukhip_server_wait(self.h, rid, x.ctypes.data if want_states else None,
+                                     cv.ctypes.data if want_cov else None, C.byref(lat))
         if rc != 0:
             raise UkhipError(last_error())
         if want_cov:
