@@ -62,6 +62,10 @@ int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options);
 /* 0: ukhip_batch_smooth_oddeven launches kernel by kernel instead of replaying
  * a captured graph (default 1). */
 void ukhip_use_graphs(int enable);
+/* 1 (default): the covariance back substitution for n > 16 runs its products on
+ * the FP64 matrix cores; 0: on the vector units, as for n <= 16. Applies to
+ * launches and graph captures made afterwards. */
+void ukhip_use_mfma(int enable);
 /* Associative-scan smoothing of every problem of the batch. */
 int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options);
 int ukhip_synchronize(void);

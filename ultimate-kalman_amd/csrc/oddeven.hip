@@ -652,6 +652,292 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
 }
 
 /* ------------------------------------------------------------------------ */
+/* back substitution on the matrix cores (NB >= 32, with covariances)         */
+/* ------------------------------------------------------------------------ */
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+/* One 16x16x4 FP64 matrix-core step, D = A B + D. Lane l supplies A[l%16][l/16]
+ * and B[l/16][l%16]; D[4r+l/16][l%16] is element r of the lane's dx4. */
+__device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+/* x <- R^{-1} x for upper-triangular R in LDS with leading dimension LD. */
+template <int NB, int LD>
+__device__ __forceinline__ void back_solve_ld(double (&x)[NB], const double* R, const double* rinv) {
+#pragma unroll
+  for (int pp = NB - 1; pp >= 0; --pp) {
+    const double xq = x[pp] * rinv[pp];
+    x[pp] = xq;
+#pragma unroll
+    for (int i = 0; i < pp; ++i) x[i] = fma(-R[i + pp * LD], xq, x[i]);
+  }
+}
+
+/* Four NB x NB blocks (column-major, odd leading dimension NB + 1, so that
+ * both the unit-stride and the LD-stride fragment reads of the matrix cores
+ * spread over the LDS banks) and the step's vectors. Block use over time:
+ *   [0] Bt -> R -> R^{-1}   [1] Y -> N = I + W T^T   [2..3] W -> [2] K = R^{-1} N */
+template <int NB>
+struct BackMShared {
+  static constexpr int LD = NB + 1;
+  double blk[4][NB * LD];
+  double r[NB]; /* cf - Bt x_{j-1} - Y x_{j+1} */
+  double xm[NB], xp[NB];
+  double rinv[NB];
+};
+
+/*
+ * The back substitution of one even step per workgroup (NB / 8 waves), the
+ * same quantities as k_backsub with covariances:
+ *   W^T = Sigma T^T (Sigma symmetric: its rows stream from HBM as matrix-core
+ *   A fragments, 16 consecutive doubles per fragment row), N = I + W T^T,
+ *   R^{-1} by per-lane back substitution of the unit vectors,
+ *   U = R^{-1} W (cross covariances; skipped when Xcur is NULL),
+ *   S = (R^{-1} N) R^{-T}.
+ * Products with R^{-1} skip its zero lower blocks. The estimate is a
+ * substitution with R (lanes of wave 1) beside the inverse (lanes of wave 0).
+ */
+template <int NB>
+__global__ void __launch_bounds__(NB * 8)
+    k_backsub_mfma(uk_level_t lv, uk_factors_t fac, double* __restrict__ xout, double* __restrict__ Sout,
+                   const double* __restrict__ Xup, const double* __restrict__ XTup, double* __restrict__ Xcur,
+                   double* __restrict__ XTcur, int n) {
+  constexpr int LD = NB + 1;
+  constexpr int NT = NB * 8; /* NW = NB / 8 waves: one per 16-row tile of the 2 NB rows of W^T */
+  constexpr int NW = NT / 64;
+  constexpr int TB = NB / 16; /* tiles per block side */
+  __shared__ BackMShared<NB> sh;
+  double* const T = sh.blk[0]; /* Bt | Y: NB x 2 NB */
+  double* const W = sh.blk[2]; /* NB x 2 NB */
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int64_t ev = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, ev);
+  const int64_t u = ev - lv.even_start[p];
+  const int64_t j = 2 * u;
+  const int64_t L = lv.L[p];
+  const bool has_m = j > 0, has_p = j + 1 < L;
+  const int64_t nn = (int64_t)n * n;
+  const int64_t ob = lv.orig_start[lv.prob[p]];
+  const int64_t stride = (int64_t)1 << lv.level;
+  const int64_t oj = ob + (j + 1) * stride - 1;
+  const int64_t om = ob + j * stride - 1;
+  const int64_t op = ob + (j + 2) * stride - 1;
+  const int64_t sj = lv.step_start[p] + j;
+  const int64_t sup = lv.next_start[p] + (j >> 1) - 1;
+  const double* Rg = fac.R + ev * nn;
+  const double* Bg = fac.Bt + ev * nn;
+  const double* Yg = fac.Y + ev * nn;
+
+  for (int idx = tid; idx < NB * NB; idx += NT) {
+    const int i = idx % NB, c = idx / NB;
+    const bool in = i < n && c < n;
+    T[i + c * LD] = (in && has_m) ? Bg[i + c * n] : 0.0;
+    T[NB * LD + i + c * LD] = (in && has_p) ? Yg[i + c * n] : 0.0;
+  }
+  for (int i = tid; i < NB; i += NT) {
+    sh.xm[i] = (has_m && i < n) ? xout[om * n + i] : 0.0;
+    sh.xp[i] = (has_p && i < n) ? xout[op * n + i] : 0.0;
+  }
+  __syncthreads();
+  if (tid < NB) {
+    double r = 0.0;
+    if (tid < n) {
+      r = fac.cf[ev * n + tid];
+      for (int q = 0; q < n; ++q) r -= T[tid + q * LD] * sh.xm[q] + T[NB * LD + tid + q * LD] * sh.xp[q];
+    }
+    sh.r[tid] = r;
+  }
+
+  /* ---- W^T = Sigma T^T: wave w owns rows [16w, 16w + 16) of W^T ---- */
+  {
+    const int rb = w / TB;                 /* row block of Sigma: 0 (u_{j-1}) or 1 (u_{j+1}) */
+    const int ri = (w % TB) * 16 + li;     /* row within the block */
+    const double* src[2];
+    bool live[2];
+    if (rb == 0) {
+      src[0] = Sout + om * nn; live[0] = has_m;            /* S_mm */
+      src[1] = Xup + sup * nn; live[1] = has_m && has_p;   /* S_mp = Cov(u_{j-1}, u_{j+1}) */
+    } else {
+      src[0] = XTup + sup * nn; live[0] = has_m && has_p;  /* S_pm */
+      src[1] = Sout + op * nn; live[1] = has_p;            /* S_pp */
+    }
+    dx4 acc[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) acc[t] = dx4{0.0, 0.0, 0.0, 0.0};
+    constexpr int KS = NB / 2; /* 2 NB columns of Sigma in steps of 4 */
+    constexpr int KB = 8;
+#pragma unroll 1
+    for (int ks0 = 0; ks0 < KS; ks0 += KB) {
+      double a[KB];
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        const int c = (ks0 + t) * 4 + lk;
+        const int cb = (ks0 + t) * 4 / NB, ci = c - cb * NB;
+        a[t] = 0.0;
+        if (live[cb] && ri < n && ci < n) a[t] = src[cb][ri + (int64_t)ci * n];
+      }
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        const int k = (ks0 + t) * 4 + lk;
+#pragma unroll
+        for (int tc = 0; tc < TB; ++tc) acc[tc] = mfma16(a[t], T[k * LD + tc * 16 + li], acc[tc]);
+      }
+    }
+    __syncthreads(); /* sh.r read T; everyone past the T reads of this phase before W lands */
+#pragma unroll
+    for (int tc = 0; tc < TB; ++tc)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) W[(w * 16 + 4 * r + lk) * LD + tc * 16 + li] = acc[tc][r];
+  }
+  __syncthreads();
+
+  /* ---- N = I + W T^T, kept in registers until T is dead ---- */
+  constexpr int NTILE = TB * TB;
+  constexpr int PER = (NTILE + NW - 1) / NW;
+  dx4 nacc[PER];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = w + s * NW;
+    nacc[s] = dx4{0.0, 0.0, 0.0, 0.0};
+    if (t < NTILE) {
+      const int ti = t % TB, tj = t / TB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        nacc[s][r] = (ti == tj && 4 * r + lk == li && ti * 16 + li < n) ? 1.0 : 0.0;
+#pragma unroll 8
+      for (int ks = 0; ks < NB / 2; ++ks) {
+        const int k = ks * 4 + lk;
+        nacc[s] = mfma16(W[k * LD + ti * 16 + li], T[k * LD + tj * 16 + li], nacc[s]);
+      }
+    }
+  }
+  __syncthreads(); /* T dead */
+  double* const Rs = sh.blk[0];
+  double* const Ns = sh.blk[1];
+  for (int idx = tid; idx < NB * NB; idx += NT) {
+    const int i = idx % NB, c = idx / NB;
+    Rs[i + c * LD] = (i < n && c < n) ? Rg[i + c * n] : (i == c ? 1.0 : 0.0);
+  }
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = w + s * NW;
+    if (t < NTILE) {
+      const int ti = t % TB, tj = t / TB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ns[(tj * 16 + li) * LD + ti * 16 + 4 * r + lk] = nacc[s][r];
+    }
+  }
+  __syncthreads();
+  if (tid < NB) sh.rinv[tid] = 1.0 / Rs[tid * LD + tid];
+  __syncthreads();
+
+  /* ---- R^{-1} (wave 0, lane q: column q) and the estimate (wave 1) ---- */
+  double v[NB];
+  if (w == 0 && lane < n) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) v[i] = (i == lane) ? 1.0 : 0.0;
+    back_solve_ld<NB, LD>(v, Rs, sh.rinv);
+  }
+  if (w == 1) {
+    double r = lane < n ? sh.r[lane] : 0.0;
+    for (int q = n - 1; q >= 0; --q) {
+      const double xq = __shfl(r, q) * sh.rinv[q];
+      if (lane == q) r = xq;
+      else if (lane < q) r -= Rs[lane + q * LD] * xq;
+    }
+    if (lane < n) xout[oj * n + lane] = r;
+  }
+  __syncthreads(); /* R consumed */
+  if (w == 0 && lane < n) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) Rs[i + lane * LD] = v[i]; /* column q of R^{-1}; columns >= n stay identity */
+  }
+  __syncthreads();
+  const double* const Ri = Rs;
+
+  /* ---- U = R^{-1} W -> cross covariances (wave w: tile column w of 2 TB) ---- */
+  if (Xcur) {
+    const int tc = w;
+    const bool left = tc < TB;
+    const bool want = left ? has_m : has_p;
+    if (want) {
+      double* dA = left ? XTcur + (sj - 1) * nn : Xcur + sj * nn; /* -U_1 -> XT[j-1]; -U_2 -> X[j] */
+      double* dB = left ? Xcur + (sj - 1) * nn : XTcur + sj * nn; /* transposes */
+      const int jc = (left ? tc : tc - TB) * 16 + li;             /* column within the n x n block */
+#pragma unroll
+      for (int ti = 0; ti < TB; ++ti) {
+        dx4 acc = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int ks = ti * 4; ks < NB / 4; ++ks) {
+          const int k = ks * 4 + lk;
+          acc = mfma16(Ri[k * LD + ti * 16 + li], W[(tc * 16 + li) * LD + k], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 16 + 4 * r + lk;
+          if (i < n && jc < n) {
+            dA[i + (int64_t)jc * n] = -acc[r];
+            dB[jc + (int64_t)i * n] = -acc[r];
+          }
+        }
+      }
+    }
+  }
+
+  /* ---- K = R^{-1} N (registers), then over W's first block ---- */
+  dx4 kacc[PER];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = w + s * NW;
+    kacc[s] = dx4{0.0, 0.0, 0.0, 0.0};
+    if (t < NTILE) {
+      const int ti = t % TB, tj = t / TB;
+#pragma unroll 4
+      for (int ks = ti * 4; ks < NB / 4; ++ks) {
+        const int k = ks * 4 + lk;
+        kacc[s] = mfma16(Ri[k * LD + ti * 16 + li], Ns[(tj * 16 + li) * LD + k], kacc[s]);
+      }
+    }
+  }
+  __syncthreads(); /* W consumed by U */
+  double* const Ks = sh.blk[2];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = w + s * NW;
+    if (t < NTILE) {
+      const int ti = t % TB, tj = t / TB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ks[(tj * 16 + li) * LD + ti * 16 + 4 * r + lk] = kacc[s][r];
+    }
+  }
+  __syncthreads();
+
+  /* ---- S = K R^{-T}: B fragment R^{-T}[k][j] = R^{-1}[j][k], zero for k < j ---- */
+  double* S = Sout + oj * nn;
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = w + s * NW;
+    if (t < NTILE) {
+      const int ti = t % TB, tj = t / TB;
+      dx4 acc = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int ks = tj * 4; ks < NB / 4; ++ks) {
+        const int k = ks * 4 + lk;
+        acc = mfma16(Ks[k * LD + ti * 16 + li], Ri[k * LD + tj * 16 + li], acc);
+      }
+      const int jc = tj * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + 4 * r + lk;
+        if (i < n && jc < n) S[jc + (int64_t)i * n] = acc[r]; /* S symmetric: S[i][jc] stored as its transpose */
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* whitening (level-0 chain)                                                  */
 /* ------------------------------------------------------------------------ */
 
@@ -874,10 +1160,29 @@ extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain
   return launch_eliminate<float>(lv, ci, co, f, fix, n, st, nullptr);
 }
 
+/* Back substitution with covariances for n > 16 on the matrix cores (default)
+ * or on the vector units (k_backsub, the kernel used for n <= 16). */
+static int g_backsub_mfma = 1;
+static bool uk_backsub_mfma_enabled() { return g_backsub_mfma != 0; }
+extern "C" void ukhip_use_mfma(int on) { g_backsub_mfma = on ? 1 : 0; }
+
 extern "C" hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                                         const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n,
                                         int cov, hipStream_t st) {
   const int nb = uk_bucket(n);
+  if (cov && nb >= 32 && uk_backsub_mfma_enabled()) {
+#define M_(NBV)                                                                                       \
+  k_backsub_mfma<NBV><<<dim3((unsigned)lv->evens), dim3(NBV * 8), 0, st>>>(*lv, *fac, xout, Sout, Xup, XTup, \
+                                                                          Xcur, XTcur, n)
+    switch (nb) {
+      case 32: M_(32); break;
+      case 48: M_(48); break;
+      case 64: M_(64); break;
+      default: return hipErrorInvalidValue;
+    }
+#undef M_
+    return hipGetLastError();
+  }
 #define L_(NBV)                                                                                                  \
   k_backsub<NBV><<<dim3((unsigned)lv->evens), dim3(((2 * NBV + 63) / 64) * 64), 0, st>>>(*lv, *fac, xout, Sout, Xup, \
                                                                                        XTup, Xcur, XTcur, n, cov)

@@ -54,6 +54,7 @@ def lib():
         L.ukhip_synchronize.restype = C.c_int
         L.ukhip_stream.restype = C.c_void_p
         L.ukhip_use_graphs.argtypes = [C.c_int]
+        L.ukhip_use_mfma.argtypes = [C.c_int]
         L.ukhip_batch_smooth_oddeven_mixed.argtypes = [C.c_void_p, C.c_int32]
         L.ukhip_batch_smooth_oddeven_mixed.restype = C.c_int
         L.ukhip_server_create.restype = C.c_void_p
