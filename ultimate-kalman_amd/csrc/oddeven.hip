@@ -142,12 +142,21 @@ struct Refl {
   T sc[2][2];
 };
 
+/* Reflector prefetch: the whole reflector is held in registers when a lane's
+ * share has at most UK_QR_FULL entries, else it streams in chunks of UK_QR_CH. */
+#ifndef UK_QR_FULL
+#define UK_QR_FULL 24
+#endif
+#ifndef UK_QR_CH
+#define UK_QR_CH 16
+#endif
+
 /*
  * One Householder step for pivot k (target row k of block 0).
  *   is_piv: these lanes own pivot column k; upd: this column is updated.
- * The pivot lanes publish the whole reflector, v_k = 1, v_i = 0 for i < k,
- * v_i = x_i / (alpha - beta) below, and tau; then every updated column
- * applies H = I - tau v v^T. The dot and the update run over all locals
+ * The pivot lanes publish the whole reflector, u_i = 0 for i < k,
+ * u_k = alpha - beta, u_i = x_i below, and tau'; then every updated column
+ * applies H = I - tau' u u^T. The dot and the update run over all locals
  * without branches (entries outside the active rows are exact zeros), so the
  * LDS reads of v issue back to back. One barrier.
  */
@@ -174,31 +183,31 @@ __device__ __forceinline__ void house_step(Col<NB, SL, T>& col, Refl<NB, SL, T>&
     }
     const T s = group_sum<SL>(s0 + s1);
     alpha = group_sum<SL>(alpha);
-    T tau = T(0), beta = alpha, scale = T(0);
+    /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
+     * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
+     * pivot lanes publish their column as it is (one division, no scaling). */
+    T taup = T(0), ukk = T(0);
     if (s != T(0)) {
-      beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-      tau = (beta - alpha) / beta;
-      scale = T(1) / (alpha - beta);
+      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+      ukk = alpha - beta;
+      taup = T(-1) / (beta * ukk);
     }
 #pragma unroll
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
-      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
+      vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
     }
 #pragma unroll
-    for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l] * scale;
-    if (sub == 0) {
-      R.sc[buf][0] = tau;
-      R.sc[buf][1] = beta;
-    }
+    for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l];
+    if (sub == 0) R.sc[buf][0] = taup;
   }
   __syncthreads();
   if (upd) {
-    const T tau = R.sc[buf][0];
+    const T tau = R.sc[buf][0]; /* tau' */
     if (tau != T(0)) {
       /* v is read from LDS in chunks of CH locals issued back to back */
       constexpr int NL = 2 * H;
-      constexpr int CH = NL <= 24 ? NL : 16;
+      constexpr int CH = NL <= UK_QR_FULL ? NL : UK_QR_CH;
       using V2 = typename Vec2<T>::type;
       const V2* v2 = reinterpret_cast<const V2*>(vv);
       T d0 = T(0), d1 = T(0), d2 = T(0), d3 = T(0);
@@ -245,12 +254,40 @@ __device__ __forceinline__ void house_step(Col<NB, SL, T>& col, Refl<NB, SL, T>&
 }
 
 /* Lanes per column and threads of a QR kernel over COLS matrix columns:
- * 4 lanes per column unless that would exceed 640 threads (10 waves), whose
- * VGPR budget would not hold the column plus a prefetched reflector. */
-template <int NB, int COLS>
+ * 4 lanes per column up to 1024 threads, else 2. MINW is the waves per SIMD
+ * asked of the register allocator so that two workgroups share a CU (every
+ * Householder step is a latency chain; a second workgroup hides it). */
+/* Per phase (1..3) and bucket: lanes per column (0: 4 lanes per column up to
+ * 1024 threads, else 2) and the workgroups per CU the register allocation
+ * targets (1: none beyond fitting one). Measured on MI355X: a second
+ * workgroup per CU hides the Householder steps' latency only where the
+ * column and reflector still fit in the registers it leaves. */
+#ifndef UK_P1_WGS48
+#define UK_P1_WGS48 2
+#endif
+#ifndef UK_P2_WGS48
+#define UK_P2_WGS48 1
+#endif
+#ifndef UK_P2_SL48
+#define UK_P2_SL48 0
+#endif
+#ifndef UK_P3_WGS48
+#define UK_P3_WGS48 4
+#endif
+#ifndef UK_P2_WGS64
+#define UK_P2_WGS64 1
+#endif
+__host__ __device__ constexpr int qr_sl(int phase, int NB) { return (phase == 2 && NB == 48) ? UK_P2_SL48 : 0; }
+__host__ __device__ constexpr int qr_wgs(int phase, int NB) {
+  return NB == 48 ? (phase == 1 ? UK_P1_WGS48 : phase == 2 ? UK_P2_WGS48 : UK_P3_WGS48)
+                  : (NB == 64 && phase == 2 ? UK_P2_WGS64 : 1);
+}
+template <int NB, int COLS, int PHASE>
 struct QRGeom {
-  static constexpr int SL = (NB <= 16 || 4 * COLS > 640) ? 2 : 4;
+  static constexpr int SL = qr_sl(PHASE, NB) ? qr_sl(PHASE, NB) : ((NB <= 16 || 4 * COLS > 1024) ? 2 : 4);
   static constexpr int NT = ((SL * COLS + 63) / 64) * 64;
+  static constexpr int MINW_ = (qr_wgs(PHASE, NB) * (NT / 64) + 3) / 4;
+  static constexpr int MINW = MINW_ < 1 ? 1 : (MINW_ > 8 ? 8 : MINW_);
 };
 
 template <int NB, int SL, class T>
@@ -286,10 +323,10 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
 template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT), (QRGeom<NB, 2 * NB + 1, 1>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
-  constexpr int SL = QRGeom<NB, 2 * NB + 1>::SL;
-  constexpr int NT = QRGeom<NB, 2 * NB + 1>::NT;
+  constexpr int SL = QRGeom<NB, 2 * NB + 1, 1>::SL;
+  constexpr int NT = QRGeom<NB, 2 * NB + 1, 1>::NT;
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t e = blockIdx.x;
@@ -341,10 +378,10 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1>::NT))
 }
 
 template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
+__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2>::NT), (QRGeom<NB, 3 * NB + 1, 2>::MINW))
     k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
-  constexpr int SL = QRGeom<NB, 3 * NB + 1>::SL;
-  constexpr int NT = QRGeom<NB, 3 * NB + 1>::NT;
+  constexpr int SL = QRGeom<NB, 3 * NB + 1, 2>::SL;
+  constexpr int NT = QRGeom<NB, 3 * NB + 1, 2>::NT;
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t e = blockIdx.x;
@@ -396,10 +433,10 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1>::NT))
 }
 
 template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, NB + 1>::NT))
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3>::NT), (QRGeom<NB, NB + 1, 3>::MINW))
     k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
-  constexpr int SL = QRGeom<NB, NB + 1>::SL;
-  constexpr int NT = QRGeom<NB, NB + 1>::NT;
+  constexpr int SL = QRGeom<NB, NB + 1, 3>::SL;
+  constexpr int NT = QRGeom<NB, NB + 1, 3>::NT;
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
   const int64_t unit = blockIdx.x;
@@ -1125,12 +1162,12 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
   };
 #define L_(NBV)                                                                                                      \
   mark(0);                                                                                                           \
-  k_phase1<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1>::NT), 0, st>>>(*lv, in, out, fac, n);  \
+  k_phase1<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1, 1>::NT), 0, st>>>(*lv, in, out, fac, n);  \
   mark(1);                                                                                                           \
-  k_phase2<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1>::NT), 0, st>>>(*lv, in, out, fac, fix, \
+  k_phase2<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1, 2>::NT), 0, st>>>(*lv, in, out, fac, fix, \
                                                                                            n);                     \
   mark(2);                                                                                                           \
-  k_phase3<NBV, T><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1>::NT), 0, st>>>(*lv, in, out, fix, n);      \
+  k_phase3<NBV, T><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1, 3>::NT), 0, st>>>(*lv, in, out, fix, n);      \
   mark(3)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
