@@ -737,7 +737,7 @@ struct BackMShared {
  * substitution with R (lanes of wave 1) beside the inverse (lanes of wave 0).
  */
 template <int NB>
-__global__ void __launch_bounds__(NB * 8)
+__global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
     k_backsub_mfma(uk_level_t lv, uk_factors_t fac, double* __restrict__ xout, double* __restrict__ Sout,
                    const double* __restrict__ Xup, const double* __restrict__ XTup, double* __restrict__ Xcur,
                    double* __restrict__ XTcur, int n) {
@@ -870,28 +870,75 @@ __global__ void __launch_bounds__(NB * 8)
   if (tid < NB) sh.rinv[tid] = 1.0 / Rs[tid * LD + tid];
   __syncthreads();
 
-  /* ---- R^{-1} (wave 0, lane q: column q) and the estimate (wave 1) ---- */
-  double v[NB];
-  if (w == 0 && lane < n) {
+  /* ---- R^{-1}, blockwise over 16 x 16 blocks; the estimate (wave 1) ----
+   * Diagonal blocks: lane t < NB solves column t % 16 of block t / 16 by
+   * substitution. Off-diagonal blocks, one superdiagonal d at a time (wave i
+   * takes block (i, i + d)): R^{-1}_ij = -R^{-1}_ii sum_{k=i+1..j} R_ik R^{-1}_kj
+   * on the matrix cores, the inner sum's D fragments serving as the outer
+   * product's B operand. Off-diagonal results are kept transposed in the
+   * (unused) lower blocks while R's upper blocks are still read, then moved. */
+  {
+    double v[16];
+    const int bb = tid >> 4, bc = tid & 15;
+    if (tid < NB) {
+      const double* Rb = Rs + bb * 16 * (LD + 1);
+      const double* rb = sh.rinv + bb * 16;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) v[i] = (i == lane) ? 1.0 : 0.0;
-    back_solve_ld<NB, LD>(v, Rs, sh.rinv);
-  }
-  if (w == 1) {
-    double r = lane < n ? sh.r[lane] : 0.0;
-    for (int q = n - 1; q >= 0; --q) {
-      const double xq = __shfl(r, q) * sh.rinv[q];
-      if (lane == q) r = xq;
-      else if (lane < q) r -= Rs[lane + q * LD] * xq;
+      for (int i = 0; i < 16; ++i) v[i] = (i == bc) ? 1.0 : 0.0;
+#pragma unroll
+      for (int pp = 15; pp >= 0; --pp) {
+        const double xq = v[pp] * rb[pp];
+        v[pp] = xq;
+#pragma unroll
+        for (int i = 0; i < pp; ++i) v[i] = fma(-Rb[i + pp * LD], xq, v[i]);
+      }
     }
-    if (lane < n) xout[oj * n + lane] = r;
-  }
-  __syncthreads(); /* R consumed */
-  if (w == 0 && lane < n) {
+    if (w == 1) {
+      double r = lane < n ? sh.r[lane] : 0.0;
+      for (int q = n - 1; q >= 0; --q) {
+        const double xq = __shfl(r, q) * sh.rinv[q];
+        if (lane == q) r = xq;
+        else if (lane < q) r -= Rs[lane + q * LD] * xq;
+      }
+      if (lane < n) xout[oj * n + lane] = r;
+    }
+    __syncthreads(); /* R's diagonal blocks consumed */
+    if (tid < NB) {
+      double* Rb = Rs + bb * 16 * (LD + 1);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) Rs[i + lane * LD] = v[i]; /* column q of R^{-1}; columns >= n stay identity */
+      for (int i = 0; i < 16; ++i) Rb[i + bc * LD] = v[i];
+    }
+    __syncthreads();
+    for (int d = 1; d < TB; ++d) {
+      if (w < TB - d) {
+        const int i0 = w * 16, j0 = (w + d) * 16;
+        dx4 x = dx4{0.0, 0.0, 0.0, 0.0};
+        for (int kb = w + 1; kb <= w + d; ++kb) {
+          const int k0 = kb * 16;
+#pragma unroll
+          for (int r4 = 0; r4 < 4; ++r4) {
+            const int kc = k0 + 4 * r4 + lk;
+            const double a = Rs[(i0 + li) + kc * LD]; /* R_ik */
+            const double b = (kb == w + d) ? Rs[kc + (j0 + li) * LD]      /* R^{-1}_jj (diagonal block) */
+                                           : Rs[(j0 + li) + kc * LD];     /* R^{-1}_kj, stored transposed */
+            x = mfma16(a, b, x);
+          }
+        }
+        dx4 y = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) y = mfma16(-Rs[(i0 + li) + (i0 + 4 * r4 + lk) * LD], x[r4], y);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Rs[(j0 + li) + (i0 + 4 * r + lk) * LD] = y[r]; /* R^{-1}_ij transposed */
+      }
+      __syncthreads();
+    }
+    /* move the off-diagonal blocks to their upper positions */
+    for (int idx = tid; idx < NB * NB; idx += NT) {
+      const int a = idx % NB, c = idx / NB;
+      if ((a >> 4) < (c >> 4)) Rs[a + c * LD] = Rs[c + a * LD];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const double* const Ri = Rs;
 
   /* ---- U = R^{-1} W -> cross covariances (wave w: tile column w of 2 TB) ---- */
