@@ -767,6 +767,34 @@ __global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
   const double* Bg = fac.Bt + ev * nn;
   const double* Yg = fac.Y + ev * nn;
 
+  /* Every global read of the kernel is issued here, so that one memory
+   * latency covers them: this wave's rows of Sigma (the A fragments of
+   * W^T = Sigma T^T, 16 consecutive doubles per fragment row) and this
+   * thread's share of R (stored to LDS once T is dead). */
+  constexpr int KS = NB / 2; /* 2 NB columns of Sigma in steps of 4 */
+  double sig[KS];
+  {
+    const int rb = w / TB;             /* row block of Sigma: 0 (u_{j-1}) or 1 (u_{j+1}) */
+    const int ri = (w % TB) * 16 + li; /* row within the block */
+    const double* src0 = rb == 0 ? Sout + om * nn : XTup + sup * nn; /* S_mm | S_pm */
+    const double* src1 = rb == 0 ? Xup + sup * nn : Sout + op * nn;  /* S_mp | S_pp */
+    const bool live0 = rb == 0 ? has_m : (has_m && has_p);
+    const bool live1 = rb == 0 ? (has_m && has_p) : has_p;
+#pragma unroll
+    for (int t = 0; t < KS; ++t) {
+      const int cb = t * 4 / NB, ci = t * 4 + lk - cb * NB;
+      sig[t] = 0.0;
+      if ((cb == 0 ? live0 : live1) && ri < n && ci < n) sig[t] = (cb == 0 ? src0 : src1)[ri + (int64_t)ci * n];
+    }
+  }
+  constexpr int RPER = (NB * NB + NT - 1) / NT;
+  double rpre[RPER];
+#pragma unroll
+  for (int e = 0; e < RPER; ++e) {
+    const int idx = tid + e * NT;
+    const int i = idx % NB, c = idx / NB;
+    rpre[e] = (idx < NB * NB && i < n && c < n) ? Rg[i + c * n] : (i == c ? 1.0 : 0.0);
+  }
   for (int idx = tid; idx < NB * NB; idx += NT) {
     const int i = idx % NB, c = idx / NB;
     const bool in = i < n && c < n;
@@ -789,38 +817,14 @@ __global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
 
   /* ---- W^T = Sigma T^T: wave w owns rows [16w, 16w + 16) of W^T ---- */
   {
-    const int rb = w / TB;                 /* row block of Sigma: 0 (u_{j-1}) or 1 (u_{j+1}) */
-    const int ri = (w % TB) * 16 + li;     /* row within the block */
-    const double* src[2];
-    bool live[2];
-    if (rb == 0) {
-      src[0] = Sout + om * nn; live[0] = has_m;            /* S_mm */
-      src[1] = Xup + sup * nn; live[1] = has_m && has_p;   /* S_mp = Cov(u_{j-1}, u_{j+1}) */
-    } else {
-      src[0] = XTup + sup * nn; live[0] = has_m && has_p;  /* S_pm */
-      src[1] = Sout + op * nn; live[1] = has_p;            /* S_pp */
-    }
     dx4 acc[TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t) acc[t] = dx4{0.0, 0.0, 0.0, 0.0};
-    constexpr int KS = NB / 2; /* 2 NB columns of Sigma in steps of 4 */
-    constexpr int KB = 8;
-#pragma unroll 1
-    for (int ks0 = 0; ks0 < KS; ks0 += KB) {
-      double a[KB];
 #pragma unroll
-      for (int t = 0; t < KB; ++t) {
-        const int c = (ks0 + t) * 4 + lk;
-        const int cb = (ks0 + t) * 4 / NB, ci = c - cb * NB;
-        a[t] = 0.0;
-        if (live[cb] && ri < n && ci < n) a[t] = src[cb][ri + (int64_t)ci * n];
-      }
+    for (int t = 0; t < KS; ++t) {
+      const int k = t * 4 + lk;
 #pragma unroll
-      for (int t = 0; t < KB; ++t) {
-        const int k = (ks0 + t) * 4 + lk;
-#pragma unroll
-        for (int tc = 0; tc < TB; ++tc) acc[tc] = mfma16(a[t], T[k * LD + tc * 16 + li], acc[tc]);
-      }
+      for (int tc = 0; tc < TB; ++tc) acc[tc] = mfma16(sig[t], T[k * LD + tc * 16 + li], acc[tc]);
     }
     __syncthreads(); /* sh.r read T; everyone past the T reads of this phase before W lands */
 #pragma unroll
@@ -853,9 +857,10 @@ __global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
   __syncthreads(); /* T dead */
   double* const Rs = sh.blk[0];
   double* const Ns = sh.blk[1];
-  for (int idx = tid; idx < NB * NB; idx += NT) {
-    const int i = idx % NB, c = idx / NB;
-    Rs[i + c * LD] = (i < n && c < n) ? Rg[i + c * n] : (i == c ? 1.0 : 0.0);
+#pragma unroll
+  for (int e = 0; e < RPER; ++e) {
+    const int idx = tid + e * NT;
+    if (idx < NB * NB) Rs[idx % NB + (idx / NB) * LD] = rpre[e];
   }
 #pragma unroll
   for (int s = 0; s < PER; ++s) {
