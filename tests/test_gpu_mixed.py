@@ -89,3 +89,19 @@ def test_mixed_batch_of_dimensions(dev, oracle):
     finally:
         oracle.lib.oracle_set_threads(1)
         b.free()
+
+
+def test_few_observations_stay_finite(dev):
+    """m << n (m = 1, 2, 4 with n = 5, 8, 14): the stacked blocks are nearly
+    dependent, and the FP32 Householder reflectors must stay within FP32's
+    range (a reflector form that squares the column's scale once produced
+    NaN here). Two refinement steps reach FP64 accuracy."""
+    specs = [dict(n=14, m=4, k=220, seed=101, unobserved=0.05, name="n14m4_k220"),
+             dict(n=8, m=2, k=178, seed=102, unobserved=0.05, name="n8m2_k178"),
+             dict(n=5, m=1, k=21, seed=103, unobserved=0.05, name="n5m1_k21"),
+             dict(n=14, m=4, k=559, seed=104, unobserved=0.05, name="n14m4_k559"),
+             dict(n=16, m=1, k=400, seed=105, unobserved=0.05, name="n16m1_k400")]
+    probs = [P.serve_problem(s) for s in specs]
+    say("few observations per step")
+    errs = _errors(dev, probs, (0, 2))
+    assert np.isfinite(errs[0]) and errs[1] < TOL, errs

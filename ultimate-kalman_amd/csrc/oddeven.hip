@@ -154,9 +154,9 @@ struct Refl {
 /*
  * One Householder step for pivot k (target row k of block 0).
  *   is_piv: these lanes own pivot column k; upd: this column is updated.
- * The pivot lanes publish the whole reflector, u_i = 0 for i < k,
- * u_k = alpha - beta, u_i = x_i below, and tau'; then every updated column
- * applies H = I - tau' u u^T. The dot and the update run over all locals
+ * The pivot lanes publish the whole reflector (u_i = 0 for i < k and its
+ * scalar; the two forms below); then every updated column applies
+ * H = I - tau u u^T. The dot and the update run over all locals
  * without branches (entries outside the active rows are exact zeros), so the
  * LDS reads of v issue back to back. One barrier.
  */
@@ -183,27 +183,50 @@ __device__ __forceinline__ void house_step(Col<NB, SL, T>& col, Refl<NB, SL, T>&
     }
     const T s = group_sum<SL>(s0 + s1);
     alpha = group_sum<SL>(alpha);
-    /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
-     * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
-     * pivot lanes publish their column as it is (one division, no scaling). */
-    T taup = T(0), ukk = T(0);
-    if (s != T(0)) {
-      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-      ukk = alpha - beta;
-      taup = T(-1) / (beta * ukk);
-    }
+    if constexpr (sizeof(T) == 8) {
+      /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
+       * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
+       * pivot lanes publish their column as it is (one division, no
+       * scaling). |beta u_k| >= beta^2 = ||x||^2, so a column whose squared
+       * norm is below 1e-280 counts as zero (no reflection). */
+      T taup = T(0), ukk = T(0);
+      if (s > T(1e-280)) {
+        const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+        ukk = alpha - beta;
+        taup = T(-1) / (beta * ukk);
+      }
 #pragma unroll
-    for (int l = 0; l < H; ++l) {
-      const int r = SL * l + sub;
-      vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
-    }
+      for (int l = 0; l < H; ++l) {
+        const int r = SL * l + sub;
+        vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
+      }
 #pragma unroll
-    for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l];
-    if (sub == 0) R.sc[buf][0] = taup;
+      for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l];
+      if (sub == 0) R.sc[buf][0] = taup;
+    } else {
+      /* single precision: the unit-pivot form v = x / (alpha - beta), v_k = 1,
+       * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
+       * the form above does, would leave FP32's range on nearly dependent
+       * columns (m << n) */
+      T tau = T(0), scale = T(0);
+      if (s != T(0)) {
+        const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
+      }
+#pragma unroll
+      for (int l = 0; l < H; ++l) {
+        const int r = SL * l + sub;
+        vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
+      }
+#pragma unroll
+      for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l] * scale;
+      if (sub == 0) R.sc[buf][0] = tau;
+    }
   }
   __syncthreads();
   if (upd) {
-    const T tau = R.sc[buf][0]; /* tau' */
+    const T tau = R.sc[buf][0];
     if (tau != T(0)) {
       /* v is read from LDS in chunks of CH locals issued back to back */
       constexpr int NL = 2 * H;
@@ -1175,6 +1198,46 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_
   }
 }
 
+/*
+ * Whitening when every K_i and C_i is given as 'w' (inverse standard
+ * deviations): V = diag(k_i), W = diag(w_i), so B = -k o F, D = k o H,
+ * c = k o c, C = w o G, o = w o o (row scalings, c/kalman_base.c:61-175).
+ * Element-parallel over spw consecutive steps per workgroup: consecutive
+ * threads touch consecutive doubles of every array.
+ */
+__global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out, int spw) {
+  const int n = raw.n, mr = raw.mr, nn = n * n, mn = mr * n;
+  const int64_t s0 = (int64_t)blockIdx.x * spw;
+  const int cnt = (int)((raw.steps - s0) < spw ? (raw.steps - s0) : spw);
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < cnt * nn; idx += 256) {
+    const int ls = idx / nn, e = idx - ls * nn;
+    const int64_t s = s0 + ls;
+    if (raw.first[s]) continue; /* no evolution equation on a chain's first step */
+    const int i = e % n;
+    const int64_t g = s * nn + e;
+    const double kw = raw.K[s * n + i];
+    out.B[g] = -kw * raw.F[g];
+    out.D[g] = raw.H_ident ? ((e / n == i) ? kw : 0.0) : kw * raw.H[g];
+  }
+  for (int idx = tid; idx < cnt * mn; idx += 256) {
+    const int ls = idx / mn, e = idx - ls * mn;
+    const int64_t s = s0 + ls;
+    const int i = e % mr;
+    const int m = raw.m[s];
+    out.C[s * nn + e] = (i < m) ? raw.C[s * mr + i] * raw.G[s * mn + e] : 0.0;
+  }
+  for (int idx = tid; idx < cnt * n; idx += 256) {
+    const int64_t s = s0 + idx / n;
+    if (!raw.first[s]) out.c[s0 * n + idx] = raw.K[s0 * n + idx] * raw.c[s0 * n + idx];
+  }
+  for (int idx = tid; idx < cnt * mr; idx += 256) {
+    const int ls = idx / mr, i = idx - ls * mr;
+    const int64_t s = s0 + ls;
+    out.o[s * n + i] = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.o[s * mr + i] : 0.0;
+  }
+}
+
 } /* namespace */
 
 /* ------------------------------------------------------------------------ */
@@ -1194,6 +1257,13 @@ extern "C" int uk_bucket(int n) {
 }
 
 extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st) {
+  if (!raw->K_full && !raw->C_full) { /* every covariance given as 'w' */
+    const int nn = raw->n * raw->n;
+    const int spw = nn >= 1024 ? 1 : 1024 / nn;
+    const int64_t blocks = (raw->steps + spw - 1) / spw;
+    k_weigh_diag<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, spw);
+    return hipGetLastError();
+  }
   const int nb = uk_bucket(raw->n > raw->mr ? raw->n : raw->mr);
 #define L_(NBV) k_weigh<NBV><<<dim3((unsigned)raw->steps), dim3(((3 * NBV + 2 + 63) / 64) * 64), 0, st>>>(*raw, *out)
   switch (nb) {
