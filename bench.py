@@ -59,6 +59,23 @@ def bucket(n):
     return next(b for b in (8, 16, 32, 48, 64) if n <= b)
 
 
+def kernel_label(slot, pr, cov=True):
+    """The kernel (as rocprofv3 names it) behind a timing slot of
+    ukhip_batch_smooth_oddeven_timed for problem pr."""
+    n = pr.n[0]
+    nb = bucket(n)
+    if slot == "k_weigh":
+        pk = getattr(pr, "packed", None)
+        if pk:
+            all_w = pk["K_type"] == "w" and pk["C_type"] == "w"
+        else:
+            all_w = all(t in (None, "w") for t in pr.K_type) and all(t in (None, "w") for t in pr.C_type)
+        return "k_weigh_diag" if all_w else f"k_weigh<{bucket(max(n, max(m_list_of(pr))))}>"
+    if slot == "k_backsub":
+        return f"k_backsub_mfma<{nb}>" if cov and nb >= 32 else f"k_backsub<{nb}>"
+    return f"{slot}<{nb}, double>"
+
+
 def traffic_of(kernel, workload):
     """HBM bytes per step of `kernel` on `workload` from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE runs, calibrated with a copy kernel),
@@ -452,9 +469,9 @@ def main():
     fl_of = {"k_weigh": fl["weigh"], "k_phase1": fl["phase1"], "k_phase2": fl["phase2"],
              "k_phase3": fl["phase3"], "k_backsub": fl["backsub"]}
     dom = max(fl_of, key=lambda k: kern[k])
-    nb = bucket(n)
+    label = kernel_label(dom, pr)
     achieved = fl_of[dom] / (kern[dom] * 1e-3) / 1e12
-    traffic = traffic_of(f"{dom}<{nb}>", args.workload if args.k is None else None)
+    traffic = traffic_of(label, args.workload if args.k is None else None)
     if traffic is not None and dom == "k_backsub":
         traffic["algorithmic_bytes_per_step"] = flops.oddeven_backsub_bytes(n, pr.k)
 
@@ -500,13 +517,15 @@ def main():
                    f"{WORKLOADS[args.workload]['desc']} (k overridden to {pr.k})",
                    "algorithm": "odd-even (Gargir-Toledo), FP64, with covariances", "problems": 1,
                    "timesteps": pr.k},
-        "roofline": {"bound": "mfma", "kernel": f"{dom}<{nb}>",
+        "roofline": {"bound": "mfma", "kernel": label,
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "compute": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                     "compute": ("FP64 matrix cores, v_mfma_f64_16x16x4f64" if "mfma" in label else "FP64 VALU") +
+                                " (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "algorithmic_flops_per_step": fl_of[dom], "launches_per_step": launches[dom],
                      "kernel_ms_per_step": kern[dom], "avg_launch_ms": kern[dom] / launches[dom]},
         "kernels_ms_per_step": {k: v for k, v in kern.items()},
+        "kernels": {k: kernel_label(k, pr) for k in fl_of},
         "kernels_tflops": {k: fl_of[k] / (kern[k] * 1e-3) / 1e12 for k in fl_of if kern[k] > 0},
         "algorithmic_flops_per_timestep": fl["total"] / pr.k,
         "whole_path_tflops": fl["total"] / (kern["total"] * 1e-3) / 1e12,
