@@ -420,7 +420,7 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-refine", type=int, default=2)
-    ap.add_argument("--serve-load", type=float, default=0.7,
+    ap.add_argument("--serve-load", type=float, default=0.5,
                     help="Poisson rate as a fraction of the FP64 server's all-at-once request rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB device-to-device once (a plain copy kernel of "

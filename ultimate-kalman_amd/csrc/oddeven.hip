@@ -1150,7 +1150,7 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_
       return;
     }
     if (q < n) {
-      const double* src = raw.F + s * nn + (int64_t)q * n;
+      const double* src = raw.F + raw.Fidx[s] * nn + (int64_t)q * n;
 #pragma unroll
       for (int i = 0; i < NB; ++i) x[i] = (i < n) ? src[i] : 0.0;
     } else if (q < 2 * n) {
@@ -1161,7 +1161,7 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_
 #pragma unroll
       for (int i = 0; i < NB; ++i) x[i] = (i < n) ? raw.c[s * n + i] : 0.0;
     }
-    const double* src = q < n ? raw.F + s * nn + (int64_t)q * n
+    const double* src = q < n ? raw.F + raw.Fidx[s] * nn + (int64_t)q * n
                         : (q < 2 * n ? (raw.H_ident ? nullptr : raw.H + s * nn + (int64_t)(q - n) * n) : raw.c + s * n);
     apply_weight<NB>(x, n, kt, sh.K, sh.K, src, (q >= n && q < 2 * n && raw.H_ident) ? q - n : -1);
     if (q < n) {
@@ -1217,7 +1217,7 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
     const int i = e % n;
     const int64_t g = s * nn + e;
     const double kw = raw.K[s * n + i];
-    out.B[g] = -kw * raw.F[g];
+    out.B[g] = -kw * raw.F[raw.Fidx[s] * nn + e];
     out.D[g] = raw.H_ident ? ((e / n == i) ? kw : 0.0) : kw * raw.H[g];
   }
   for (int idx = tid; idx < cnt * mn; idx += 256) {

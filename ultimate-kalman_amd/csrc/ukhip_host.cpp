@@ -242,6 +242,7 @@ static int cov_code(char t) {
  * that a steady stream of batches does not allocate and free pinned memory
  * (hipHostFree can wait for the device). */
 struct PinnedPool {
+  bool pageable = false; /* plain host memory (buffers that are never DMA'd) */
   std::mutex mu;
   std::map<size_t, std::vector<void*>> free_;
   static size_t cls(size_t bytes) {
@@ -261,6 +262,11 @@ struct PinnedPool {
       }
     }
     void* p = nullptr;
+    if (pageable) {
+      p = std::malloc(c);
+      if (!p) uk_set_error("out of host memory (%zu bytes)", c);
+      return p;
+    }
     if (!hip_ok(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc(pinned pool)")) return nullptr;
     return p;
   }
@@ -272,7 +278,10 @@ struct PinnedPool {
   void clear() {
     std::lock_guard<std::mutex> g(mu);
     for (auto& kv : free_)
-      for (void* p : kv.second) (void)hipHostFree(p);
+      for (void* p : kv.second) {
+        if (pageable) std::free(p);
+        else (void)hipHostFree(p);
+      }
     free_.clear();
   }
 };
@@ -282,7 +291,8 @@ struct Staging {
   int n = 0, mr = 0;
   int64_t steps = 0;
   int H_ident = 1, K_full = 0, C_full = 0;
-  size_t offF = 0, offH = 0, offc = 0, offK = 0, offG = 0, offo = 0, offC = 0, offKt = 0, offCt = 0, offm = 0,
+  int64_t nF = 0; /* F blocks (0: one per step) */
+  size_t offFidx = 0, offF = 0, offH = 0, offc = 0, offK = 0, offG = 0, offo = 0, offC = 0, offKt = 0, offCt = 0, offm = 0,
          offfirst = 0, bytes = 0;
   char* host = nullptr;
 
@@ -294,7 +304,9 @@ struct Staging {
       o += (b + 255) & ~(size_t)255;
       return r;
     };
-    offF = take(S * nn * 8);
+    if (nF <= 0) nF = steps;
+    offFidx = take(S * 8);
+    offF = take((size_t)nF * nn * 8);
     offH = H_ident ? 0 : take(S * nn * 8);
     offc = take(S * n * 8);
     offK = take(S * (K_full ? nn : (size_t)n) * 8);
@@ -308,6 +320,7 @@ struct Staging {
     bytes = o;
   }
   double* F() { return (double*)(host + offF); }
+  int64_t* Fidx() { return (int64_t*)(host + offFidx); }
   double* H() { return (double*)(host + offH); }
   double* c() { return (double*)(host + offc); }
   double* K() { return (double*)(host + offK); }
@@ -418,6 +431,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->raw.K_full = st.K_full;
   b->raw.C_full = st.C_full;
   b->raw.F = (const double*)(r + st.offF);
+  b->raw.Fidx = (const int64_t*)(r + st.offFidx);
   b->raw.H = st.H_ident ? nullptr : (const double*)(r + st.offH);
   b->raw.c = (const double*)(r + st.offc);
   b->raw.K = (const double*)(r + st.offK);
@@ -636,6 +650,7 @@ static bool pack_equations(kalman_step_equations_t** eq, int64_t k, Staging& st)
   for (int64_t i = 0; i < k; ++i) {
     kalman_step_equations_t* e = eq[i];
     st.first()[i] = i == 0;
+    st.Fidx()[i] = i;
     if (i > 0) {
       copy_cols(st.F() + i * nn, n, e->F, n, n);
       if (!st.H_ident) copy_cols(st.H() + i * nn, n, e->H, n, n);
@@ -662,7 +677,7 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
   const int n = P[0].n;
   if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("state dimension %d outside 1..%d", n, UKHIP_MAX_N); return false; }
   int mr = 1;
-  int64_t S = 0;
+  int64_t S = 0, nF = 0;
   st.H_ident = 1;
   st.K_full = 0;
   st.C_full = 0;
@@ -677,48 +692,109 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
     if (q.C_type != 'w') st.C_full = 1;
     mr = std::max(mr, q.m);
     S += q.k;
+    nF += q.F_shared ? 1 : q.k;
     lengths.push_back(q.k);
   }
   st.n = n;
   st.mr = mr;
   st.steps = S;
+  st.nF = nF;
   if (!staging_alloc(st)) return false;
-  std::memset(st.host, 0, st.bytes);
+  /* Every field a kernel reads is written below (no blanket clearing): rows
+   * of G, o, C at or beyond a step's m_i and the evolution fields of a
+   * problem's first step are never read. */
   const size_t nn = (size_t)n * n;
   const int Kw = st.K_full ? n * n : n, Cw = st.C_full ? mr * mr : mr;
-  int64_t s = 0;
+  /* first step and first F block of every problem */
+  std::vector<int64_t> s0v(nprob), fbv(nprob);
+  int64_t s_acc = 0, f_acc = 0;
   for (int p = 0; p < nprob; ++p) {
+    s0v[p] = s_acc;
+    fbv[p] = f_acc;
+    s_acc += P[p].k;
+    f_acc += P[p].F_shared ? 1 : P[p].k;
+  }
+  auto pack_one = [&](int p) {
     const ukhip_problem_t& q = P[p];
     const int m = q.m;
+    const int64_t k = q.k;
+    const int64_t s0 = s0v[p], fb = fbv[p];
     const int kw = q.K_type == 'w' ? n : n * n;
     const int cw = q.C_type == 'w' ? m : m * m;
-    for (int64_t i = 0; i < q.k; ++i, ++s) {
-      st.first()[s] = i == 0;
-      if (i > 0) {
-        std::memcpy(st.F() + s * nn, q.F + (q.F_shared ? 0 : i * nn), nn * 8);
-        if (!st.H_ident) {
-          if (q.H) std::memcpy(st.H() + s * nn, q.H + i * nn, nn * 8);
-          else for (int d = 0; d < n; ++d) st.H()[s * nn + (size_t)d * n + d] = 1.0;
-        }
-        if (q.c) std::memcpy(st.c() + s * n, q.c + i * n, (size_t)n * 8);
-        std::memcpy(st.K() + s * Kw, q.K + i * kw, (size_t)kw * 8);
-        st.Kt()[s] = (int8_t)cov_code(q.K_type);
-      }
+    const int8_t kt = (int8_t)cov_code(q.K_type), ct = (int8_t)cov_code(q.C_type);
+    /* per-step scalars */
+    for (int64_t i = 0; i < k; ++i) {
+      const int64_t s = s0 + i;
       const int mi = q.m_i ? q.m_i[i] : m;
-      if (mi > 0) {
-        st.m()[s] = mi;
-        for (int col = 0; col < n; ++col)
-          std::memcpy(st.G() + s * (size_t)mr * n + (size_t)col * mr, q.G + i * (size_t)m * n + (size_t)col * m,
-                      (size_t)mi * 8);
-        std::memcpy(st.o() + s * (size_t)mr, q.o + i * (size_t)m, (size_t)mi * 8);
-        if (q.C_type == 'w') std::memcpy(st.C() + s * (size_t)Cw, q.C + i * (size_t)cw, (size_t)mi * 8);
-        else
-          for (int col = 0; col < mi; ++col)
-            std::memcpy(st.C() + s * (size_t)Cw + (size_t)col * mr, q.C + i * (size_t)cw + (size_t)col * m,
-                        (size_t)mi * 8);
-        st.Ct()[s] = (int8_t)cov_code(q.C_type);
+      st.first()[s] = i == 0;
+      st.Fidx()[s] = q.F_shared ? fb : fb + i;
+      st.Kt()[s] = kt;
+      st.m()[s] = mi;
+      st.Ct()[s] = ct;
+    }
+    /* evolution fields: whole problems at once (the first step's entries are
+     * copied along and never read) */
+    std::memcpy(st.F() + fb * nn, q.F, (q.F_shared ? 1 : (size_t)k) * nn * 8);
+    if (!st.H_ident) {
+      if (q.H) {
+        std::memcpy(st.H() + s0 * nn, q.H, (size_t)k * nn * 8);
+      } else {
+        std::memset(st.H() + s0 * nn, 0, (size_t)k * nn * 8);
+        for (int64_t i = 0; i < k; ++i)
+          for (int d = 0; d < n; ++d) st.H()[(s0 + i) * nn + (size_t)d * n + d] = 1.0;
       }
     }
+    if (q.c) std::memcpy(st.c() + s0 * n, q.c, (size_t)k * n * 8);
+    else std::memset(st.c() + s0 * n, 0, (size_t)k * n * 8);
+    if (kw == Kw) {
+      std::memcpy(st.K() + s0 * Kw, q.K, (size_t)k * kw * 8);
+    } else {
+      for (int64_t i = 0; i < k; ++i) std::memcpy(st.K() + (s0 + i) * Kw, q.K + i * kw, (size_t)kw * 8);
+    }
+    /* observations: one copy when the request's row count is the batch's
+     * stride, else rows 0 .. m_i - 1 step by step (rows beyond m_i are never read) */
+    if (m > 0) {
+      if (m == mr) {
+        std::memcpy(st.G() + s0 * (size_t)mr * n, q.G, (size_t)k * m * n * 8);
+        std::memcpy(st.o() + s0 * (size_t)mr, q.o, (size_t)k * m * 8);
+        if (cw == Cw) std::memcpy(st.C() + s0 * (size_t)Cw, q.C, (size_t)k * cw * 8);
+        else
+          for (int64_t i = 0; i < k; ++i) std::memcpy(st.C() + (s0 + i) * (size_t)Cw, q.C + i * (size_t)cw, (size_t)cw * 8);
+      } else {
+        for (int64_t i = 0; i < k; ++i) {
+          const int mi = st.m()[s0 + i];
+          if (mi <= 0) continue;
+          double* G = st.G() + (s0 + i) * (size_t)mr * n;
+          const double* g = q.G + i * (size_t)m * n;
+          for (int col = 0; col < n; ++col)
+            for (int r = 0; r < mi; ++r) G[(size_t)col * mr + r] = g[(size_t)col * m + r];
+          double* o = st.o() + (s0 + i) * (size_t)mr;
+          for (int r = 0; r < mi; ++r) o[r] = q.o[i * (size_t)m + r];
+          double* C = st.C() + (s0 + i) * (size_t)Cw;
+          const double* cc = q.C + i * (size_t)cw;
+          if (q.C_type == 'w') {
+            for (int r = 0; r < mi; ++r) C[r] = cc[r];
+          } else {
+            for (int col = 0; col < mi; ++col)
+              for (int r = 0; r < mi; ++r) C[(size_t)col * mr + r] = cc[(size_t)col * m + r];
+          }
+        }
+      }
+    }
+  };
+  /* problems write disjoint staging ranges: large batches pack on several threads */
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<int64_t>({(int64_t)std::min(8u, hw), (int64_t)nprob, std::max<int64_t>(1, S / 16384)});
+  if (nt <= 1) {
+    for (int p = 0; p < nprob; ++p) pack_one(p);
+  } else {
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+      pool.emplace_back([&] {
+        for (int p; (p = next.fetch_add(1)) < nprob;) pack_one(p);
+      });
+    for (auto& th : pool) th.join();
   }
   return true;
 }
@@ -1195,14 +1271,26 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-/* One request: its own copy of the inputs (packed as ukhip_problem_t), its
- * results and timestamps. */
+/* The pinned download buffers of one batch. They stay with the batch's
+ * requests until each has been waited for (the results are copied out of
+ * them once, by ukhip_server_wait). */
+struct ResultBuf {
+  double* x = nullptr;
+  double* S = nullptr;
+  size_t xb = 0, Sb = 0;
+  int refs = 0;
+};
+
+/* One request: its own copy of the inputs (packed as ukhip_problem_t), where
+ * its results are, and timestamps. */
 struct Request {
   int64_t id = 0;
   ukhip_problem_t prob{};
-  std::vector<double> F, H, c, K, G, o, C;
-  std::vector<int32_t> m_i;
-  std::vector<double> states, covs;
+  char* in = nullptr; /* the inputs prob points into: one block from the server's input pool */
+  size_t in_bytes = 0;
+  ResultBuf* res = nullptr;
+  const double* rx = nullptr; /* k*n estimates in res */
+  const double* rS = nullptr; /* k*n*n covariances in res, or NULL */
   Clock::time_point t_submit, t_done;
   int status = 0; /* 0 queued, 1 running, 2 done, -1 failed */
   std::string error;
@@ -1219,6 +1307,7 @@ struct InFlight {
   double* host_S = nullptr; /* pinned: steps * n * n (FP64 mode) */
   size_t bytes = 0;
   int64_t b_steps = 0;
+  Clock::time_point t_start;
 };
 
 /* Device bytes of a batch of requests of state dimension n (the layout of
@@ -1236,6 +1325,7 @@ struct ukhip_server {
   ukhip_server_options_t opt{};
   DevArena arena;
   PinnedPool pinned;
+  PinnedPool inputs; /* pageable: requests' copies of their inputs */
   std::vector<hipStream_t> streams;
   std::vector<bool> slot_busy;
   std::mutex mu;
@@ -1247,18 +1337,41 @@ struct ukhip_server {
   std::thread worker;
   int64_t next_id = 0;
   size_t in_flight_bytes = 0;
+  bool log = false; /* UKHIP_SERVER_LOG set: one stderr line per batch */
+  Clock::time_point t0 = Clock::now();
 };
 
+static double ms_since(Clock::time_point a, Clock::time_point b = Clock::now()) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+/* Drops one request's reference to its batch's result buffers (lock held). */
+static void server_unref(ukhip_server* s, ResultBuf* rb) {
+  if (!rb || --rb->refs > 0) return;
+  s->pinned.put(rb->x, rb->xb);
+  s->pinned.put(rb->S, rb->Sb);
+  delete rb;
+}
+
+/* Returns a request's input block to the pool (its batch has been packed). */
+static void release_inputs(ukhip_server* s, Request* r) {
+  if (r->in) s->inputs.put(r->in, r->in_bytes);
+  r->in = nullptr;
+}
+
 static void server_fail(ukhip_server* s, Request* r, const std::string& why) {
+  release_inputs(s, r);
   r->status = -1;
   r->error = why;
   r->t_done = Clock::now();
-  (void)s;
 }
 
-/* Starts one batch from the queue (lock held by the caller). Returns false
- * when nothing could be admitted. */
-static bool server_admit(ukhip_server* s) {
+/* Starts one batch from the queue. Called with the lock held; the lock is
+ * released while the batch is packed, uploaded and launched (only this
+ * scheduler thread allocates from the arena and starts batches, and the
+ * stream slot is reserved first), so submissions and waits do not stall
+ * behind it. Returns false when nothing could be admitted. */
+static bool server_admit(ukhip_server* s, std::unique_lock<std::mutex>& lk) {
   if (s->queue.empty()) return false;
   int slot = -1;
   for (size_t i = 0; i < s->slot_busy.size(); ++i)
@@ -1300,9 +1413,14 @@ static bool server_admit(ukhip_server* s) {
     probs.push_back(r->prob);
     r->status = 1;
   }
+  s->slot_busy[slot] = true;
+  const size_t queued = s->queue.size();
+  lk.unlock();
+  const Clock::time_point ta = Clock::now();
   std::vector<int64_t> lengths;
   f->st.pool = &s->pinned;
   bool ok = pack_problems((int32_t)probs.size(), probs.data(), f->st, lengths);
+  const Clock::time_point tp = Clock::now();
   if (ok) {
     f->bytes = batch_device_bytes(n, f->st.mr, f->st.steps, mixed);
     f->b = batch_from_staging(f->st, lengths, st, &s->arena, false);
@@ -1327,6 +1445,8 @@ static bool server_admit(ukhip_server* s) {
     s->pinned.put(f->host_S, S * nn * 8);
     if (f->done) (void)hipEventDestroy(f->done);
     delete f;
+    lk.lock();
+    s->slot_busy[slot] = false;
     if (arena_full && !s->flight.empty()) { /* fragmented arena: retry after a batch retires */
       for (auto it = take.rbegin(); it != take.rend(); ++it) {
         (*it)->status = 0;
@@ -1339,7 +1459,13 @@ static bool server_admit(ukhip_server* s) {
     return true;
   }
   f->b_steps = f->st.steps;
-  s->slot_busy[slot] = true;
+  if (s->log)
+    std::fprintf(stderr, "[ukhip server] %.3f ms: batch n=%d requests=%zu steps=%lld pack %.3f ms enqueue %.3f ms, "
+                 "%zu still queued, slot %d\n", ms_since(s->t0, ta), n, take.size(), (long long)f->b_steps,
+                 ms_since(ta, tp), ms_since(tp), queued, slot);
+  f->t_start = ta;
+  for (auto* r : take) release_inputs(s, r); /* packed and uploading from staging */
+  lk.lock();
   s->in_flight_bytes += f->bytes;
   s->flight.push_back(f);
   return true;
@@ -1355,20 +1481,35 @@ static bool server_reap(ukhip_server* s) {
     const bool ok = q == hipSuccess;
     const int n = f->b->n;
     const size_t nn = (size_t)n * n;
+    const size_t S = (size_t)f->b_steps;
+    ResultBuf* rb = nullptr;
+    if (ok) {
+      rb = new ResultBuf();
+      rb->x = f->host_x;
+      rb->S = f->host_S;
+      rb->xb = S * n * 8;
+      rb->Sb = S * nn * 8;
+      rb->refs = (int)f->reqs.size();
+    }
     for (size_t p = 0; p < f->reqs.size(); ++p) {
       Request* r = f->reqs[p];
       if (!ok) { server_fail(s, r, hipGetErrorString(q)); continue; }
-      const int64_t o = f->b->orig_start[p], k = f->b->k[p];
-      r->states.assign(f->host_x + o * n, f->host_x + (o + k) * n);
-      if (f->host_S) r->covs.assign(f->host_S + o * nn, f->host_S + (o + k) * nn);
+      const int64_t o = f->b->orig_start[p];
+      r->res = rb;
+      r->rx = f->host_x + o * n;
+      r->rS = f->host_S ? f->host_S + o * nn : nullptr;
       r->t_done = Clock::now();
       r->status = 2;
     }
+    if (s->log)
+      std::fprintf(stderr, "[ukhip server] %.3f ms: slot %d done, %lld steps in %.3f ms\n", ms_since(s->t0), f->slot,
+                   (long long)f->b_steps, ms_since(f->t_start));
     batch_release(f->b);
     staging_free(f->st);
-    const size_t S = (size_t)f->b_steps;
-    s->pinned.put(f->host_x, S * n * 8);
-    s->pinned.put(f->host_S, S * nn * 8);
+    if (!ok) {
+      s->pinned.put(f->host_x, S * n * 8);
+      s->pinned.put(f->host_S, S * nn * 8);
+    }
     (void)hipEventDestroy(f->done);
     s->slot_busy[f->slot] = false;
     s->in_flight_bytes -= f->bytes;
@@ -1384,7 +1525,7 @@ static void server_loop(ukhip_server* s) {
   while (true) {
     const bool reaped = server_reap(s);
     if (reaped) s->cv_done.notify_all();
-    while (server_admit(s)) {
+    while (server_admit(s, lk)) {
     }
     if (s->stop && s->queue.empty() && s->flight.empty()) break;
     /* in flight: poll the events at a short interval; idle: sleep until a submission */
@@ -1420,14 +1561,10 @@ extern "C" ukhip_server_t* ukhip_server_create(const ukhip_server_options_t* opt
     return nullptr;
   }
   s->slot_busy.assign(s->streams.size(), false);
+  s->inputs.pageable = true;
+  s->log = std::getenv("UKHIP_SERVER_LOG") != nullptr;
   s->worker = std::thread(server_loop, s);
   return s;
-}
-
-template <class T>
-static void copy_in(std::vector<T>& dst, const T* src, size_t count) {
-  if (src) dst.assign(src, src + count);
-  else dst.clear();
 }
 
 extern "C" int64_t ukhip_server_submit(ukhip_server_t* s, const ukhip_problem_t* p) {
@@ -1436,25 +1573,39 @@ extern "C" int64_t ukhip_server_submit(ukhip_server_t* s, const ukhip_problem_t*
     uk_set_error("ukhip_server_submit: need 1 <= n <= %d, k >= 1, 0 <= m <= n", UKHIP_MAX_N);
     return -1;
   }
-  Request* r = new Request();
+  /* the request's own copy of its inputs, in one block reused from the
+   * server's pool (once warm, the copy runs without page faults) */
   const size_t n = p->n, m = p->m, k = p->k;
-  copy_in(r->F, p->F, (p->F_shared ? 1 : k) * n * n);
-  copy_in(r->H, p->H, k * n * n);
-  copy_in(r->c, p->c, k * n);
-  copy_in(r->K, p->K, k * (p->K_type == 'w' ? n : n * n));
-  copy_in(r->G, p->G, k * m * n);
-  copy_in(r->o, p->o, k * m);
-  copy_in(r->C, p->C, k * (p->C_type == 'w' ? m : m * m));
-  copy_in(r->m_i, p->m_i, k);
+  const void* src[8] = {p->F, p->H, p->c, p->K, p->G, p->o, p->C, p->m_i};
+  const size_t bytes[8] = {(p->F_shared ? 1 : k) * n * n * 8, k * n * n * 8, k * n * 8,
+                           k * (p->K_type == 'w' ? n : n * n) * 8, k * m * n * 8, k * m * 8,
+                           k * (p->C_type == 'w' ? m : m * m) * 8, k * 4};
+  size_t off[8], total = 0;
+  for (int f = 0; f < 8; ++f) {
+    off[f] = total;
+    if (src[f]) total += (bytes[f] + 63) & ~(size_t)63;
+  }
+  Request* r = new Request();
+  r->in_bytes = std::max<size_t>(total, 64);
+  r->in = (char*)s->inputs.get(r->in_bytes);
+  if (!r->in) {
+    delete r;
+    return -1;
+  }
+  const void* dst[8];
+  for (int f = 0; f < 8; ++f) {
+    dst[f] = src[f] ? r->in + off[f] : nullptr;
+    if (src[f]) std::memcpy(r->in + off[f], src[f], bytes[f]);
+  }
   r->prob = *p;
-  r->prob.F = r->F.empty() ? nullptr : r->F.data();
-  r->prob.H = r->H.empty() ? nullptr : r->H.data();
-  r->prob.c = r->c.empty() ? nullptr : r->c.data();
-  r->prob.K = r->K.empty() ? nullptr : r->K.data();
-  r->prob.G = r->G.empty() ? nullptr : r->G.data();
-  r->prob.o = r->o.empty() ? nullptr : r->o.data();
-  r->prob.C = r->C.empty() ? nullptr : r->C.data();
-  r->prob.m_i = r->m_i.empty() ? nullptr : r->m_i.data();
+  r->prob.F = (const double*)dst[0];
+  r->prob.H = (const double*)dst[1];
+  r->prob.c = (const double*)dst[2];
+  r->prob.K = (const double*)dst[3];
+  r->prob.G = (const double*)dst[4];
+  r->prob.o = (const double*)dst[5];
+  r->prob.C = (const double*)dst[6];
+  r->prob.m_i = (const int32_t*)dst[7];
   {
     std::lock_guard<std::mutex> g(s->mu);
     r->id = s->next_id++;
@@ -1487,16 +1638,19 @@ extern "C" int ukhip_server_wait(ukhip_server_t* s, int64_t id, double* states, 
   if (r->status < 0) {
     uk_set_error("request %lld: %s", (long long)id, r->error.c_str());
     rc = -1;
-  } else {
-    if (states) std::memcpy(states, r->states.data(), r->states.size() * 8);
+  } else { /* r->res stays allocated while this request holds a reference */
+    const size_t kn = (size_t)r->prob.k * r->prob.n, knn = kn * r->prob.n;
+    if (states) std::memcpy(states, r->rx, kn * 8);
     if (covs) {
-      if (r->covs.empty()) {
-        const size_t cnt = (size_t)r->prob.k * r->prob.n * r->prob.n;
-        for (size_t i = 0; i < cnt; ++i) covs[i] = kalman_nan;
+      if (!r->rS) {
+        for (size_t i = 0; i < knn; ++i) covs[i] = kalman_nan;
       } else {
-        std::memcpy(covs, r->covs.data(), r->covs.size() * 8);
+        std::memcpy(covs, r->rS, knn * 8);
       }
     }
+    lk.lock();
+    server_unref(s, r->res);
+    lk.unlock();
   }
   if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   delete r;
@@ -1511,9 +1665,14 @@ extern "C" void ukhip_server_destroy(ukhip_server_t* s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  for (auto& kv : s->all) delete kv.second;
+  for (auto& kv : s->all) {
+    server_unref(s, kv.second->res);
+    release_inputs(s, kv.second);
+    delete kv.second;
+  }
   for (auto st : s->streams) (void)hipStreamDestroy(st);
   (void)hipFree(s->arena.base);
   s->pinned.clear();
+  s->inputs.clear();
   delete s;
 }

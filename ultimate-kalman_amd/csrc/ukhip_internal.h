@@ -26,7 +26,8 @@ typedef struct {
   int32_t H_ident;  /* 1: every H_i is the identity (H not stored) */
   int32_t K_full;   /* 1: K stored n*n per step, else n (all 'w') */
   int32_t C_full;   /* 1: C stored mr*mr per step, else mr (all 'w') */
-  const double* F;  /* steps * n*n            (first step of a problem unused) */
+  const double* F;  /* F blocks, n*n each: step s uses block Fidx[s] (first step of a problem unused) */
+  const int64_t* Fidx; /* steps: F block of the step (a problem with time-invariant F has one block) */
   const double* H;  /* steps * n*n or NULL */
   const double* c;  /* steps * n */
   const double* K;  /* steps * (n*n | n) */
