@@ -72,7 +72,9 @@ def kernel_label(slot, pr, cov=True):
             all_w = all(t in (None, "w") for t in pr.K_type) and all(t in (None, "w") for t in pr.C_type)
         return "k_weigh_diag" if all_w else f"k_weigh<{bucket(max(n, max(m_list_of(pr))))}>"
     if slot == "k_backsub":
-        return f"k_backsub_mfma<{nb}>" if cov and nb >= 32 else f"k_backsub<{nb}>"
+        if nb >= 32:
+            return f"k_backsub_mfma<{nb}>" if cov else f"k_estimate<{nb}>"
+        return f"k_backsub<{nb}>"
     return f"{slot}<{nb}, double>"
 
 

@@ -85,3 +85,28 @@ def test_ill_conditioned_n64(dev, oracle):
     ev = max(max_rel_err(list(xv), ox), max_rel_err(list(cv), oc))
     say(f"ill-conditioned n=64: matrix cores {em:.2e}, vector {ev:.2e}")
     assert em < TOL and ev < TOL
+
+
+@pytest.mark.parametrize("n,m", [(17, 5), (40, 20), (64, 32)])
+def test_estimates_only_path(dev, oracle, n, m):
+    """KALMAN_NO_COVARIANCE (one wave per step, k_estimate) gives the
+    estimates of the covariance path (another summation order: within 1e-12)
+    and of the oracle (1e-9)."""
+    from ukhip import capi
+    F = 0.97 * P.random_orthogonal(P.rng_for(500 + n), n)
+    probs = [P.iid_problem(k, n, m, seed=600 + n + k, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
+                           unobserved=lambda i: i % 10 == 9, name=f"n{n}m{m}_k{k}") for k in (1, 2, 9, 301)]
+    b = dev.Batch(probs)
+    try:
+        b.smooth("oddeven")
+        full = [b.download(i, want_cov=False)[0] for i in range(len(probs))]
+        b.smooth("oddeven", options=capi.KALMAN_NO_COVARIANCE)
+        est = [b.download(i, want_cov=False)[0] for i in range(len(probs))]
+    finally:
+        b.free()
+    for i, pr in enumerate(probs):
+        d = max_rel_err(list(est[i]), list(full[i]))
+        ox, _ = oracle.oddeven(pr, capi.KALMAN_NO_COVARIANCE)
+        e = max_rel_err(list(est[i]), ox)
+        say(f"  {pr.name}: estimates-only vs covariance path {d:.2e}, vs oracle {e:.2e}")
+        assert d < 1e-12 and e < TOL, pr.name

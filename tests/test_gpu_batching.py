@@ -95,11 +95,12 @@ def test_graph_replays_match_eager(dev, mixed):
             b.smooth("oddeven", options=capi.KALMAN_NO_COVARIANCE)
             L.ukhip_use_graphs(1)
             say(f"  replay {rep} identical to eager")
-        # the no-covariance graph is captured separately and leaves estimates unchanged
+        # the no-covariance graph is captured separately; its estimates are the
+        # covariance path's to rounding (for n > 16 another kernel computes them)
         b.smooth("oddeven", options=capi.KALMAN_NO_COVARIANCE)
         for i in range(len(mixed)):
             x, _ = b.download(i, want_cov=False)
-            assert np.array_equal(x, ref[i][0])
+            assert max_rel_err(list(x), list(ref[i][0])) < 1e-12
     finally:
         L.ukhip_use_graphs(1)
         b.free()

@@ -712,6 +712,67 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
 }
 
 /* ------------------------------------------------------------------------ */
+/* estimates only (KALMAN_NO_COVARIANCE), n > 16                               */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+/*
+ * x_j = R^{-1} (cf - Bt x_{j-1} - Y x_{j+1}) for one even step per wave: lane
+ * i owns row i. Bt and Y stream column by column (coalesced), the
+ * neighbours' estimates and each solved x_q reach every lane by readlane,
+ * and lane i keeps row i of R in registers for the substitution.
+ */
+template <int NB>
+__global__ void __launch_bounds__(64) k_estimate(uk_level_t lv, uk_factors_t fac, double* __restrict__ xout, int n) {
+  const int i = threadIdx.x;
+  const int64_t ev = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, ev);
+  const int64_t u = ev - lv.even_start[p];
+  const int64_t j = 2 * u;
+  const int64_t L = lv.L[p];
+  const bool has_m = j > 0, has_p = j + 1 < L;
+  const int64_t nn = (int64_t)n * n;
+  const int64_t ob = lv.orig_start[lv.prob[p]];
+  const int64_t stride = (int64_t)1 << lv.level;
+  const int64_t oj = ob + (j + 1) * stride - 1;
+  const int64_t om = ob + j * stride - 1;
+  const int64_t op = ob + (j + 2) * stride - 1;
+  const double* Rg = fac.R + ev * nn;
+  const double* Bg = fac.Bt + ev * nn;
+  const double* Yg = fac.Y + ev * nn;
+  const bool row = i < n;
+  double rrow[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) rrow[q] = (row && q < n && q >= i) ? Rg[i + (int64_t)q * n] : 0.0;
+  const double rinv = row ? 1.0 / Rg[i + (int64_t)i * n] : 0.0;
+  const double xm = (has_m && row) ? xout[om * n + i] : 0.0;
+  const double xp = (has_p && row) ? xout[op * n + i] : 0.0;
+  double r = row ? fac.cf[ev * n + i] : 0.0;
+#pragma unroll 8
+  for (int q = 0; q < NB; ++q) {
+    if (q < n) {
+      const double b = (has_m && row) ? Bg[i + (int64_t)q * n] : 0.0;
+      const double y = (has_p && row) ? Yg[i + (int64_t)q * n] : 0.0;
+      r = fma(-b, lane_bcast(xm, q), r);
+      r = fma(-y, lane_bcast(xp, q), r);
+    }
+  }
+#pragma unroll
+  for (int q = NB - 1; q >= 0; --q) {
+    if (q < n) {
+      const double xq = lane_bcast(r * rinv, q);
+      r = (i == q) ? xq : fma(-rrow[q], xq, r);
+    }
+  }
+  if (row) xout[oj * n + i] = r;
+}
+
+/* ------------------------------------------------------------------------ */
 /* back substitution on the matrix cores (NB >= 32, with covariances)         */
 /* ------------------------------------------------------------------------ */
 
@@ -1329,6 +1390,17 @@ extern "C" hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t
                                         const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n,
                                         int cov, hipStream_t st) {
   const int nb = uk_bucket(n);
+  if (!cov && nb >= 32) {
+#define E_(NBV) k_estimate<NBV><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, *fac, xout, n)
+    switch (nb) {
+      case 32: E_(32); break;
+      case 48: E_(48); break;
+      case 64: E_(64); break;
+      default: return hipErrorInvalidValue;
+    }
+#undef E_
+    return hipGetLastError();
+  }
   if (cov && nb >= 32 && uk_backsub_mfma_enabled()) {
 #define M_(NBV)                                                                                       \
   k_backsub_mfma<NBV><<<dim3((unsigned)lv->evens), dim3(NBV * 8), 0, st>>>(*lv, *fac, xout, Sout, Xup, XTup, \
