@@ -79,6 +79,17 @@ int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements);
 
 /* Copies problem p's estimates (k*n) and covariances (k*n*n, may be NULL) to the host. */
 int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covariances);
+/* Problem p's covariances in one of three formats, formed on the device so that
+ * only that format crosses PCIe: 'C' the n x n matrices (k*n*n), 'D' their
+ * diagonals (k*n), 'W' the inverse Cholesky factor W = L^{-1} of S = L L^T
+ * (W^T W = S^{-1}; lower triangular, column-major, k*n*n), the reference's 'W'
+ * covariance representation (c/kalman_base.c:61-175). */
+int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format, double* out);
+/* Option bit (not a reference option) for kalman_smooth_oddeven,
+ * kalman_smooth_associative and kalman_create_options: covariances are
+ * returned in the 'W' representation, covariance_type 'W', as the reference's
+ * UltimateKalman filter returns them. */
+#define UKHIP_COVARIANCE_W (1u << 24)
 
 /* Timed odd-even run: device milliseconds per kernel, measured with HIP events
  * recorded on the library stream around every launch and summed over the

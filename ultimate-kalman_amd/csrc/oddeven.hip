@@ -1299,6 +1299,50 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
   }
 }
 
+/*
+ * Covariance output formats, one step per workgroup (64 threads):
+ *   fmt 1 ('D'): the diagonal, n doubles per step;
+ *   fmt 2 ('W'): W = L^{-1} for the Cholesky factor S = L L^T, so that
+ *   W^T W = S^{-1}: the reference's 'W' representation of a covariance
+ *   (c/kalman_base.c:61-175), lower triangular, n*n per step, column-major.
+ */
+template <int NB>
+__global__ void __launch_bounds__(64) k_cov_format(const double* __restrict__ S, int n, int fmt,
+                                                   double* __restrict__ dst) {
+  __shared__ double L[NB * NB];
+  const int tid = threadIdx.x;
+  const int64_t s = blockIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  const double* Ss = S + s * nn;
+  if (fmt == 1) {
+    if (tid < n) dst[s * n + tid] = Ss[tid * (n + 1)];
+    return;
+  }
+  for (int i = tid; i < n * n; i += 64) L[i] = Ss[i];
+  __syncthreads();
+  lds_cholesky<64>(L, n, tid); /* lower factor in place, ld n (ends with a barrier) */
+  if (tid < n) {
+    const int j = tid;
+    double x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      double v = 0.0;
+      if (i < n && i >= j) {
+        double acc = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < i; ++q)
+          if (q >= j) acc = fma(-L[i + q * n], x[q], acc);
+        v = acc / L[i + i * n];
+      }
+      x[i] = v;
+    }
+    double* W = dst + s * nn + (int64_t)j * n;
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < n) W[i] = x[i];
+  }
+}
+
 } /* namespace */
 
 /* ------------------------------------------------------------------------ */
@@ -1422,5 +1466,20 @@ extern "C" hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t
     default: return hipErrorInvalidValue;
   }
 #undef L_
+  return hipGetLastError();
+}
+
+/* Covariance output formats of `steps` consecutive n x n covariances S:
+ * fmt 1 diagonal (n per step), 2 inverse Cholesky factor W (n*n per step). */
+extern "C" hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n, int fmt, double* dst,
+                                           hipStream_t st) {
+  if (steps <= 0) return hipSuccess;
+  const int nb = uk_bucket(n);
+#define C_(NBV) k_cov_format<NBV><<<dim3((unsigned)steps), dim3(64), 0, st>>>(S, n, fmt, dst)
+  switch (nb) {
+    UK_DISPATCH(8, C_) UK_DISPATCH(16, C_) UK_DISPATCH(32, C_) UK_DISPATCH(48, C_) UK_DISPATCH(64, C_)
+    default: return hipErrorInvalidValue;
+  }
+#undef C_
   return hipGetLastError();
 }

@@ -33,6 +33,7 @@ int uk_bucket(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                const uk_factors_t* fac, double* fix, int n, hipStream_t st, hipEvent_t* ev);
+hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n, int fmt, double* dst, hipStream_t st);
 hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
                              hipStream_t st);
@@ -1183,6 +1184,24 @@ extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states,
   return hip_ok(hipStreamSynchronize(g_stream), "download") ? 0 : -1;
 }
 
+extern "C" int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format, double* out) {
+  if (!b || p < 0 || p >= b->nprob || !out) { uk_set_error("bad batch, problem index or output"); return -1; }
+  if (!b->parts.empty()) return ukhip_batch_download_cov(b->parts[b->where[p].first], b->where[p].second, format, out);
+  if (format == 'C') return ukhip_batch_download(b, p, nullptr, out);
+  if (format != 'D' && format != 'W') { uk_set_error("covariance format '%c': use 'C', 'D' or 'W'", format); return -1; }
+  const size_t n = b->n, nn = n * n;
+  const int64_t o = b->orig_start[p], k = b->k[p];
+  const size_t bytes = (size_t)k * (format == 'D' ? n : nn) * 8;
+  double* d = nullptr;
+  if (!hip_ok(hipMalloc((void**)&d, bytes), "hipMalloc(covariance format)")) return -1;
+  bool ok = hip_ok(uk_launch_cov_format(b->Sout + o * nn, k, (int)n, format == 'D' ? 1 : 2, d, g_stream),
+                   "covariance format launch") &&
+            hip_ok(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, g_stream), "download") &&
+            hip_ok(hipStreamSynchronize(g_stream), "download");
+  (void)hipFree(d);
+  return ok ? 0 : -1;
+}
+
 /* Device pointers for sibling translation units (associative path). */
 extern "C" void uk_batch_view(ukhip_batch* b, const uk_raw_t** raw, double** xout, double** Sout, int64_t* steps,
                               int* nprob, const int64_t** k, void** assoc_ws, size_t* assoc_bytes) {
@@ -1229,8 +1248,10 @@ static void smooth_equations(kalman_options_t options, kalman_step_equations_t**
     return;
   }
   const int n = b->n;
+  const bool wform = cov && (options & UKHIP_COVARIANCE_W) != 0;
   std::vector<double> xs((size_t)length * n), Ss(cov ? (size_t)length * n * n : 0);
-  if (ukhip_batch_download(b, 0, xs.data(), cov ? Ss.data() : nullptr) != 0) {
+  if (ukhip_batch_download(b, 0, xs.data(), (cov && !wform) ? Ss.data() : nullptr) != 0 ||
+      (wform && ukhip_batch_download_cov(b, 0, 'W', Ss.data()) != 0)) {
     batch_release(b);
     return;
   }
@@ -1249,7 +1270,7 @@ static void smooth_equations(kalman_options_t options, kalman_step_equations_t**
     } else {
       e->covariance = NULL;
     }
-    e->covariance_type = 'C';
+    e->covariance_type = wform ? 'W' : 'C';
   }
 }
 

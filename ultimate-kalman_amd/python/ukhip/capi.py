@@ -32,6 +32,7 @@ SMOOTHER_ARGTYPES = [C.c_uint32, C.POINTER(C.POINTER(StepEquations)), C.c_int32]
 KALMAN_ALGORITHM_ODDEVEN = 1 << 2
 KALMAN_ALGORITHM_ASSOCIATIVE = 1 << 3
 KALMAN_NO_COVARIANCE = 1 << 16
+UKHIP_COVARIANCE_W = 1 << 24  # include/ukhip.h: covariances in the 'W' representation
 
 
 def _as_fortran(a):

@@ -51,6 +51,8 @@ def lib():
         L.ukhip_batch_smooth_oddeven_timed.restype = C.c_int
         L.ukhip_batch_download.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ukhip_batch_download.restype = C.c_int
+        L.ukhip_batch_download_cov.argtypes = [C.c_void_p, C.c_int32, C.c_char, C.c_void_p]
+        L.ukhip_batch_download_cov.restype = C.c_int
         L.ukhip_synchronize.restype = C.c_int
         L.ukhip_stream.restype = C.c_void_p
         L.ukhip_use_graphs.argtypes = [C.c_int]
@@ -256,6 +258,15 @@ class Batch:
         if want_cov:
             cv = np.swapaxes(cv, 1, 2)  # column-major per step
         return x, cv
+
+    def download_cov(self, i, fmt):
+        """Problem i's covariances as 'C' (k x n x n), 'D' (diagonals, k x n) or
+        'W' (inverse Cholesky factors W = L^{-1}, k x n x n, lower triangular)."""
+        p = self.problems[i]
+        n = p.n[0]
+        out = np.empty((p.k, n) if fmt == "D" else (p.k, n, n))
+        _check(lib().ukhip_batch_download_cov(self.h, i, fmt.encode(), out.ctypes.data), "download_cov")
+        return out if fmt == "D" else np.swapaxes(out, 1, 2)
 
     def free(self):
         if self.h:
