@@ -130,6 +130,35 @@ int ukhip_server_wait(ukhip_server_t* s, int64_t id, double* states, double* cov
 /* Finishes every submitted request, then frees the server. */
 void ukhip_server_destroy(ukhip_server_t* s);
 
+/* ---- batched sequential filter (UltimateKalman) ------------------------------
+ * nfilters independent filters of one state dimension n advance together, one
+ * evolve + observe step per call, one launch for all of them: the reference's
+ * KALMAN_ALGORITHM_ULTIMATE filter (c/kalman_ultimate.c:140-470: kalman_evolve,
+ * kalman_observe, kalman_estimate, kalman_covariance) batched on the device.
+ * The filter state stays in HBM; each step's inputs are host arrays holding
+ * all filters' matrices, column-major, filter after filter. */
+typedef struct ukhip_filter ukhip_filter_t;
+typedef struct {
+  const double* F;    /* nfilters * n*n, or one n*n when F_shared (ignored on the first step) */
+  int32_t F_shared;
+  const double* H;    /* nfilters * n*n, or NULL for H = I */
+  const double* c;    /* nfilters * n, or NULL for c = 0 */
+  const double* K;    /* nfilters * n: the 'w' representation (inverse standard deviations) */
+  const double* G;    /* nfilters * m*n, ld m */
+  const double* o;    /* nfilters * m */
+  const int32_t* m_i; /* nfilters observation counts (0 = unobserved), or NULL for m */
+  const double* C;    /* nfilters * m: 'w' representation */
+} ukhip_filter_inputs_t;
+ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int32_t m);
+/* One step of every filter (the first call has no evolution equation). */
+int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in);
+/* The latest filtered estimates (nfilters * n; NaN while a filter's state is
+ * not yet determined, as the reference returns) and, if R is not NULL, the
+ * covariances in the reference's 'W' representation: R (nfilters * n*n, upper
+ * triangular, column-major) with R^T R the inverse covariance. */
+int ukhip_filter_estimates(ukhip_filter_t* f, double* states, double* R);
+void ukhip_filter_free(ukhip_filter_t* f);
+
 /* The HIP stream (hipStream_t) every libukhip launch uses. */
 void* ukhip_stream(void);
 

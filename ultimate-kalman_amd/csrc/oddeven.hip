@@ -131,6 +131,22 @@ struct Col {
 #pragma unroll
     for (int l = 0; l < H; ++l) a[H + l] = a[l];
   }
+  /* block 0 <- block 1, block 1 <- 0 */
+  __device__ __forceinline__ void move_1_to_0() {
+#pragma unroll
+    for (int l = 0; l < H; ++l) {
+      a[l] = a[H + l];
+      a[H + l] = T(0);
+    }
+  }
+  /* rows r < rows of block blk scaled by sign * s[r] */
+  __device__ __forceinline__ void scale_block(int blk, const T* __restrict__ s, int rows, int sub, T sign) {
+#pragma unroll
+    for (int l = 0; l < H; ++l) {
+      const int r = SL * l + sub;
+      if (r < rows) a[blk * H + l] *= sign * s[r];
+    }
+  }
 };
 
 /* LDS reflector buffers: [buffer][sub][2H + 4] (the +4 skews the sub-lane
@@ -1343,6 +1359,120 @@ __global__ void __launch_bounds__(64) k_cov_format(const double* __restrict__ S,
   }
 }
 
+/*
+ * One step (evolve + observe) of every filter of a batch of sequential
+ * UltimateKalman filters, one workgroup per filter (c/kalman_ultimate.c:140-470):
+ *   evolve:  QR of [R_{i-1}; -V F_i] applied to [0; V H_i] and [y_{i-1}; V c_i];
+ *            rows n..2n-1 of the last two are the prediction (R~, y~);
+ *   observe: QR of [R~; W G_i] applied to [y~; W o_i] -> (R_i, y_i);
+ *   x_i = R_i^{-1} y_i once the state is determined (z_i = n), NaN before.
+ * Zero rows stand for the reference's not-yet-square ("flat") factors: they
+ * stay zero under the reflectors. Columns (SL lanes each): [0,n) u_{i-1},
+ * [n,2n) u_i, 2n right-hand side. V = diag(K), W = diag(C) ('w' covariances).
+ */
+template <int NB>
+struct FilterShared {
+  double R[NB * NB];
+  double y[NB];
+  double rinv[NB];
+};
+
+template <int NB>
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT))
+    k_filter_step(uk_filter_t st, uk_filter_in_t in, int n, int first) {
+  constexpr int SL = QRGeom<NB, 2 * NB + 1, 1>::SL;
+  constexpr int NT = QRGeom<NB, 2 * NB + 1, 1>::NT;
+  __shared__ Refl<NB, SL, double> refl;
+  __shared__ FilterShared<NB> fs;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t f = blockIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  const int mr = in.mr;
+  const bool isP = c < n, isB = c >= n && c < 2 * n, isY = c == 2 * n;
+  const int bc = isB ? c - n : 0;
+  const double* kw = in.K + f * n;
+  const int zprev = first ? 0 : st.z[f];
+  refl_clear<NB, SL, NT, double>(refl, tid);
+  Col<NB, SL, double> col;
+  col.zero_block(0);
+  col.zero_block(1);
+  if (!first) {
+    if (isP) {
+      col.load_block(0, st.R + f * nn + (int64_t)c * n, n, sub);
+      col.load_block(1, in.F + (in.F_shared ? 0 : f * nn) + (int64_t)c * n, n, sub);
+      col.scale_block(1, kw, n, sub, -1.0);
+    }
+    if (isB) {
+      if (in.H) {
+        col.load_block(1, in.H + f * nn + (int64_t)bc * n, n, sub);
+        col.scale_block(1, kw, n, sub, 1.0);
+      } else {
+#pragma unroll
+        for (int l = 0; l < NB / SL; ++l) {
+          const int r = SL * l + sub;
+          col.a[NB / SL + l] = (r == bc) ? kw[r] : 0.0;
+        }
+      }
+    }
+    if (isY) {
+      col.load_block(0, st.y + f * n, n, sub);
+      if (in.c) {
+        col.load_block(1, in.c + f * n, n, sub);
+        col.scale_block(1, kw, n, sub, 1.0);
+      }
+    }
+    house_qr<NB, SL, double>(col, refl, c, sub, 0, n, isP || isB || isY);
+    if (isB || isY) col.move_1_to_0();
+  }
+  const int m = in.m_i ? in.m_i[f] : mr;
+  if (m > 0) {
+    const double* cw = in.C + f * mr;
+    if (isB) {
+      col.load_block(1, in.G + f * mr * n + (int64_t)bc * mr, m, sub);
+      col.scale_block(1, cw, m, sub, 1.0);
+    }
+    if (isY) {
+      col.load_block(1, in.o + f * mr, m, sub);
+      col.scale_block(1, cw, m, sub, 1.0);
+    }
+  }
+  house_qr<NB, SL, double>(col, refl, c, sub, n, n, isB || isY);
+  if (isB) {
+    col.store_triu(st.R + f * nn + (int64_t)bc * n, n, bc, sub);
+#pragma unroll
+    for (int l = 0; l < NB / SL; ++l) {
+      const int r = SL * l + sub;
+      fs.R[r + bc * NB] = (r <= bc && r < n) ? col.a[l] : (r == bc ? 1.0 : 0.0);
+    }
+  }
+  if (isY) {
+    col.store_block(0, st.y + f * n, n, sub);
+#pragma unroll
+    for (int l = 0; l < NB / SL; ++l) {
+      const int r = SL * l + sub;
+      fs.y[r] = r < n ? col.a[l] : 0.0;
+    }
+  }
+  const int z = min(n, zprev + m);
+  __syncthreads();
+  if (tid < n) fs.rinv[tid] = 1.0 / fs.R[tid + tid * NB];
+  __syncthreads();
+  if (tid < 64) {
+    double r = tid < n ? fs.y[tid] : 0.0;
+    if (z == n) {
+      for (int q = n - 1; q >= 0; --q) {
+        const double xq = __shfl(r, q) * fs.rinv[q];
+        if (tid == q) r = xq;
+        else if (tid < q) r -= fs.R[tid + q * NB] * xq;
+      }
+    } else {
+      r = __builtin_nan("");
+    }
+    if (tid < n) st.x[f * n + tid] = r;
+  }
+  if (tid == 0) st.z[f] = z;
+}
+
 } /* namespace */
 
 /* ------------------------------------------------------------------------ */
@@ -1481,5 +1611,19 @@ extern "C" hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n
     default: return hipErrorInvalidValue;
   }
 #undef C_
+  return hipGetLastError();
+}
+
+/* One step of every filter of a batch (k_filter_step). */
+extern "C" hipError_t uk_launch_filter_step(const uk_filter_t* st, const uk_filter_in_t* in, int nf, int n, int first,
+                                            hipStream_t s) {
+  const int nb = uk_bucket(n > in->mr ? n : in->mr);
+#define F_(NBV) \
+  k_filter_step<NBV><<<dim3((unsigned)nf), dim3(QRGeom<NBV, 2 * NBV + 1, 1>::NT), 0, s>>>(*st, *in, n, first)
+  switch (nb) {
+    UK_DISPATCH(8, F_) UK_DISPATCH(16, F_) UK_DISPATCH(32, F_) UK_DISPATCH(48, F_) UK_DISPATCH(64, F_)
+    default: return hipErrorInvalidValue;
+  }
+#undef F_
   return hipGetLastError();
 }

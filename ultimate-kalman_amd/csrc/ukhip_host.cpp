@@ -33,6 +33,8 @@ int uk_bucket(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                const uk_factors_t* fac, double* fix, int n, hipStream_t st, hipEvent_t* ev);
+hipError_t uk_launch_filter_step(const uk_filter_t* st, const uk_filter_in_t* in, int nf, int n, int first,
+                                 hipStream_t s);
 hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n, int fmt, double* dst, hipStream_t st);
 hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
@@ -1696,4 +1698,110 @@ extern "C" void ukhip_server_destroy(ukhip_server_t* s) {
   s->pinned.clear();
   s->inputs.clear();
   delete s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched sequential filter                                                  */
+/* ------------------------------------------------------------------------ */
+
+struct ukhip_filter {
+  int nf = 0, n = 0, mr = 0;
+  int64_t steps = 0;
+  uk_filter_t st{};
+  void* d_state = nullptr;
+  char* d_in = nullptr;
+  char* h_in = nullptr; /* pinned staging of one step's inputs */
+  size_t in_bytes = 0;
+};
+
+extern "C" ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int32_t m) {
+  if (!ensure_stream()) return nullptr;
+  if (nfilters < 1 || n < 1 || n > UKHIP_MAX_N || m < 0 || m > n) {
+    uk_set_error("ukhip_filter_create: need nfilters >= 1, 1 <= n <= %d, 0 <= m <= n", UKHIP_MAX_N);
+    return nullptr;
+  }
+  ukhip_filter* f = new ukhip_filter();
+  f->nf = nfilters;
+  f->n = n;
+  f->mr = std::max(m, 1);
+  const size_t P = (size_t)nfilters, nn = (size_t)n * n, mr = (size_t)f->mr;
+  const size_t state_bytes = P * (nn + 2 * (size_t)n) * 8 + P * 4;
+  /* one step's inputs: F, H, c, K, G, o, C, m_i */
+  f->in_bytes = P * (2 * nn + 2 * (size_t)n + mr * n + 2 * mr) * 8 + P * 4;
+  bool ok = hip_ok(hipMalloc(&f->d_state, state_bytes), "hipMalloc(filter state)") &&
+            hip_ok(hipMemsetAsync(f->d_state, 0, state_bytes, g_stream), "hipMemset(filter state)") &&
+            hip_ok(hipMalloc((void**)&f->d_in, f->in_bytes), "hipMalloc(filter inputs)") &&
+            hip_ok(hipHostMalloc((void**)&f->h_in, f->in_bytes, hipHostMallocDefault), "hipHostMalloc(filter inputs)");
+  if (!ok) {
+    ukhip_filter_free(f);
+    return nullptr;
+  }
+  char* d = (char*)f->d_state;
+  f->st.R = (double*)d;
+  f->st.y = (double*)(d + P * nn * 8);
+  f->st.x = (double*)(d + P * (nn + n) * 8);
+  f->st.z = (int32_t*)(d + P * (nn + 2 * (size_t)n) * 8);
+  return f;
+}
+
+extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in) {
+  if (!f || !in) { uk_set_error("NULL filter or inputs"); return -1; }
+  const bool first = f->steps == 0;
+  const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n, mr = (size_t)f->mr;
+  if (!in->K && !first) { uk_set_error("ukhip_filter_step: K is required after the first step"); return -1; }
+  if (!first && !in->F) { uk_set_error("ukhip_filter_step: F is required after the first step"); return -1; }
+  bool observed = in->m_i == nullptr;
+  for (size_t p = 0; in->m_i && p < P; ++p) {
+    if (in->m_i[p] < 0 || in->m_i[p] > f->mr) { uk_set_error("ukhip_filter_step: m_i outside 0..m"); return -1; }
+    observed = observed || in->m_i[p] > 0;
+  }
+  if (observed && (!in->G || !in->o || !in->C)) { uk_set_error("ukhip_filter_step: G, o and C are required"); return -1; }
+  /* the previous step's upload reads the staging buffer */
+  if (!hip_ok(hipStreamSynchronize(g_stream), "ukhip_filter_step")) return -1;
+  size_t off = 0;
+  uk_filter_in_t din{};
+  auto put = [&](const void* src, size_t bytes) -> const void* {
+    if (!src) return nullptr;
+    std::memcpy(f->h_in + off, src, bytes);
+    const void* dp = f->d_in + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return dp;
+  };
+  const size_t mcols = (size_t)f->mr; /* rows of G, o, C per filter as given: ld m */
+  din.F = (const double*)put(first ? nullptr : in->F, (in->F_shared ? 1 : P) * nn * 8);
+  din.F_shared = in->F_shared;
+  din.H = (const double*)put(first ? nullptr : in->H, P * nn * 8);
+  din.c = (const double*)put(first ? nullptr : in->c, P * n * 8);
+  din.K = (const double*)put(in->K, P * n * 8);
+  din.G = (const double*)put(in->G, P * mcols * n * 8);
+  din.o = (const double*)put(in->o, P * mcols * 8);
+  din.C = (const double*)put(in->C, P * mcols * 8);
+  din.m_i = (const int32_t*)put(in->m_i, P * 4);
+  din.mr = (int32_t)mr;
+  if (!din.K) din.K = f->st.y; /* first step: K is never read */
+  if (off > f->in_bytes + 8 * 256) { uk_set_error("ukhip_filter_step: staging overflow"); return -1; }
+  if (!hip_ok(hipMemcpyAsync(f->d_in, f->h_in, off, hipMemcpyHostToDevice, g_stream), "filter upload") ||
+      !hip_ok(uk_launch_filter_step(&f->st, &din, f->nf, f->n, first ? 1 : 0, g_stream), "filter step launch"))
+    return -1;
+  ++f->steps;
+  return 0;
+}
+
+extern "C" int ukhip_filter_estimates(ukhip_filter_t* f, double* states, double* R) {
+  if (!f) { uk_set_error("NULL filter"); return -1; }
+  const size_t P = (size_t)f->nf, n = (size_t)f->n;
+  if (states && !hip_ok(hipMemcpyAsync(states, f->st.x, P * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
+    return -1;
+  if (R && !hip_ok(hipMemcpyAsync(R, f->st.R, P * n * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(g_stream), "download") ? 0 : -1;
+}
+
+extern "C" void ukhip_filter_free(ukhip_filter_t* f) {
+  if (!f) return;
+  (void)hipStreamSynchronize(g_stream);
+  if (f->d_state) (void)hipFree(f->d_state);
+  if (f->d_in) (void)hipFree(f->d_in);
+  if (f->h_in) (void)hipHostFree(f->h_in);
+  delete f;
 }

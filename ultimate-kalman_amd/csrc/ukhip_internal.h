@@ -96,4 +96,33 @@ typedef struct {
   const int64_t* orig_start; /* indexed by batch problem */
 } uk_level_t;
 
+/* Batched sequential filter (UltimateKalman, c/kalman_ultimate.c:140-470):
+ * per filter the information factor R (n x n, upper triangular, zero rows
+ * while the state is undetermined), its right-hand side y (n), the number of
+ * determined rows z, and the estimate x (n). */
+typedef struct {
+  double* R;
+  double* y;
+  int32_t* z;
+  double* x;
+} uk_filter_t;
+
+/* One step's inputs for all filters (device pointers): F (n*n per filter, or
+ * one when F_shared), H (n*n per filter, or NULL for I), c (n per filter, or
+ * NULL for 0), K ('w': n inverse standard deviations per filter), G (mr*n per
+ * filter, ld mr), o (mr), m_i (observation rows per filter, or NULL for mr),
+ * C ('w': mr per filter). */
+typedef struct {
+  const double* F;
+  const double* H;
+  const double* c;
+  const double* K;
+  const double* G;
+  const double* o;
+  const int32_t* m_i;
+  const double* C;
+  int32_t F_shared;
+  int32_t mr;
+} uk_filter_in_t;
+
 #endif

@@ -4,6 +4,7 @@ Loads the in-tree ultimate-kalman_amd/libukhip.so. There is no fallback: if
 the library or a HIP device is missing, the calls fail loudly.
 """
 import ctypes as C
+import ctypes as _ctypes  # for methods whose arguments shadow C
 import os
 
 import numpy as np
@@ -67,6 +68,13 @@ def lib():
         L.ukhip_server_wait.restype = C.c_int
         L.ukhip_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
         L.ukhip_server_destroy.argtypes = [C.c_void_p]
+        L.ukhip_filter_create.restype = C.c_void_p
+        L.ukhip_filter_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+        L.ukhip_filter_step.argtypes = [C.c_void_p, C.c_void_p]
+        L.ukhip_filter_step.restype = C.c_int
+        L.ukhip_filter_estimates.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ukhip_filter_estimates.restype = C.c_int
+        L.ukhip_filter_free.argtypes = [C.c_void_p]
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
@@ -139,6 +147,73 @@ def problem_struct(p, keep):
     s.C_type = pk["C_type"].encode()
     s.C = keep.ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
     return s
+
+
+class FilterInputs(C.Structure):
+    """ukhip_filter_inputs_t (include/ukhip.h)."""
+    _fields_ = [("F", C.c_void_p), ("F_shared", C.c_int32), ("H", C.c_void_p), ("c", C.c_void_p),
+                ("K", C.c_void_p), ("G", C.c_void_p), ("o", C.c_void_p), ("m_i", C.c_void_p), ("C", C.c_void_p)]
+
+
+class Filter:
+    """Batched sequential UltimateKalman filter (ukhip_filter_*): nfilters
+    filters of state dimension n, m observation rows per step at most."""
+
+    def __init__(self, nfilters, n, m):
+        L = lib()
+        L.ukhip_clear_error()
+        self.h = L.ukhip_filter_create(nfilters, n, m)
+        if not self.h:
+            raise UkhipError("ukhip_filter_create: " + last_error())
+        self.nf, self.n, self.m = nfilters, n, m
+
+    def step(self, K, G, o, C, F=None, H=None, c=None, m_i=None):
+        """One evolve + observe step. Arrays hold all filters: F (nf, n, n) or
+        (n, n) shared, H/c optional, K (nf, n) and C (nf, m) inverse standard
+        deviations, G (nf, m, n), o (nf, m), m_i (nf,) observation counts."""
+        keep = []
+
+        def ptr(a, dtype=np.float64):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data
+
+        def colmajor(a):  # (..., rows, cols) -> column-major per matrix
+            return None if a is None else np.swapaxes(np.asarray(a, dtype=np.float64), -1, -2)
+
+        s = FilterInputs()
+        Fa = None if F is None else np.asarray(F, dtype=np.float64)
+        s.F_shared = 1 if (Fa is not None and Fa.ndim == 2) else 0
+        s.F = ptr(colmajor(Fa))
+        s.H = ptr(colmajor(H))
+        s.c = ptr(c)
+        s.K = ptr(K)
+        s.G = ptr(colmajor(G))
+        s.o = ptr(o)
+        s.C = ptr(C)
+        s.m_i = ptr(m_i, np.int32)
+        _check(lib().ukhip_filter_step(self.h, _ctypes.byref(s)), "ukhip_filter_step")
+
+    def estimates(self, want_R=True):
+        """(states (nf, n), R (nf, n, n) upper triangular with R^T R the inverse covariance)."""
+        x = np.empty((self.nf, self.n))
+        R = np.empty((self.nf, self.n, self.n)) if want_R else None
+        _check(lib().ukhip_filter_estimates(self.h, x.ctypes.data, R.ctypes.data if want_R else None),
+               "ukhip_filter_estimates")
+        return x, (np.swapaxes(R, 1, 2) if want_R else None)
+
+    def close(self):
+        if self.h:
+            lib().ukhip_filter_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServerOptions(C.Structure):
