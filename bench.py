@@ -301,6 +301,39 @@ def run_serving(args):
     return out
 
 
+def run_filter(args):
+    """The batched sequential filter: --filter-count UltimateKalman filters
+    (n = --filter-n, m = n/2, F = 0.98 Q per filter, G ~ N(0,1/n) per step,
+    K = I, C = 0.1 I as 'w' inverse standard deviations) advanced
+    --filter-steps steps; each step uploads every filter's inputs from host
+    memory (PCIe included) and runs one launch."""
+    nf, n = args.filter_count, args.filter_n
+    m = max(1, n // 2)
+    rng = P.rng_for(4242)
+    F = np.stack([0.98 * P.random_orthogonal(P.rng_for(5000 + p), n) for p in range(min(nf, 64))])
+    F = np.concatenate([F] * ((nf + len(F) - 1) // len(F)))[:nf]
+    K = np.ones((nf, n))
+    Cw = np.full((nf, m), 1 / np.sqrt(0.1))
+    steps = [dict(G=rng.standard_normal((nf, m, n)) / np.sqrt(n), o=rng.standard_normal((nf, m)))
+             for _ in range(8)]
+    flt = device.Filter(nf, n, m)
+    for i in range(2):
+        flt.step(F=F, K=K, C=Cw, **steps[i % 8])
+    device.lib().ukhip_synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.filter_steps):
+        flt.step(F=F, K=K, C=Cw, **steps[i % 8])
+    x, _ = flt.estimates(want_R=False)
+    wall = time.perf_counter() - t0
+    flt.close()
+    assert np.all(np.isfinite(x))
+    print(f"[bench] filter: {nf} filters n={n} m={m}, {args.filter_steps} steps in {wall:.3f} s", file=sys.stderr,
+          flush=True)
+    return {"filters": nf, "n": n, "m": m, "steps": args.filter_steps, "ms_per_step": wall * 1e3 / args.filter_steps,
+            "value": nf * args.filter_steps / wall, "unit": "filter-steps/s",
+            "note": "host inputs uploaded every step (PCIe included)"}
+
+
 def device_bytes_per_step(n, m):
     """libukhip's odd-even device memory per step (inputs, two chains, factors,
     cross covariances, outputs), in bytes."""
@@ -416,6 +449,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-precision", action="store_true", help="skip the n=64 precision workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-filter", action="store_true", help="skip the batched sequential filter")
+    ap.add_argument("--filter-count", type=int, default=4096)
+    ap.add_argument("--filter-n", type=int, default=16)
+    ap.add_argument("--filter-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-pool", type=int, default=250)
     ap.add_argument("--serve-budget-gb", type=float, default=32.0)
@@ -540,6 +577,8 @@ def main():
         out["precision_workload"] = precision_workload(args)
     if not args.no_serving:
         out["serving"] = run_serving(args)
+    if not args.no_filter:
+        out["sequential_filter"] = run_filter(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_k)
