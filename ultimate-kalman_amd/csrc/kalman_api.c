@@ -103,40 +103,30 @@ kalman_matrix_t* kalman_covariance_matrix_explicit(kalman_matrix_t* cov, char ty
 /* kalman_t over explicit steps                                               */
 /* ------------------------------------------------------------------------ */
 
+/* The steps live in an ingest object (ukhip_host.cpp): kalman_evolve and
+ * kalman_observe write each step's matrices once into the device layout's
+ * fields, kalman_smooth uploads them as they are. */
+typedef struct uk_ingest uk_ingest_t;
+uk_ingest_t* uk_ingest_create(void);
+void uk_ingest_free(uk_ingest_t* g);
+int64_t uk_ingest_count(const uk_ingest_t* g);
+int64_t uk_ingest_first(const uk_ingest_t* g);
+int32_t uk_ingest_dimension(const uk_ingest_t* g);
+int uk_ingest_evolve(uk_ingest_t* g, int32_t n, const kalman_matrix_t* H, const kalman_matrix_t* F,
+                     const kalman_matrix_t* c, const kalman_matrix_t* K, char K_type);
+int uk_ingest_observe(uk_ingest_t* g, const kalman_matrix_t* G, const kalman_matrix_t* o, const kalman_matrix_t* C,
+                      char C_type);
+int uk_ingest_smooth(uk_ingest_t* g, uint32_t options);
+int uk_ingest_valid(const uk_ingest_t* g, int64_t si);
+const double* uk_ingest_state(const uk_ingest_t* g, int64_t si);
+const double* uk_ingest_covariance(const uk_ingest_t* g, int64_t si, char* type);
+void uk_ingest_forget(uk_ingest_t* g, int64_t si);
+void uk_ingest_rollback(uk_ingest_t* g, int64_t si);
+
 struct kalman_st {
-  kalman_step_equations_t** steps; /* live steps are steps[start .. start+count-1] */
-  int64_t cap, start, count;
-  kalman_step_index_t first_index; /* logical index of steps[start] */
-  kalman_step_equations_t* current;
+  uk_ingest_t* steps;
   uint32_t options;
 };
-
-static kalman_step_equations_t* step_new(void) {
-  kalman_step_equations_t* s = (kalman_step_equations_t*)calloc(1, sizeof(kalman_step_equations_t));
-  s->step = -1;
-  s->dimension = -1;
-  s->covariance_type = 'C';
-  return s;
-}
-
-static void step_clear_observation(kalman_step_equations_t* s) {
-  matrix_free(s->G);
-  matrix_free(s->o);
-  matrix_free(s->C);
-  matrix_free(s->state);
-  matrix_free(s->covariance);
-  s->G = s->o = s->C = s->state = s->covariance = NULL;
-}
-
-static void step_free(kalman_step_equations_t* s) {
-  if (!s) return;
-  matrix_free(s->H);
-  matrix_free(s->F);
-  matrix_free(s->K);
-  matrix_free(s->c);
-  step_clear_observation(s);
-  free(s);
-}
 
 kalman_t* kalman_create_options(uint32_t options) {
   const uint32_t algo = options & (KALMAN_ALGORITHM_ULTIMATE | KALMAN_ALGORITHM_CONVENTIONAL |
@@ -147,8 +137,7 @@ kalman_t* kalman_create_options(uint32_t options) {
     return NULL;
   }
   kalman_t* k = (kalman_t*)calloc(1, sizeof(kalman_t));
-  k->cap = 1024;
-  k->steps = (kalman_step_equations_t**)calloc((size_t)k->cap, sizeof(*k->steps));
+  k->steps = uk_ingest_create();
   k->options = options;
   return k;
 }
@@ -157,130 +146,71 @@ kalman_t* kalman_create(void) { return kalman_create_options(KALMAN_ALGORITHM_OD
 
 void kalman_free(kalman_t* k) {
   if (!k) return;
-  for (int64_t i = 0; i < k->count; i++) step_free(k->steps[k->start + i]);
-  step_free(k->current);
-  free(k->steps);
+  uk_ingest_free(k->steps);
   free(k);
 }
 
-kalman_step_index_t kalman_earliest(kalman_t* k) { return (k && k->count > 0) ? k->first_index : -1; }
+kalman_step_index_t kalman_earliest(kalman_t* k) {
+  return (k && uk_ingest_count(k->steps) > 0) ? (kalman_step_index_t)uk_ingest_first(k->steps) : -1;
+}
 kalman_step_index_t kalman_latest(kalman_t* k) {
-  return (k && k->count > 0) ? k->first_index + (kalman_step_index_t)(k->count - 1) : -1;
-}
-
-static kalman_step_equations_t* step_at(kalman_t* k, kalman_step_index_t si) {
-  if (!k || k->count == 0) return NULL;
-  if (si < 0) si = kalman_latest(k);
-  if (si < k->first_index || si > kalman_latest(k)) {
-    uk_set_error("step %lld is not in memory", (long long)si);
-    return NULL;
-  }
-  return k->steps[k->start + (si - k->first_index)];
-}
-
-static void append(kalman_t* k, kalman_step_equations_t* s) {
-  if (k->start + k->count == k->cap) {
-    if (k->count <= k->cap / 2) { /* shift back */
-      memmove(k->steps, k->steps + k->start, (size_t)k->count * sizeof(*k->steps));
-      k->start = 0;
-    } else {
-      k->cap *= 2;
-      k->steps = (kalman_step_equations_t**)realloc(k->steps, (size_t)k->cap * sizeof(*k->steps));
-    }
-  }
-  k->steps[k->start + k->count++] = s;
+  return (k && uk_ingest_count(k->steps) > 0)
+             ? (kalman_step_index_t)(uk_ingest_first(k->steps) + uk_ingest_count(k->steps) - 1)
+             : -1;
 }
 
 void kalman_evolve(kalman_t* k, int32_t n_i, kalman_matrix_t* H_i, kalman_matrix_t* F_i, kalman_matrix_t* c_i,
                    kalman_matrix_t* K_i, char K_type) {
   if (!k) return;
-  step_free(k->current);
-  kalman_step_equations_t* s = k->current = step_new();
-  s->dimension = n_i;
-  if (k->count == 0) {
-    s->step = 0; /* the first step has no evolution equation */
-    return;
-  }
-  s->step = kalman_latest(k) + 1;
-  if (!H_i || !F_i || !c_i) uk_set_error("kalman_evolve: H, F and c are required after the first step");
-  s->H = matrix_create_copy(H_i);
-  s->F = matrix_create_copy(F_i);
-  s->c = matrix_create_copy(c_i);
-  s->K = matrix_create_copy(K_i);
-  s->K_type = K_type;
+  (void)uk_ingest_evolve(k->steps, n_i, H_i, F_i, c_i, K_i, K_type);
 }
 
 void kalman_observe(kalman_t* k, kalman_matrix_t* G_i, kalman_matrix_t* o_i, kalman_matrix_t* C_i, char C_type) {
-  if (!k || !k->current) {
-    uk_set_error("kalman_observe: call kalman_evolve first");
+  if (!k) {
+    uk_set_error("kalman_observe: NULL kalman_t");
     return;
   }
-  kalman_step_equations_t* s = k->current;
-  if (o_i != NULL) {
-    s->G = matrix_create_copy(G_i);
-    s->o = matrix_create_copy(o_i);
-    s->C = matrix_create_copy(C_i);
-    s->C_type = C_type;
-  }
-  if (k->count == 0) k->first_index = s->step;
-  append(k, s);
-  k->current = NULL;
+  (void)uk_ingest_observe(k->steps, G_i, o_i, C_i, C_type);
 }
 
 void kalman_smooth(kalman_t* k) {
-  if (!k || k->count == 0) return;
-  kalman_step_equations_t** live = k->steps + k->start;
-  if (k->options & KALMAN_ALGORITHM_ODDEVEN)
-    kalman_smooth_oddeven((kalman_options_t)k->options, live, (kalman_step_index_t)k->count);
-  if (k->options & KALMAN_ALGORITHM_ASSOCIATIVE)
-    kalman_smooth_associative((kalman_options_t)k->options, live, (kalman_step_index_t)k->count);
+  if (!k) return;
+  (void)uk_ingest_smooth(k->steps, k->options);
 }
 
 kalman_matrix_t* kalman_estimate(kalman_t* k, kalman_step_index_t si) {
-  kalman_step_equations_t* s = step_at(k, si);
-  if (!s) return NULL;
-  if (!s->state) return matrix_create_constant(s->dimension, 1, kalman_nan);
-  return matrix_create_copy(s->state);
+  if (!k || !uk_ingest_valid(k->steps, si)) return NULL;
+  const int n = uk_ingest_dimension(k->steps);
+  const double* x = uk_ingest_state(k->steps, si);
+  if (!x) return matrix_create_constant(n, 1, kalman_nan);
+  kalman_matrix_t* m = matrix_create(n, 1);
+  memcpy(m->elements, x, (size_t)n * sizeof(double));
+  return m;
 }
 
 kalman_matrix_t* kalman_covariance(kalman_t* k, kalman_step_index_t si) {
-  kalman_step_equations_t* s = step_at(k, si);
-  if (!s) return NULL;
-  if (!s->covariance) return matrix_create_constant(s->dimension, s->dimension, kalman_nan);
-  return matrix_create_copy(s->covariance);
+  if (!k || !uk_ingest_valid(k->steps, si)) return NULL;
+  const int n = uk_ingest_dimension(k->steps);
+  const double* S = uk_ingest_covariance(k->steps, si, NULL);
+  if (!S) return matrix_create_constant(n, n, kalman_nan);
+  kalman_matrix_t* m = matrix_create(n, n);
+  memcpy(m->elements, S, (size_t)n * n * sizeof(double));
+  return m;
 }
 
 char kalman_covariance_type(kalman_t* k, kalman_step_index_t si) {
-  kalman_step_equations_t* s = step_at(k, si);
-  return s ? s->covariance_type : '-';
+  char t = 'C';
+  if (!k || !uk_ingest_valid(k->steps, si)) return '-';
+  (void)uk_ingest_covariance(k->steps, si, &t);
+  return t;
 }
 
 void kalman_forget(kalman_t* k, kalman_step_index_t si) {
-  if (!k || k->count == 0) return;
-  if (si < 0) si = kalman_latest(k) - 1;
-  if (si > kalman_latest(k) - 1) return; /* the last step is never dropped */
-  while (k->count > 0 && k->first_index <= si) {
-    step_free(k->steps[k->start]);
-    k->start++;
-    k->count--;
-    k->first_index++;
-  }
+  if (k) uk_ingest_forget(k->steps, si);
 }
 
 void kalman_rollback(kalman_t* k, kalman_step_index_t si) {
-  if (!k || k->count == 0) return;
-  if (si > kalman_latest(k) || si < k->first_index) return;
-  while (k->count > 0) {
-    kalman_step_equations_t* s = k->steps[k->start + k->count - 1];
-    k->count--;
-    if (s->step == si) { /* back to just after evolve */
-      step_clear_observation(s);
-      step_free(k->current);
-      k->current = s;
-      break;
-    }
-    step_free(s);
-  }
+  if (k) uk_ingest_rollback(k->steps, si);
 }
 
 kalman_matrix_t* kalman_perftest(kalman_t* kalman, kalman_matrix_t* H, kalman_matrix_t* F, kalman_matrix_t* c,

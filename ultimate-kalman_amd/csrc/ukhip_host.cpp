@@ -397,7 +397,9 @@ static void batch_release(ukhip_batch* b) {
 
 /* Allocates device memory (from `arena`, or hipMalloc), uploads staging and
  * builds the level tables, on stream `us`; with `sync` false the upload may
- * still be in flight on return (st.host and b->tab_host must outlive it). */
+ * still be in flight on return (st.host and b->tab_host must outlive it).
+ * With st.host NULL only the layout is used: the caller uploads the raw
+ * fields itself (at st's offsets in b->d_raw). */
 static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& lengths, hipStream_t us = nullptr,
                                        DevArena* arena = nullptr, bool sync = true) {
   if (!us) us = g_stream;
@@ -422,7 +424,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->d_raw = dev_alloc(b, st.bytes, "device memory for the raw inputs");
   if (!b->d_raw) { batch_release(b); return nullptr; }
   b->raw_bytes = st.bytes;
-  if (!hip_ok(hipMemcpyAsync(b->d_raw, st.host, st.bytes, hipMemcpyHostToDevice, us), "upload")) {
+  if (st.host && !hip_ok(hipMemcpyAsync(b->d_raw, st.host, st.bytes, hipMemcpyHostToDevice, us), "upload")) {
     batch_release(b);
     return nullptr;
   }
@@ -1804,4 +1806,265 @@ extern "C" void ukhip_filter_free(ukhip_filter_t* f) {
   if (f->d_in) (void)hipFree(f->d_in);
   if (f->h_in) (void)hipHostFree(f->h_in);
   delete f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* kalman_t ingest: steps appended straight into structure-of-arrays fields   */
+/* ------------------------------------------------------------------------ */
+
+/* The steps of one kalman_t (c/kalman_explicit_representation.c:164-252) as
+ * the device layout's fields, slot after slot (slot j of a field at j*stride):
+ * kalman_evolve / kalman_observe write a step's matrices into its slot once,
+ * kalman_smooth uploads every field's live range as it is (no per-matrix
+ * allocations, no repacking) and keeps the results compactly. Live steps are
+ * slots [start, start + count); slot start + count is the step between
+ * kalman_evolve and kalman_observe. A field's stride changes only when a wider
+ * representation first appears (a non-'w' K or C, more observation rows, a
+ * non-identity H); then that field is re-laid once. */
+struct uk_ingest {
+  int n = -1, mr = 1;
+  int64_t start = 0, count = 0, step0 = 0;
+  bool current = false;
+  bool H_any = false, K_full = false, C_full = false;
+  std::vector<double> F, H, c, K, G, o, C, x, S;
+  std::vector<int8_t> Kt, Ct, hasx, hasS;
+  std::vector<int32_t> m;
+  char Stype = 'C';
+  int64_t cap = 0;
+
+  size_t nn() const { return (size_t)n * n; }
+  size_t Kw() const { return K_full ? nn() : (size_t)n; }
+  size_t Cw() const { return C_full ? (size_t)mr * mr : (size_t)mr; }
+  void grow(int64_t slots) {
+    if (slots <= cap) return;
+    int64_t nc = std::max<int64_t>(1024, cap);
+    while (nc < slots) nc *= 2;
+    if (start > 0 && count + 1 <= nc / 2) compact();
+    if (start + count + 1 <= cap) return;
+    cap = nc;
+    F.resize(cap * nn());
+    if (H_any) H.resize(cap * nn());
+    c.resize(cap * n);
+    K.resize(cap * Kw());
+    G.resize(cap * (size_t)mr * n);
+    o.resize(cap * (size_t)mr);
+    C.resize(cap * Cw());
+    x.resize(cap * n);
+    S.resize(cap * nn());
+    Kt.resize(cap);
+    Ct.resize(cap);
+    hasx.resize(cap);
+    hasS.resize(cap);
+    m.resize(cap);
+  }
+  /* moves the live slots (and the pending one) to slot 0 */
+  void compact() {
+    const int64_t k = count + (current ? 1 : 0);
+    auto mv = [&](auto& v, size_t stride) {
+      if (!v.empty()) std::memmove(v.data(), v.data() + start * stride, k * stride * sizeof(v[0]));
+    };
+    mv(F, nn());
+    if (H_any) mv(H, nn());
+    mv(c, n); mv(K, Kw()); mv(G, (size_t)mr * n); mv(o, mr); mv(C, Cw()); mv(x, n); mv(S, nn());
+    mv(Kt, 1); mv(Ct, 1); mv(hasx, 1); mv(hasS, 1); mv(m, 1);
+    start = 0;
+  }
+  /* re-lays a field from stride a to stride b (b >= a), the first `keep` entries of each slot */
+  static void restride(std::vector<double>& v, int64_t slots, size_t a, size_t b, size_t keep) {
+    std::vector<double> w(slots * b, 0.0);
+    for (int64_t j = 0; j < slots; ++j) std::memcpy(&w[j * b], &v[j * a], keep * 8);
+    v.swap(w);
+  }
+};
+
+extern "C" uk_ingest* uk_ingest_create(void) { return new uk_ingest(); }
+extern "C" void uk_ingest_free(uk_ingest* g) { delete g; }
+extern "C" int64_t uk_ingest_count(const uk_ingest* g) { return g ? g->count : 0; }
+extern "C" int64_t uk_ingest_first(const uk_ingest* g) { return g ? g->step0 : 0; }
+extern "C" int32_t uk_ingest_dimension(const uk_ingest* g) { return g ? g->n : -1; }
+
+extern "C" int uk_ingest_evolve(uk_ingest* g, int32_t n, const kalman_matrix_t* Hm, const kalman_matrix_t* Fm,
+                                const kalman_matrix_t* cm, const kalman_matrix_t* Km, char K_type) {
+  if (!g) return -1;
+  if (g->n < 0) {
+    if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("kalman_evolve: state dimension %d outside 1..%d", n, UKHIP_MAX_N); return -1; }
+    g->n = n;
+  } else if (n != g->n) {
+    uk_set_error("kalman_evolve: dimension %d differs from %d (libukhip smooths constant dimensions)", n, g->n);
+    return -1;
+  }
+  g->grow(g->start + g->count + 1);
+  const int64_t j = g->start + g->count;
+  g->current = true;
+  g->hasx[j] = g->hasS[j] = 0;
+  g->m[j] = 0;
+  if (g->count == 0) return 0; /* the first step has no evolution equation */
+  const size_t nn = g->nn();
+  if (!Hm || !Fm || !cm) { uk_set_error("kalman_evolve: H, F and c are required after the first step"); g->current = false; return -1; }
+  if (Fm->row_dim != n || Fm->col_dim != n || Hm->row_dim != n || Hm->col_dim != n || cm->row_dim != n ||
+      cm->col_dim != 1 || cov_code(K_type) < 0 || !cov_shape_ok(Km, K_type, n)) {
+    uk_set_error("kalman_evolve: H, F, c or K (type '%c') has the wrong shape or type", K_type);
+    g->current = false;
+    return -1;
+  }
+  if (K_type != 'w' && !g->K_full) {
+    uk_ingest::restride(g->K, g->cap, n, nn, n);
+    g->K_full = true;
+  }
+  if (!g->H_any && !is_identity(Hm)) {
+    g->H.assign(g->cap * nn, 0.0);
+    for (int64_t q = 0; q < g->cap; ++q)
+      for (int d = 0; d < n; ++d) g->H[q * nn + (size_t)d * n + d] = 1.0;
+    g->H_any = true;
+  }
+  copy_cols(&g->F[j * nn], n, Fm, n, n);
+  if (g->H_any) copy_cols(&g->H[j * nn], n, Hm, n, n);
+  copy_cols(&g->c[j * n], n, cm, n, 1);
+  copy_cols(&g->K[j * g->Kw()], n, Km, Km->row_dim, Km->col_dim);
+  g->Kt[j] = (int8_t)cov_code(K_type);
+  return 0;
+}
+
+extern "C" int uk_ingest_observe(uk_ingest* g, const kalman_matrix_t* Gm, const kalman_matrix_t* om,
+                                 const kalman_matrix_t* Cm, char C_type) {
+  if (!g || !g->current) { uk_set_error("kalman_observe: call kalman_evolve first"); return -1; }
+  const int n = g->n;
+  const int64_t j = g->start + g->count;
+  if (om) {
+    const int mi = om->row_dim;
+    if (mi > n || om->col_dim != 1 || !Gm || Gm->row_dim != mi || Gm->col_dim != n || cov_code(C_type) < 0 ||
+        !cov_shape_ok(Cm, C_type, mi)) {
+      uk_set_error("kalman_observe: G, o or C (type '%c') has the wrong shape, or more rows than states", C_type);
+      return -1;
+    }
+    if (mi > g->mr || (C_type != 'w' && !g->C_full)) {
+      const int nm = std::max(mi, g->mr);
+      const bool cf = g->C_full || C_type != 'w';
+      std::vector<double> G2(g->cap * (size_t)nm * n, 0.0), o2(g->cap * (size_t)nm, 0.0),
+          C2(g->cap * (cf ? (size_t)nm * nm : (size_t)nm), 0.0);
+      for (int64_t q = 0; q < g->cap; ++q) {
+        for (int col = 0; col < n; ++col)
+          std::memcpy(&G2[q * nm * n + (size_t)col * nm], &g->G[q * g->mr * n + (size_t)col * g->mr], g->mr * 8);
+        std::memcpy(&o2[q * nm], &g->o[q * g->mr], g->mr * 8);
+        const size_t cw2 = cf ? (size_t)nm * nm : (size_t)nm;
+        if (g->C_full)
+          for (int col = 0; col < g->mr; ++col)
+            std::memcpy(&C2[q * cw2 + (size_t)col * nm], &g->C[q * g->Cw() + (size_t)col * g->mr], g->mr * 8);
+        else
+          std::memcpy(&C2[q * cw2], &g->C[q * g->Cw()], g->mr * 8);
+      }
+      g->G.swap(G2);
+      g->o.swap(o2);
+      g->C.swap(C2);
+      g->mr = nm;
+      g->C_full = cf;
+    }
+    const int mr = g->mr;
+    copy_cols(&g->G[j * (size_t)mr * n], mr, Gm, mi, n);
+    copy_cols(&g->o[j * (size_t)mr], mr, om, mi, 1);
+    if (C_type == 'w') copy_cols(&g->C[j * g->Cw()], mr, Cm, mi, 1);
+    else copy_cols(&g->C[j * g->Cw()], mr, Cm, mi, mi);
+    g->Ct[j] = (int8_t)cov_code(C_type);
+    g->m[j] = mi;
+  } else {
+    g->m[j] = 0;
+  }
+  g->current = false;
+  g->count++;
+  return 0;
+}
+
+extern "C" int uk_ingest_smooth(uk_ingest* g, uint32_t options) {
+  if (!g || g->count == 0) return 0;
+  if (!ensure_stream()) return -1;
+  const bool assoc = (options & KALMAN_ALGORITHM_ASSOCIATIVE) != 0;
+  const int n = g->n;
+  const size_t nn = g->nn(), S = (size_t)g->count, j0 = (size_t)g->start;
+  Staging st;
+  st.n = n;
+  st.mr = g->mr;
+  st.steps = g->count;
+  st.nF = g->count;
+  st.H_ident = g->H_any ? 0 : 1;
+  st.K_full = g->K_full ? 1 : 0;
+  st.C_full = g->C_full ? 1 : 0;
+  st.layout();
+  std::vector<int64_t> lengths{g->count};
+  ukhip_batch* b = batch_from_staging(st, lengths, g_stream, nullptr, true);
+  if (!b) return -1;
+  std::vector<int64_t> fidx(S);
+  std::vector<int8_t> first(S, 0);
+  for (size_t i = 0; i < S; ++i) fidx[i] = (int64_t)i;
+  first[0] = 1; /* the earliest live step: its evolution equation refers to a forgotten state */
+  char* d = (char*)b->d_raw;
+  auto up = [&](size_t off, const void* src, size_t bytes) {
+    return bytes == 0 || hip_ok(hipMemcpyAsync(d + off, src, bytes, hipMemcpyHostToDevice, g_stream), "upload");
+  };
+  bool ok = up(st.offFidx, fidx.data(), S * 8) && up(st.offF, &g->F[j0 * nn], S * nn * 8) &&
+            (!g->H_any || up(st.offH, &g->H[j0 * nn], S * nn * 8)) && up(st.offc, &g->c[j0 * n], S * n * 8) &&
+            up(st.offK, &g->K[j0 * g->Kw()], S * g->Kw() * 8) && up(st.offG, &g->G[j0 * g->mr * n], S * g->mr * n * 8) &&
+            up(st.offo, &g->o[j0 * g->mr], S * g->mr * 8) && up(st.offC, &g->C[j0 * g->Cw()], S * g->Cw() * 8) &&
+            up(st.offKt, &g->Kt[j0], S) && up(st.offCt, &g->Ct[j0], S) && up(st.offm, &g->m[j0], S * 4) &&
+            up(st.offfirst, first.data(), S);
+  const int cov = assoc ? 1 : ((options & KALMAN_NO_COVARIANCE) == 0);
+  if (ok) ok = assoc ? uk_assoc_smooth(b, options) == 0 : run_oddeven(b, cov, nullptr, g_stream);
+  const bool wform = cov && (options & UKHIP_COVARIANCE_W) != 0;
+  if (ok) ok = ukhip_batch_download(b, 0, &g->x[j0 * n], (cov && !wform) ? &g->S[j0 * nn] : nullptr) == 0;
+  if (ok && wform) ok = ukhip_batch_download_cov(b, 0, 'W', &g->S[j0 * nn]) == 0;
+  batch_release(b);
+  if (!ok) return -1;
+  for (size_t i = 0; i < S; ++i) {
+    g->hasx[j0 + i] = 1;
+    g->hasS[j0 + i] = cov ? 1 : 0;
+  }
+  g->Stype = wform ? 'W' : 'C';
+  return 0;
+}
+
+static int64_t ingest_slot(const uk_ingest* g, int64_t si) {
+  if (!g || g->count == 0) return -1;
+  if (si < 0) si = g->step0 + g->count - 1;
+  if (si < g->step0 || si >= g->step0 + g->count) {
+    uk_set_error("step %lld is not in memory", (long long)si);
+    return -1;
+  }
+  return g->start + (si - g->step0);
+}
+
+extern "C" const double* uk_ingest_state(const uk_ingest* g, int64_t si) {
+  const int64_t j = ingest_slot(g, si);
+  return (j >= 0 && g->hasx[j]) ? &g->x[j * g->n] : nullptr;
+}
+
+extern "C" const double* uk_ingest_covariance(const uk_ingest* g, int64_t si, char* type) {
+  const int64_t j = ingest_slot(g, si);
+  if (type) *type = g ? g->Stype : 'C';
+  return (j >= 0 && g->hasS[j]) ? &g->S[j * g->nn()] : nullptr;
+}
+
+extern "C" int uk_ingest_valid(const uk_ingest* g, int64_t si) { return ingest_slot(g, si) >= 0; }
+
+/* c/kalman_base.c kalman_forget: drops the steps up to si (default: all but the latest), never the latest. */
+extern "C" void uk_ingest_forget(uk_ingest* g, int64_t si) {
+  if (!g || g->count == 0) return;
+  const int64_t latest = g->step0 + g->count - 1;
+  if (si < 0) si = latest - 1;
+  if (si > latest - 1) return;
+  const int64_t d = si - g->step0 + 1;
+  if (d <= 0) return;
+  g->start += d;
+  g->count -= d;
+  g->step0 += d;
+}
+
+/* c/kalman_base.c kalman_rollback: drops the steps after si; step si returns
+ * to the state just after its kalman_evolve (observation and results cleared). */
+extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
+  if (!g || g->count == 0) return;
+  if (si > g->step0 + g->count - 1 || si < g->step0) return;
+  g->count = si - g->step0;
+  const int64_t j = g->start + g->count;
+  g->m[j] = 0;
+  g->hasx[j] = g->hasS[j] = 0;
+  g->current = true;
 }
