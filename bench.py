@@ -347,6 +347,11 @@ def run_batch(args):
     each chunk is uploaded, smoothed once untimed (graph capture), then smoothed
     --steps times with its inputs resident; a step of this workload is one
     smoothing of every problem, its time the sum over chunks."""
+    if not args.no_autotune:
+        t = time.perf_counter()
+        table = device.autotune(32, 16384)
+        print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
+              flush=True)
     specs = P.batch_specs(count=args.batch_count)
     budget = args.hbm_budget_gb * 1e9
     chunks, cur, used = [], [], 0.0
@@ -450,6 +455,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the n=64 precision workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-filter", action="store_true", help="skip the batched sequential filter")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in QR launch table")
     ap.add_argument("--filter-count", type=int, default=4096)
     ap.add_argument("--filter-n", type=int, default=16)
     ap.add_argument("--filter-steps", type=int, default=200)
@@ -474,6 +480,12 @@ def main():
         print(json.dumps(run_batch(args)), flush=True)
         return
 
+    t = time.perf_counter()
+    table = None
+    if not args.no_autotune:
+        table = device.autotune(WORKLOADS[args.workload]["n"], 16384)
+        print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
+              flush=True)
     t = time.perf_counter()
     pr = make_problem(args.workload, args.k)
     print(f"[bench] generated {pr.name} k={pr.k} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
@@ -565,6 +577,8 @@ def main():
                      "kernel_ms_per_step": kern[dom], "avg_launch_ms": kern[dom] / launches[dom]},
         "kernels_ms_per_step": {k: v for k, v in kern.items()},
         "kernels": {k: kernel_label(k, pr) for k in fl_of},
+        "qr_launch_table": table if table is not None else device.launch_table(),
+        "qr_launch_table_source": "autotuned (ukhip_autotune)" if table is not None else "built-in",
         "kernels_tflops": {k: fl_of[k] / (kern[k] * 1e-3) / 1e12 for k in fl_of if kern[k] > 0},
         "algorithmic_flops_per_timestep": fl["total"] / pr.k,
         "whole_path_tflops": fl["total"] / (kern["total"] * 1e-3) / 1e12,

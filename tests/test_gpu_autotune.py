@@ -1,0 +1,61 @@
+"""The QR launch table and its run-time autotuner (ukhip_autotune).
+
+Every launch variant (1, 2, 4 workgroups per CU asked of the register
+allocator) runs the same instructions: results must be bit-identical across
+variants. The autotuner must leave a valid table (every entry 1, 2 or 4).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from ukhip import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from ukhip import device
+    device.lib()
+    return device
+
+
+@pytest.mark.parametrize("n,m", [(24, 12), (40, 20), (64, 32)])
+def test_variants_bit_identical(dev, n, m):
+    L = dev.lib()
+    F = 0.97 * P.random_orthogonal(P.rng_for(800 + n), n)
+    probs = [P.iid_problem(k, n, m, seed=810 + k, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
+                           unobserved=lambda i: i % 10 == 9, name=f"n{n}_k{k}") for k in (3, 200)]
+    saved = {p: L.ukhip_qr_launch_get(p, n) for p in (1, 2, 3)}
+    L.ukhip_use_graphs(0)
+    try:
+        ref = None
+        for p in (1, 2, 3):
+            for wg in (1, 2, 4):
+                assert L.ukhip_qr_launch_set(p, n, wg) == 0
+                b = dev.Batch(probs)
+                b.smooth("oddeven")
+                out = [b.download(i) for i in range(len(probs))]
+                b.free()
+                L.ukhip_qr_launch_set(p, n, saved[p])
+                if ref is None:
+                    ref = out
+                for (x0, c0), (x1, c1) in zip(ref, out):
+                    assert np.array_equal(x0, x1) and np.array_equal(c0, c1), (p, wg)
+        say(f"n={n}: 9 launch variants bit-identical")
+    finally:
+        for p, w in saved.items():
+            L.ukhip_qr_launch_set(p, n, w)
+        L.ukhip_use_graphs(1)
+
+
+def test_autotune_table(dev):
+    table = dev.autotune(64, 1024)
+    say(f"autotuned launch table: {table}")
+    assert set(table.values()) <= {1, 2, 4}

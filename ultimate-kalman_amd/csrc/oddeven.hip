@@ -321,11 +321,14 @@ __host__ __device__ constexpr int qr_wgs(int phase, int NB) {
   return NB == 48 ? (phase == 1 ? UK_P1_WGS48 : phase == 2 ? UK_P2_WGS48 : UK_P3_WGS48)
                   : (NB == 64 && phase == 2 ? UK_P2_WGS64 : 1);
 }
-template <int NB, int COLS, int PHASE>
+/* WG > 0 overrides the table's workgroups per CU (the variants the run-time
+ * autotuner chooses among). */
+template <int NB, int COLS, int PHASE, int WG = 0>
 struct QRGeom {
   static constexpr int SL = qr_sl(PHASE, NB) ? qr_sl(PHASE, NB) : ((NB <= 16 || 4 * COLS > 1024) ? 2 : 4);
   static constexpr int NT = ((SL * COLS + 63) / 64) * 64;
-  static constexpr int MINW_ = (qr_wgs(PHASE, NB) * (NT / 64) + 3) / 4;
+  static constexpr int W = WG > 0 ? WG : qr_wgs(PHASE, NB);
+  static constexpr int MINW_ = (W * (NT / 64) + 3) / 4;
   static constexpr int MINW = MINW_ < 1 ? 1 : (MINW_ > 8 ? 8 : MINW_);
 };
 
@@ -361,8 +364,8 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
-template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT), (QRGeom<NB, 2 * NB + 1, 1>::MINW))
+template <int NB, class T, int WG = 0>
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
   constexpr int SL = QRGeom<NB, 2 * NB + 1, 1>::SL;
   constexpr int NT = QRGeom<NB, 2 * NB + 1, 1>::NT;
@@ -416,8 +419,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT), (QRGeom<NB, 2
   }
 }
 
-template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2>::NT), (QRGeom<NB, 3 * NB + 1, 2>::MINW))
+template <int NB, class T, int WG = 0>
+__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG>::MINW))
     k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
   constexpr int SL = QRGeom<NB, 3 * NB + 1, 2>::SL;
   constexpr int NT = QRGeom<NB, 3 * NB + 1, 2>::NT;
@@ -471,8 +474,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2>::NT), (QRGeom<NB, 3
   }
 }
 
-template <int NB, class T>
-__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3>::NT), (QRGeom<NB, NB + 1, 3>::MINW))
+template <int NB, class T, int WG = 0>
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG>::NT), (QRGeom<NB, NB + 1, 3, WG>::MINW))
     k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
   constexpr int SL = QRGeom<NB, NB + 1, 3>::SL;
   constexpr int NT = QRGeom<NB, NB + 1, 3>::NT;
@@ -1509,7 +1512,54 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
   return hipGetLastError();
 }
 
+/* Run-time launch table of the FP64 phase kernels at NB >= 32: workgroups per
+ * CU asked of the register allocator, per phase and bucket (32, 48, 64);
+ * starts from the measured table, rewritten by ukhip_autotune. */
+static int g_qr_wg[3][3] = {{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
+                            {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
+                            {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}};
+static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
+
+extern "C" int ukhip_qr_launch_get(int phase, int n) {
+  const int bi = qr_bucket_index(uk_bucket(n));
+  return (phase < 1 || phase > 3 || bi < 0) ? 0 : g_qr_wg[phase - 1][bi];
+}
+extern "C" int ukhip_qr_launch_set(int phase, int n, int wg) {
+  const int bi = qr_bucket_index(uk_bucket(n));
+  if (phase < 1 || phase > 3 || bi < 0 || (wg != 1 && wg != 2 && wg != 4)) return -1;
+  g_qr_wg[phase - 1][bi] = wg;
+  return 0;
+}
+
 namespace {
+template <int NB, class T, int WG>
+void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
+               hipStream_t st, int phase) {
+  if (phase == 1)
+    k_phase1<NB, T, WG><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 2 * NB + 1, 1, WG>::NT), 0, st>>>(*lv, in, out,
+                                                                                                      fac, n);
+  else if (phase == 2)
+    k_phase2<NB, T, WG><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 3 * NB + 1, 2, WG>::NT), 0, st>>>(*lv, in, out,
+                                                                                                      fac, fix, n);
+  else
+    k_phase3<NB, T, WG><<<dim3((unsigned)lv->units), dim3(QRGeom<NB, NB + 1, 3, WG>::NT), 0, st>>>(*lv, in, out, fix,
+                                                                                                  n);
+}
+
+template <int NB, class T>
+void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
+                  int n, hipStream_t st, int phase) {
+  if constexpr (sizeof(T) == 8 && NB >= 32) {
+    switch (g_qr_wg[phase - 1][qr_bucket_index(NB)]) {
+      case 1: phases_nb<NB, T, 1>(lv, in, out, fac, fix, n, st, phase); break;
+      case 2: phases_nb<NB, T, 2>(lv, in, out, fac, fix, n, st, phase); break;
+      default: phases_nb<NB, T, 4>(lv, in, out, fac, fix, n, st, phase); break;
+    }
+  } else {
+    phases_nb<NB, T, 0>(lv, in, out, fac, fix, n, st, phase);
+  }
+}
+
 template <class T>
 hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac,
                             T* fix, int n, hipStream_t st, hipEvent_t* ev) {
@@ -1517,15 +1567,12 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
-#define L_(NBV)                                                                                                      \
-  mark(0);                                                                                                           \
-  k_phase1<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 2 * NBV + 1, 1>::NT), 0, st>>>(*lv, in, out, fac, n);  \
-  mark(1);                                                                                                           \
-  k_phase2<NBV, T><<<dim3((unsigned)lv->evens), dim3(QRGeom<NBV, 3 * NBV + 1, 2>::NT), 0, st>>>(*lv, in, out, fac, fix, \
-                                                                                           n);                     \
-  mark(2);                                                                                                           \
-  k_phase3<NBV, T><<<dim3((unsigned)lv->units), dim3(QRGeom<NBV, NBV + 1, 3>::NT), 0, st>>>(*lv, in, out, fix, n);      \
-  mark(3)
+#define L_(NBV)                                                   \
+  mark(0);                                                        \
+  for (int ph = 1; ph <= 3; ++ph) {                               \
+    phase_launch<NBV, T>(lv, in, out, fac, fix, n, st, ph);       \
+    mark(ph);                                                     \
+  }
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;

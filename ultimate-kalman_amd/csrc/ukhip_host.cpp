@@ -15,6 +15,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdarg>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -2067,4 +2068,68 @@ extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
   g->m[j] = 0;
   g->hasx[j] = g->hasS[j] = 0;
   g->current = true;
+}
+
+/* ------------------------------------------------------------------------ */
+/* run-time autotuning of the QR launch table                                 */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int ukhip_qr_launch_get(int phase, int n);
+extern "C" int ukhip_qr_launch_set(int phase, int n, int wg);
+
+/* For every bucket (32, 48, 64) up to n_max: a synthetic problem (n = bucket,
+ * m = n/2, k steps, F = 0.98 times a Householder reflection, G uniform with
+ * variance 1/n, 'w' covariances) is smoothed with each phase kernel in each
+ * of its launch variants (1, 2, 4 workgroups per CU asked of the register
+ * allocator), timed with HIP events; the fastest variant of each phase goes
+ * into the launch table. The variants execute the same instructions, so the
+ * choice changes timing only, never results. Batches whose smoothing was
+ * captured into a graph earlier keep their captured launches. */
+extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
+  if (!ensure_stream()) return -1;
+  if (k < 64) k = 4096;
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  auto uni = [&]() { /* xorshift64*, uniform in (-1, 1) */
+    st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
+    return (double)((st * 0x2545F4914F6CDD1Dull) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  };
+  for (int nb : {32, 48, 64}) {
+    if (nb > std::max(32, (int)n_max)) break;
+    const int n = nb, m = nb / 2;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> v(n), F(nn), K(n, 1.0), G((size_t)k * m * n), o((size_t)k * m), C((size_t)m, 1.0 / std::sqrt(0.1));
+    double vv = 0.0;
+    for (int i = 0; i < n; ++i) { v[i] = uni(); vv += v[i] * v[i]; }
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) F[i + (size_t)j * n] = 0.98 * ((i == j ? 1.0 : 0.0) - 2.0 * v[i] * v[j] / vv);
+    const double gs = std::sqrt(3.0 / n);
+    for (auto& g : G) g = gs * uni();
+    for (auto& x : o) x = uni();
+    std::vector<double> Kall((size_t)k * n, 1.0), Call((size_t)k * m);
+    for (size_t i = 0; i < Call.size(); ++i) Call[i] = C[i % m];
+    ukhip_problem_t p{};
+    p.n = n; p.m = m; p.k = k;
+    p.F = F.data(); p.F_shared = 1;
+    p.K = Kall.data(); p.K_type = 'w';
+    p.G = G.data(); p.o = o.data(); p.C = Call.data(); p.C_type = 'w';
+    ukhip_batch_t* b = ukhip_batch_create(1, &p);
+    if (!b) return -1;
+    for (int phase = 1; phase <= 3; ++phase) {
+      int best = ukhip_qr_launch_get(phase, n);
+      double best_ms = 1e30;
+      for (int wg : {1, 2, 4}) {
+        ukhip_qr_launch_set(phase, n, wg);
+        double ms[UKHIP_OE_SLOTS];
+        double t = 0.0;
+        for (int rep = 0; rep < 3; ++rep) {
+          if (ukhip_batch_smooth_oddeven_timed(b, 0, ms, nullptr) != 0) { ukhip_batch_free(b); return -1; }
+          if (rep > 0) t += ms[UKHIP_OE_PHASE1 + phase - 1];
+        }
+        if (t < best_ms) { best_ms = t; best = wg; }
+      }
+      ukhip_qr_launch_set(phase, n, best);
+    }
+    ukhip_batch_free(b);
+  }
+  return 0;
 }

@@ -68,6 +68,10 @@ def lib():
         L.ukhip_server_wait.restype = C.c_int
         L.ukhip_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
         L.ukhip_server_destroy.argtypes = [C.c_void_p]
+        L.ukhip_autotune.argtypes = [C.c_int32, C.c_int32]
+        L.ukhip_autotune.restype = C.c_int
+        L.ukhip_qr_launch_get.argtypes = [C.c_int, C.c_int]
+        L.ukhip_qr_launch_set.argtypes = [C.c_int, C.c_int, C.c_int]
         L.ukhip_filter_create.restype = C.c_void_p
         L.ukhip_filter_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
         L.ukhip_filter_step.argtypes = [C.c_void_p, C.c_void_p]
@@ -78,6 +82,18 @@ def lib():
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
+
+
+def autotune(n_max=64, k=4096):
+    """ukhip_autotune; returns the launch table {(phase, bucket): workgroups per CU}."""
+    L = lib()
+    _check(L.ukhip_autotune(n_max, k), "ukhip_autotune")
+    return launch_table()
+
+
+def launch_table():
+    L = lib()
+    return {f"phase{p}<{nb}>": L.ukhip_qr_launch_get(p, nb) for nb in (32, 48, 64) for p in (1, 2, 3)}
 
 
 def last_error():
