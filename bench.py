@@ -347,7 +347,9 @@ def run_batch(args):
     each chunk is uploaded, smoothed once untimed (graph capture), then smoothed
     --steps times with its inputs resident; a step of this workload is one
     smoothing of every problem, its time the sum over chunks."""
-    if not args.no_autotune:
+    if args.launch_table:
+        device.set_launch_table(json.loads(args.launch_table))
+    elif not args.no_autotune:
         t = time.perf_counter()
         table = device.autotune(32, 16384)
         print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
@@ -456,6 +458,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-filter", action="store_true", help="skip the batched sequential filter")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in QR launch table")
+    ap.add_argument("--launch-table", default=None, metavar="JSON",
+                    help='set QR launch-table entries instead of autotuning, e.g. \'{"phase2<48, float>": 4}\'')
     ap.add_argument("--filter-count", type=int, default=4096)
     ap.add_argument("--filter-n", type=int, default=16)
     ap.add_argument("--filter-steps", type=int, default=200)
@@ -482,8 +486,11 @@ def main():
 
     t = time.perf_counter()
     table = None
-    if not args.no_autotune:
-        table = device.autotune(WORKLOADS[args.workload]["n"], 16384)
+    if args.launch_table:
+        device.set_launch_table(json.loads(args.launch_table))
+    elif not args.no_autotune:
+        n_tune = WORKLOADS[args.workload]["n"] if args.no_precision else max(WORKLOADS[args.workload]["n"], 64)
+        table = device.autotune(n_tune, 16384)
         print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
               flush=True)
     t = time.perf_counter()
@@ -578,7 +585,8 @@ def main():
         "kernels_ms_per_step": {k: v for k, v in kern.items()},
         "kernels": {k: kernel_label(k, pr) for k in fl_of},
         "qr_launch_table": table if table is not None else device.launch_table(),
-        "qr_launch_table_source": "autotuned (ukhip_autotune)" if table is not None else "built-in",
+        "qr_launch_table_source": ("autotuned (ukhip_autotune)" if table is not None
+                                   else "set by --launch-table" if args.launch_table else "built-in"),
         "kernels_tflops": {k: fl_of[k] / (kern[k] * 1e-3) / 1e12 for k in fl_of if kern[k] > 0},
         "algorithmic_flops_per_timestep": fl["total"] / pr.k,
         "whole_path_tflops": fl["total"] / (kern["total"] * 1e-3) / 1e12,

@@ -194,3 +194,27 @@ def test_smoother_without_device_fails_loudly(lib):
         fn(0, eqs.ptrs, pr.k)
         assert b"HIP device" in lib.ukhip_last_error()
         assert all(not eqs.structs[i].state for i in range(pr.k))
+
+
+def test_launch_table_set_and_get():
+    """The QR launch table is host state: entries round-trip without a device,
+    bad entries are refused, and set_launch_table parses launch_table()'s keys."""
+    from ukhip import device
+    L = device.lib()
+    saved = device.launch_table()
+    try:
+        assert set(saved.values()) <= {1, 2, 4} and len(saved) == 18
+        assert L.ukhip_qr_launch_set(1, 2, 48, 4) == 0 and L.ukhip_qr_launch_get(1, 2, 48) == 4
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3) == -1
+        assert L.ukhip_qr_launch_set(0, 4, 48, 1) == -1
+        assert L.ukhip_qr_launch_set(0, 1, 8, 1) == -1      # n <= 16 has no variants
+        device.set_launch_table({"phase3<64, double>": 2, "phase1<32, float>": 1})
+        t = device.launch_table()
+        assert t["phase3<64, double>"] == 2 and t["phase1<32, float>"] == 1
+        with pytest.raises(ValueError):
+            device.set_launch_table({"phase2<48>": 1})
+        with pytest.raises(RuntimeError):
+            device.set_launch_table({"phase2<48, double>": 8})
+    finally:
+        device.set_launch_table(saved)
+    assert device.launch_table() == saved

@@ -32,18 +32,18 @@ def test_variants_bit_identical(dev, n, m):
     F = 0.97 * P.random_orthogonal(P.rng_for(800 + n), n)
     probs = [P.iid_problem(k, n, m, seed=810 + k, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
                            unobserved=lambda i: i % 10 == 9, name=f"n{n}_k{k}") for k in (3, 200)]
-    saved = {p: L.ukhip_qr_launch_get(p, n) for p in (1, 2, 3)}
+    saved = {p: L.ukhip_qr_launch_get(0, p, n) for p in (1, 2, 3)}
     L.ukhip_use_graphs(0)
     try:
         ref = None
         for p in (1, 2, 3):
             for wg in (1, 2, 4):
-                assert L.ukhip_qr_launch_set(p, n, wg) == 0
+                assert L.ukhip_qr_launch_set(0, p, n, wg) == 0
                 b = dev.Batch(probs)
                 b.smooth("oddeven")
                 out = [b.download(i) for i in range(len(probs))]
                 b.free()
-                L.ukhip_qr_launch_set(p, n, saved[p])
+                L.ukhip_qr_launch_set(0, p, n, saved[p])
                 if ref is None:
                     ref = out
                 for (x0, c0), (x1, c1) in zip(ref, out):
@@ -51,8 +51,34 @@ def test_variants_bit_identical(dev, n, m):
         say(f"n={n}: 9 launch variants bit-identical")
     finally:
         for p, w in saved.items():
-            L.ukhip_qr_launch_set(p, n, w)
+            L.ukhip_qr_launch_set(0, p, n, w)
         L.ukhip_use_graphs(1)
+
+
+@pytest.mark.parametrize("n,m", [(40, 20)])
+def test_fp32_variants_bit_identical(dev, n, m):
+    """The FP32 elimination's launch variants: identical mixed-precision estimates."""
+    L = dev.lib()
+    F = 0.97 * P.random_orthogonal(P.rng_for(820 + n), n)
+    pr = P.iid_problem(300, n, m, seed=830, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1, name=f"n{n}_k300")
+    saved = {p: L.ukhip_qr_launch_get(1, p, n) for p in (1, 2, 3)}
+    try:
+        ref = None
+        for p in (1, 2, 3):
+            for wg in (1, 2, 4):
+                assert L.ukhip_qr_launch_set(1, p, n, wg) == 0
+                b = dev.Batch([pr])
+                b.smooth_mixed(1)
+                x = b.download(0, want_cov=False)[0]
+                b.free()
+                L.ukhip_qr_launch_set(1, p, n, saved[p])
+                if ref is None:
+                    ref = x
+                assert np.array_equal(ref, x), (p, wg)
+        say(f"n={n}: 9 FP32 launch variants bit-identical")
+    finally:
+        for p, w in saved.items():
+            L.ukhip_qr_launch_set(1, p, n, w)
 
 
 def test_autotune_table(dev):

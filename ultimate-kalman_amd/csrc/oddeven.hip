@@ -1512,22 +1512,27 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
   return hipGetLastError();
 }
 
-/* Run-time launch table of the FP64 phase kernels at NB >= 32: workgroups per
- * CU asked of the register allocator, per phase and bucket (32, 48, 64);
- * starts from the measured table, rewritten by ukhip_autotune. */
-static int g_qr_wg[3][3] = {{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
-                            {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
-                            {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}};
+/* Run-time launch table of the phase kernels at NB >= 32 (FP64, and the FP32
+ * elimination of the mixed-precision path): workgroups per CU asked of the
+ * register allocator, per precision, phase and bucket (32, 48, 64); starts
+ * from the measured table, rewritten by ukhip_autotune. */
+static int g_qr_wg[2][3][3] = {{{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
+                                {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
+                                {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}},
+                               {{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
+                                {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
+                                {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}}};
 static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
 
-extern "C" int ukhip_qr_launch_get(int phase, int n) {
+/* fp32: 0 the FP64 kernels, 1 the FP32 elimination of the mixed-precision path */
+extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n) {
   const int bi = qr_bucket_index(uk_bucket(n));
-  return (phase < 1 || phase > 3 || bi < 0) ? 0 : g_qr_wg[phase - 1][bi];
+  return (phase < 1 || phase > 3 || bi < 0) ? 0 : g_qr_wg[fp32 ? 1 : 0][phase - 1][bi];
 }
-extern "C" int ukhip_qr_launch_set(int phase, int n, int wg) {
+extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int wg) {
   const int bi = qr_bucket_index(uk_bucket(n));
   if (phase < 1 || phase > 3 || bi < 0 || (wg != 1 && wg != 2 && wg != 4)) return -1;
-  g_qr_wg[phase - 1][bi] = wg;
+  g_qr_wg[fp32 ? 1 : 0][phase - 1][bi] = wg;
   return 0;
 }
 
@@ -1549,8 +1554,8 @@ void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, 
 template <int NB, class T>
 void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                   int n, hipStream_t st, int phase) {
-  if constexpr (sizeof(T) == 8 && NB >= 32) {
-    switch (g_qr_wg[phase - 1][qr_bucket_index(NB)]) {
+  if constexpr (NB >= 32) {
+    switch (g_qr_wg[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)]) {
       case 1: phases_nb<NB, T, 1>(lv, in, out, fac, fix, n, st, phase); break;
       case 2: phases_nb<NB, T, 2>(lv, in, out, fac, fix, n, st, phase); break;
       default: phases_nb<NB, T, 4>(lv, in, out, fac, fix, n, st, phase); break;

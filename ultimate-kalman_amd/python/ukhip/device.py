@@ -70,8 +70,8 @@ def lib():
         L.ukhip_server_destroy.argtypes = [C.c_void_p]
         L.ukhip_autotune.argtypes = [C.c_int32, C.c_int32]
         L.ukhip_autotune.restype = C.c_int
-        L.ukhip_qr_launch_get.argtypes = [C.c_int, C.c_int]
-        L.ukhip_qr_launch_set.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.ukhip_qr_launch_get.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.ukhip_qr_launch_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.ukhip_filter_create.restype = C.c_void_p
         L.ukhip_filter_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
         L.ukhip_filter_step.argtypes = [C.c_void_p, C.c_void_p]
@@ -93,7 +93,21 @@ def autotune(n_max=64, k=4096):
 
 def launch_table():
     L = lib()
-    return {f"phase{p}<{nb}>": L.ukhip_qr_launch_get(p, nb) for nb in (32, 48, 64) for p in (1, 2, 3)}
+    return {f"phase{p}<{nb}, {t}>": L.ukhip_qr_launch_get(f, p, nb)
+            for f, t in ((0, "double"), (1, "float")) for nb in (32, 48, 64) for p in (1, 2, 3)}
+
+
+def set_launch_table(table):
+    """Sets launch-table entries from a mapping in launch_table()'s form
+    ({"phase2<48, double>": 1, ...}), e.g. one saved from an earlier autotune."""
+    import re
+    L = lib()
+    for key, wg in table.items():
+        mt = re.fullmatch(r"phase([123])<(\d+), (double|float)>", key.strip())
+        if mt is None:
+            raise ValueError(f"launch table key {key!r}: expected 'phase<1..3><<bucket>, double|float>'")
+        _check(L.ukhip_qr_launch_set(int(mt.group(3) == "float"), int(mt.group(1)), int(mt.group(2)), int(wg)),
+               f"ukhip_qr_launch_set {key}={wg}")
 
 
 def last_error():
