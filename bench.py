@@ -419,6 +419,9 @@ def run_batch(args):
                    "hbm_budget_gb": args.hbm_budget_gb},
         "whole_path_tflops": flops_total / (ms_total * 1e-3) / 1e12,
         "mixed_precision": mixed_acc or None,
+        "qr_launch_table": device.launch_table(),
+        "qr_launch_table_source": ("set by --launch-table" if args.launch_table else
+                                   "built-in" if args.no_autotune else "autotuned (ukhip_autotune)"),
     }
 
 
