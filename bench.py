@@ -462,7 +462,8 @@ def main():
     ap.add_argument("--no-filter", action="store_true", help="skip the batched sequential filter")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in QR launch table")
     ap.add_argument("--launch-table", default=None, metavar="JSON",
-                    help='set QR launch-table entries instead of autotuning, e.g. \'{"phase2<48, float>": 4}\'')
+                    help='set QR launch-table entries instead of autotuning, e.g. \'{"phase2<48, double>": "c2w2"}\' '
+                         '(c: columns per lane group, w: workgroups per CU)')
     ap.add_argument("--filter-count", type=int, default=4096)
     ap.add_argument("--filter-n", type=int, default=16)
     ap.add_argument("--filter-steps", type=int, default=200)

@@ -68,14 +68,19 @@ void ukhip_use_graphs(int enable);
 void ukhip_use_mfma(int enable);
 /* Launch table of the odd-even QR phase kernels (phase 1..3) for state
  * dimensions n > 16, per precision (fp32 = 0: the FP64 kernels, 1: the FP32
- * elimination of the mixed-precision path): the workgroups per CU (1, 2 or 4)
- * their register allocation leaves room for. ukhip_autotune times every
+ * elimination of the mixed-precision path). An entry is a compiled variant:
+ * the matrix columns each group of lanes holds (columns_per_group; 2 shares
+ * one register copy of each Householder vector between two columns) and the
+ * workgroups per CU the register allocation leaves room for: 1, 2 or 4 with
+ * one column per group, 2, 3 or 4 with two. ukhip_autotune times every
  * variant on a synthetic problem of k steps (0: 4096) per bucket up to n_max
- * and keeps the fastest; the variants run the same instructions, so results do
- * not change. Graphs captured earlier keep their launches. Returns 0 or -1. */
+ * and keeps the fastest; the variants run the same operations on every
+ * column, so results do not change. Graphs captured earlier keep their
+ * launches. get returns the workgroups per CU (0: no such entry) and stores
+ * the columns per group; set and autotune return 0 or -1. */
 int ukhip_autotune(int32_t n_max, int32_t k);
-int ukhip_qr_launch_get(int fp32, int phase, int n);
-int ukhip_qr_launch_set(int fp32, int phase, int n, int workgroups_per_cu);
+int ukhip_qr_launch_get(int fp32, int phase, int n, int* columns_per_group);
+int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_group, int workgroups_per_cu);
 /* Associative-scan smoothing of every problem of the batch. */
 int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options);
 int ukhip_synchronize(void);

@@ -198,23 +198,32 @@ def test_smoother_without_device_fails_loudly(lib):
 
 def test_launch_table_set_and_get():
     """The QR launch table is host state: entries round-trip without a device,
-    bad entries are refused, and set_launch_table parses launch_table()'s keys."""
+    variants that are not compiled are refused, and set_launch_table parses
+    launch_table()'s form."""
     from ukhip import device
     L = device.lib()
     saved = device.launch_table()
     try:
-        assert set(saved.values()) <= {1, 2, 4} and len(saved) == 18
-        assert L.ukhip_qr_launch_set(1, 2, 48, 4) == 0 and L.ukhip_qr_launch_get(1, 2, 48) == 4
-        assert L.ukhip_qr_launch_set(0, 2, 48, 3) == -1
-        assert L.ukhip_qr_launch_set(0, 4, 48, 1) == -1
-        assert L.ukhip_qr_launch_set(0, 1, 8, 1) == -1      # n <= 16 has no variants
-        device.set_launch_table({"phase3<64, double>": 2, "phase1<32, float>": 1})
+        assert len(saved) == 18
+        assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4"}
+        cpl = C.c_int(0)
+        assert L.ukhip_qr_launch_set(1, 2, 48, 2, 3) == 0
+        assert L.ukhip_qr_launch_get(1, 2, 48, C.byref(cpl)) == 3 and cpl.value == 2
+        assert L.ukhip_qr_launch_set(0, 2, 48, 1, 3) == -1   # not compiled
+        assert L.ukhip_qr_launch_set(0, 2, 48, 2, 1) == -1
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 2) == -1
+        assert L.ukhip_qr_launch_set(0, 4, 48, 1, 1) == -1
+        assert L.ukhip_qr_launch_set(0, 1, 8, 1, 1) == -1     # n <= 16 has no variants
+        assert L.ukhip_qr_launch_get(0, 1, 8, None) == 0
+        device.set_launch_table({"phase3<64, double>": "c2w4", "phase1<32, float>": "c1w1"})
         t = device.launch_table()
-        assert t["phase3<64, double>"] == 2 and t["phase1<32, float>"] == 1
+        assert t["phase3<64, double>"] == "c2w4" and t["phase1<32, float>"] == "c1w1"
         with pytest.raises(ValueError):
-            device.set_launch_table({"phase2<48>": 1})
+            device.set_launch_table({"phase2<48>": "c1w1"})
+        with pytest.raises(ValueError):
+            device.set_launch_table({"phase2<48, double>": 2})
         with pytest.raises(RuntimeError):
-            device.set_launch_table({"phase2<48, double>": 8})
+            device.set_launch_table({"phase2<48, double>": "c1w8"})
     finally:
         device.set_launch_table(saved)
     assert device.launch_table() == saved

@@ -1,9 +1,11 @@
 """The QR launch table and its run-time autotuner (ukhip_autotune).
 
-Every launch variant (1, 2, 4 workgroups per CU asked of the register
-allocator) runs the same instructions: results must be bit-identical across
-variants. The autotuner must leave a valid table (every entry 1, 2 or 4).
+Every launch variant (one column per lane group at 1, 2 or 4 workgroups per
+CU asked of the register allocator, two columns per group at 2, 3 or 4) runs
+the same operations on every column: results must be bit-identical across
+variants. The autotuner must leave a valid table.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -12,6 +14,8 @@ import pytest
 from ukhip import problems as P
 
 pytestmark = pytest.mark.gpu
+
+VARIANTS = [(1, 1), (1, 2), (1, 4), (2, 2), (2, 3), (2, 4)]  # (columns per group, workgroups per CU)
 
 
 def say(msg):
@@ -26,62 +30,68 @@ def dev():
     return device
 
 
+def _get(L, fp32, p, n):
+    cpl = C.c_int(0)
+    wg = L.ukhip_qr_launch_get(fp32, p, n, C.byref(cpl))
+    return cpl.value, wg
+
+
 @pytest.mark.parametrize("n,m", [(24, 12), (40, 20), (64, 32)])
 def test_variants_bit_identical(dev, n, m):
     L = dev.lib()
     F = 0.97 * P.random_orthogonal(P.rng_for(800 + n), n)
     probs = [P.iid_problem(k, n, m, seed=810 + k, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
                            unobserved=lambda i: i % 10 == 9, name=f"n{n}_k{k}") for k in (3, 200)]
-    saved = {p: L.ukhip_qr_launch_get(0, p, n) for p in (1, 2, 3)}
+    saved = {p: _get(L, 0, p, n) for p in (1, 2, 3)}
     L.ukhip_use_graphs(0)
     try:
         ref = None
         for p in (1, 2, 3):
-            for wg in (1, 2, 4):
-                assert L.ukhip_qr_launch_set(0, p, n, wg) == 0
+            for cpl, wg in VARIANTS:
+                assert L.ukhip_qr_launch_set(0, p, n, cpl, wg) == 0
                 b = dev.Batch(probs)
                 b.smooth("oddeven")
                 out = [b.download(i) for i in range(len(probs))]
                 b.free()
-                L.ukhip_qr_launch_set(0, p, n, saved[p])
+                L.ukhip_qr_launch_set(0, p, n, *saved[p])
                 if ref is None:
                     ref = out
                 for (x0, c0), (x1, c1) in zip(ref, out):
-                    assert np.array_equal(x0, x1) and np.array_equal(c0, c1), (p, wg)
-        say(f"n={n}: 9 launch variants bit-identical")
+                    assert np.array_equal(x0, x1) and np.array_equal(c0, c1), (p, cpl, wg)
+        say(f"n={n}: {3 * len(VARIANTS)} launch variants bit-identical")
     finally:
-        for p, w in saved.items():
-            L.ukhip_qr_launch_set(0, p, n, w)
+        for p, v in saved.items():
+            L.ukhip_qr_launch_set(0, p, n, *v)
         L.ukhip_use_graphs(1)
 
 
-@pytest.mark.parametrize("n,m", [(40, 20)])
+@pytest.mark.parametrize("n,m", [(40, 20), (48, 24)])
 def test_fp32_variants_bit_identical(dev, n, m):
     """The FP32 elimination's launch variants: identical mixed-precision estimates."""
     L = dev.lib()
     F = 0.97 * P.random_orthogonal(P.rng_for(820 + n), n)
-    pr = P.iid_problem(300, n, m, seed=830, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1, name=f"n{n}_k300")
-    saved = {p: L.ukhip_qr_launch_get(1, p, n) for p in (1, 2, 3)}
+    pr = P.iid_problem(301, n, m, seed=830, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1, name=f"n{n}_k301")
+    saved = {p: _get(L, 1, p, n) for p in (1, 2, 3)}
     try:
         ref = None
         for p in (1, 2, 3):
-            for wg in (1, 2, 4):
-                assert L.ukhip_qr_launch_set(1, p, n, wg) == 0
+            for cpl, wg in VARIANTS:
+                assert L.ukhip_qr_launch_set(1, p, n, cpl, wg) == 0
                 b = dev.Batch([pr])
                 b.smooth_mixed(1)
                 x = b.download(0, want_cov=False)[0]
                 b.free()
-                L.ukhip_qr_launch_set(1, p, n, saved[p])
+                L.ukhip_qr_launch_set(1, p, n, *saved[p])
                 if ref is None:
                     ref = x
-                assert np.array_equal(ref, x), (p, wg)
-        say(f"n={n}: 9 FP32 launch variants bit-identical")
+                assert np.array_equal(ref, x), (p, cpl, wg)
+        say(f"n={n}: {3 * len(VARIANTS)} FP32 launch variants bit-identical")
     finally:
-        for p, w in saved.items():
-            L.ukhip_qr_launch_set(1, p, n, w)
+        for p, v in saved.items():
+            L.ukhip_qr_launch_set(1, p, n, *v)
 
 
 def test_autotune_table(dev):
     table = dev.autotune(64, 1024)
     say(f"autotuned launch table: {table}")
-    assert set(table.values()) <= {1, 2, 4}
+    assert set(table.values()) <= {f"c{c}w{w}" for c, w in VARIANTS}

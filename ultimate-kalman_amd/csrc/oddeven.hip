@@ -158,185 +158,276 @@ struct Refl {
   T sc[2][2];
 };
 
-/* Reflector prefetch: the whole reflector is held in registers when a lane's
- * share has at most UK_QR_FULL entries, else it streams in chunks of UK_QR_CH. */
+/* Reflector prefetch: with one column per lane group the whole reflector is
+ * held in registers when a lane's share has at most UK_QR_FULL entries, else
+ * it streams in chunks of UK_QR_CH; with two columns (whose registers leave
+ * less room) it streams in chunks of at most UK_QR_CH2 whenever it is longer,
+ * read once for the dot products and once for the updates. */
 #ifndef UK_QR_FULL
 #define UK_QR_FULL 24
 #endif
 #ifndef UK_QR_CH
 #define UK_QR_CH 16
 #endif
+#ifndef UK_QR_CH2
+#define UK_QR_CH2 12
+#endif
+/* the largest even divisor of nl not above cap (chunks tile the share exactly) */
+__host__ __device__ constexpr int qr_chunk(int nl, int cap) {
+  int c = cap < nl ? cap : nl;
+  c -= c & 1;
+  while (c > 2 && nl % c != 0) c -= 2;
+  return c;
+}
 
 /*
- * One Householder step for pivot k (target row k of block 0).
- *   is_piv: these lanes own pivot column k; upd: this column is updated.
- * The pivot lanes publish the whole reflector (u_i = 0 for i < k and its
- * scalar; the two forms below); then every updated column applies
- * H = I - tau u u^T. The dot and the update run over all locals
- * without branches (entries outside the active rows are exact zeros), so the
- * LDS reads of v issue back to back. One barrier.
+ * Step-dependent row ranges. A column's locals are cut into granules of GL
+ * locals (GL * SL rows of one block); the pivot steps k in segment
+ * SEG = k / (GL * SL) share which granules can hold a nonzero of the
+ * Householder vector:
+ *   block 0: rows >= k only (u_i = 0 above the pivot), so granules >= SEG;
+ *   block 1: all rows, or with TRI1 (the stacked block starts upper
+ *   triangular, as R~ does in phase 2, and every reflector so far kept it so)
+ *   rows <= k only, so granules <= SEG.
+ * The other granules of u are exact zeros: skipping them changes no result
+ * bit (their products are exact zeros in every dot product and update), and
+ * the step code is specialised per segment, so the skipping costs no branch.
  */
-template <int NB, int SL, class T>
-__device__ __forceinline__ void house_step(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, bool is_piv, bool upd,
-                                           int k) {
+__host__ __device__ constexpr int qr_gl(int H) { return H <= 16 ? 2 : 4; }
+__host__ __device__ constexpr bool qr_need(int l, int H, int SEG, bool TRI1) {
+  return l < H ? (l / qr_gl(H) >= SEG) : (!TRI1 || (l - H) / qr_gl(H) <= SEG);
+}
+
+/*
+ * One Householder step for pivot k (target row k of block 0), over the CPL
+ * columns a lane group holds; k lies in segment SEG.
+ *   piv_slot: which of this group's columns is pivot column k (-1: none);
+ *   upd: bit s set when column s is updated (uniform over the group's lanes).
+ * The pivot lanes publish the reflector (u_i = 0 for i < k and its scalar;
+ * the two forms below); then every updated column applies H = I - tau u u^T.
+ * The dot and the update run over the segment's granules without branches,
+ * so the LDS reads of v issue back to back; the reflector read into registers
+ * serves all CPL columns. Each column's operations and their order do not
+ * depend on CPL. One barrier.
+ */
+template <int NB, int SL, class T, int SEG, bool TRI1>
+__device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, int k) {
   constexpr int H = NB / SL;
   T* vv = &R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE];
-  if (is_piv) {
-    T s0 = T(0), s1 = T(0), alpha = T(0);
+  T s0 = T(0), s1 = T(0), alpha = T(0);
+#pragma unroll
+  for (int l = 0; l < H; ++l) {
+    if (!qr_need(l, H, SEG, TRI1)) continue;
+    const int r = SL * l + sub;
+    const T x = col.a[l];
+    alpha = (r == k) ? x : alpha;
+    const T xm = (r > k) ? x : T(0);
+    if (l & 1) s1 = fma(xm, xm, s1);
+    else s0 = fma(xm, xm, s0);
+  }
+#pragma unroll
+  for (int l = H; l < 2 * H; ++l) {
+    if (!qr_need(l, H, SEG, TRI1)) continue;
+    if (l & 1) s1 = fma(col.a[l], col.a[l], s1);
+    else s0 = fma(col.a[l], col.a[l], s0);
+  }
+  const T s = group_sum<SL>(s0 + s1);
+  alpha = group_sum<SL>(alpha);
+  if constexpr (sizeof(T) == 8) {
+    /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
+     * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
+     * pivot lanes publish their column as it is (one division, no
+     * scaling). |beta u_k| >= beta^2 = ||x||^2, so a column whose squared
+     * norm is below 1e-280 counts as zero (no reflection). */
+    T taup = T(0), ukk = T(0);
+    if (s > T(1e-280)) {
+      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+      ukk = alpha - beta;
+      taup = T(-1) / (beta * ukk);
+    }
 #pragma unroll
     for (int l = 0; l < H; ++l) {
+      if (!qr_need(l, H, SEG, TRI1)) continue;
       const int r = SL * l + sub;
-      const T x = col.a[l];
-      alpha = (r == k) ? x : alpha;
-      const T xm = (r > k) ? x : T(0);
-      if (l & 1) s1 = fma(xm, xm, s1);
-      else s0 = fma(xm, xm, s0);
+      vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
     }
 #pragma unroll
-    for (int l = H; l < 2 * H; ++l) {
-      if (l & 1) s1 = fma(col.a[l], col.a[l], s1);
-      else s0 = fma(col.a[l], col.a[l], s0);
+    for (int l = H; l < 2 * H; ++l)
+      if (qr_need(l, H, SEG, TRI1)) vv[l] = col.a[l];
+    if (sub == 0) R.sc[buf][0] = taup;
+  } else {
+    /* single precision: the unit-pivot form v = x / (alpha - beta), v_k = 1,
+     * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
+     * the form above does, would leave FP32's range on nearly dependent
+     * columns (m << n) */
+    T tau = T(0), scale = T(0);
+    if (s != T(0)) {
+      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
+      tau = (beta - alpha) / beta;
+      scale = T(1) / (alpha - beta);
     }
-    const T s = group_sum<SL>(s0 + s1);
-    alpha = group_sum<SL>(alpha);
-    if constexpr (sizeof(T) == 8) {
-      /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
-       * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
-       * pivot lanes publish their column as it is (one division, no
-       * scaling). |beta u_k| >= beta^2 = ||x||^2, so a column whose squared
-       * norm is below 1e-280 counts as zero (no reflection). */
-      T taup = T(0), ukk = T(0);
-      if (s > T(1e-280)) {
-        const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-        ukk = alpha - beta;
-        taup = T(-1) / (beta * ukk);
-      }
 #pragma unroll
-      for (int l = 0; l < H; ++l) {
-        const int r = SL * l + sub;
-        vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
-      }
-#pragma unroll
-      for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l];
-      if (sub == 0) R.sc[buf][0] = taup;
-    } else {
-      /* single precision: the unit-pivot form v = x / (alpha - beta), v_k = 1,
-       * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
-       * the form above does, would leave FP32's range on nearly dependent
-       * columns (m << n) */
-      T tau = T(0), scale = T(0);
-      if (s != T(0)) {
-        const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-        tau = (beta - alpha) / beta;
-        scale = T(1) / (alpha - beta);
-      }
-#pragma unroll
-      for (int l = 0; l < H; ++l) {
-        const int r = SL * l + sub;
-        vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
-      }
-#pragma unroll
-      for (int l = H; l < 2 * H; ++l) vv[l] = col.a[l] * scale;
-      if (sub == 0) R.sc[buf][0] = tau;
+    for (int l = 0; l < H; ++l) {
+      if (!qr_need(l, H, SEG, TRI1)) continue;
+      const int r = SL * l + sub;
+      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
     }
+#pragma unroll
+    for (int l = H; l < 2 * H; ++l)
+      if (qr_need(l, H, SEG, TRI1)) vv[l] = col.a[l] * scale;
+    if (sub == 0) R.sc[buf][0] = tau;
   }
+}
+
+template <int NB, int SL, int CPL, class T, int SEG, bool TRI1>
+__device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int buf, int sub,
+                                           int piv_slot, unsigned upd, int k) {
+  constexpr int H = NB / SL;
+#pragma unroll
+  for (int s = 0; s < CPL; ++s)
+    if (s == piv_slot) house_publish<NB, SL, T, SEG, TRI1>(cols[s], R, buf, sub, k);
   __syncthreads();
   if (upd) {
     const T tau = R.sc[buf][0];
     if (tau != T(0)) {
-      /* v is read from LDS in chunks of CH locals issued back to back */
+      /* v is read from LDS in chunks of CH locals issued back to back. With
+       * CPL > 1 every column of the group runs the same branch-free code (a
+       * column that is not updated gets the scale 0 and is left as it is), so
+       * the columns' chains interleave. */
       constexpr int NL = 2 * H;
-      constexpr int CH = NL <= UK_QR_FULL ? NL : UK_QR_CH;
+      constexpr int CH = CPL == 1 ? (NL <= UK_QR_FULL ? NL : qr_chunk(NL, UK_QR_CH)) : qr_chunk(NL, UK_QR_CH2);
+      static_assert(NL % CH == 0 && CH % 2 == 0, "reflector chunks must tile a lane's share");
+      static_assert(H % qr_gl(H) == 0, "granules must tile a block");
       using V2 = typename Vec2<T>::type;
-      const V2* v2 = reinterpret_cast<const V2*>(vv);
-      T d0 = T(0), d1 = T(0), d2 = T(0), d3 = T(0);
+      const V2* v2 = reinterpret_cast<const V2*>(&R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE]);
+      T d0[CPL], d1[CPL], d2[CPL], d3[CPL];
+#pragma unroll
+      for (int s = 0; s < CPL; ++s) d0[s] = d1[s] = d2[s] = d3[s] = T(0);
 #pragma unroll
       for (int c0 = 0; c0 < NL; c0 += CH) {
         T vr[CH];
 #pragma unroll
         for (int l = 0; l < CH; l += 2) {
+          if (!qr_need(c0 + l, H, SEG, TRI1)) continue;
           const V2 t = v2[(c0 + l) / 2];
           vr[l] = t.x;
           vr[l + 1] = t.y;
         }
 #pragma unroll
-        for (int l = 0; l < CH; l += 4) {
-          d0 = fma(vr[l], col.a[c0 + l], d0);
-          d1 = fma(vr[l + 1], col.a[c0 + l + 1], d1);
-          if (l + 2 < CH) d2 = fma(vr[l + 2], col.a[c0 + l + 2], d2);
-          if (l + 3 < CH) d3 = fma(vr[l + 3], col.a[c0 + l + 3], d3);
+        for (int s = 0; s < CPL; ++s) {
+          T* a = cols[s].a;
+#pragma unroll
+          for (int l = 0; l < CH; l += 4) {
+            if (qr_need(c0 + l, H, SEG, TRI1)) {
+              d0[s] = fma(vr[l], a[c0 + l], d0[s]);
+              d1[s] = fma(vr[l + 1], a[c0 + l + 1], d1[s]);
+            }
+            if (l + 2 < CH && qr_need(c0 + l + 2, H, SEG, TRI1)) {
+              d2[s] = fma(vr[l + 2], a[c0 + l + 2], d2[s]);
+              d3[s] = fma(vr[l + 3], a[c0 + l + 3], d3[s]);
+            }
+          }
         }
         if (CH == NL) { /* whole reflector in registers: update without re-reading */
-          const T d = group_sum<SL>((d0 + d1) + (d2 + d3));
-          const T tw = tau * d;
 #pragma unroll
-          for (int l = 0; l < NL; ++l) col.a[l] = fma(-tw, vr[l], col.a[l]);
+          for (int s = 0; s < CPL; ++s) {
+            const T d = group_sum<SL>((d0[s] + d1[s]) + (d2[s] + d3[s]));
+            const T tw = (CPL == 1 || ((upd >> s) & 1u)) ? tau * d : T(0);
+#pragma unroll
+            for (int l = 0; l < NL; ++l)
+              if (qr_need(l, H, SEG, TRI1)) cols[s].a[l] = fma(-tw, vr[l], cols[s].a[l]);
+          }
           return;
         }
       }
-      const T d = group_sum<SL>((d0 + d1) + (d2 + d3));
-      const T tw = tau * d;
+      T tw[CPL];
+#pragma unroll
+      for (int s = 0; s < CPL; ++s) {
+        const T d = group_sum<SL>((d0[s] + d1[s]) + (d2[s] + d3[s]));
+        tw[s] = (CPL == 1 || ((upd >> s) & 1u)) ? tau * d : T(0);
+      }
+      /* compiler-only memory clobber (no instruction): the update pass reads
+       * v from LDS again instead of keeping the first pass's values live */
+      asm volatile("" ::: "memory");
 #pragma unroll
       for (int c0 = 0; c0 < NL; c0 += CH) {
         T vr[CH];
 #pragma unroll
         for (int l = 0; l < CH; l += 2) {
+          if (!qr_need(c0 + l, H, SEG, TRI1)) continue;
           const V2 t = v2[(c0 + l) / 2];
           vr[l] = t.x;
           vr[l + 1] = t.y;
         }
 #pragma unroll
-        for (int l = 0; l < CH; ++l) col.a[c0 + l] = fma(-tw, vr[l], col.a[c0 + l]);
+        for (int s = 0; s < CPL; ++s) {
+#pragma unroll
+          for (int l = 0; l < CH; ++l)
+            if (qr_need(c0 + l, H, SEG, TRI1)) cols[s].a[c0 + l] = fma(-tw[s], vr[l], cols[s].a[c0 + l]);
+        }
       }
     }
   }
 }
 
-/* Lanes per column and threads of a QR kernel over COLS matrix columns:
- * 4 lanes per column up to 1024 threads, else 2. MINW is the waves per SIMD
- * asked of the register allocator so that two workgroups share a CU (every
- * Householder step is a latency chain; a second workgroup hides it). */
-/* Per phase (1..3) and bucket: lanes per column (0: 4 lanes per column up to
- * 1024 threads, else 2) and the workgroups per CU the register allocation
- * targets (1: none beyond fitting one). Measured on MI355X: a second
- * workgroup per CU hides the Householder steps' latency only where the
- * column and reflector still fit in the registers it leaves. */
-#ifndef UK_P1_WGS48
-#define UK_P1_WGS48 2
-#endif
-#ifndef UK_P2_WGS48
-#define UK_P2_WGS48 1
-#endif
-#ifndef UK_P2_SL48
-#define UK_P2_SL48 0
-#endif
-#ifndef UK_P3_WGS48
-#define UK_P3_WGS48 4
-#endif
-#ifndef UK_P2_WGS64
-#define UK_P2_WGS64 1
-#endif
-__host__ __device__ constexpr int qr_sl(int phase, int NB) { return (phase == 2 && NB == 48) ? UK_P2_SL48 : 0; }
-__host__ __device__ constexpr int qr_wgs(int phase, int NB) {
-  return NB == 48 ? (phase == 1 ? UK_P1_WGS48 : phase == 2 ? UK_P2_WGS48 : UK_P3_WGS48)
-                  : (NB == 64 && phase == 2 ? UK_P2_WGS64 : 1);
-}
+/* Lanes per column group and threads of a QR kernel over COLS matrix
+ * columns: 4 lanes per group up to 1024 threads, else 2. MINW is the waves
+ * per SIMD asked of the register allocator so that W workgroups share a CU
+ * (every Householder step is a latency chain; more workgroups hide it where
+ * the columns and the reflector still fit in the registers they leave). */
 /* WG > 0 overrides the table's workgroups per CU (the variants the run-time
- * autotuner chooses among). */
-template <int NB, int COLS, int PHASE, int WG = 0>
+ * autotuner chooses among); CPL is the number of columns each lane group
+ * holds (GRP = ceil(COLS / CPL) groups). */
+template <int NB, int COLS, int PHASE, int WG = 0, int CPL = 1>
 struct QRGeom {
-  static constexpr int SL = qr_sl(PHASE, NB) ? qr_sl(PHASE, NB) : ((NB <= 16 || 4 * COLS > 1024) ? 2 : 4);
-  static constexpr int NT = ((SL * COLS + 63) / 64) * 64;
-  static constexpr int W = WG > 0 ? WG : qr_wgs(PHASE, NB);
+  static constexpr int GRP = (COLS + CPL - 1) / CPL;
+  static constexpr int SL = (NB <= 16 || 4 * GRP > 1024) ? 2 : 4;
+  static constexpr int NT = ((SL * GRP + 63) / 64) * 64;
+  static constexpr int W = WG > 0 ? WG : 1;
   static constexpr int MINW_ = (W * (NT / 64) + 3) / 4;
   static constexpr int MINW = MINW_ < 1 ? 1 : (MINW_ > 8 ? 8 : MINW_);
 };
 
+/* Pivot steps of segment SEG and the segments after it. DONE: a pivot
+ * column that is finished (piv0 <= column < piv0 + k) is not updated any more;
+ * the reflectors leave its rows up to its pivot as they are (u is zero
+ * there), and the rest of it is not stored. */
+template <int NB, int SL, int CPL, class T, bool TRI1, bool DONE, int SEG = 0>
+__device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub,
+                                              int piv0, int n, unsigned upd) {
+  constexpr int H = NB / SL, ROWS = qr_gl(H) * SL;
+  if constexpr (SEG * ROWS < NB) {
+    const int k1 = n < (SEG + 1) * ROWS ? n : (SEG + 1) * ROWS;
+    for (int k = SEG * ROWS; k < k1; ++k) {
+      const int pc = piv0 + k;
+      unsigned u = upd;
+      if (DONE) {
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+          const int cc = grp * CPL + s;
+          if (cc >= piv0 && cc < pc) u &= ~(1u << s);
+        }
+      }
+      house_step<NB, SL, CPL, T, SEG, TRI1>(cols, R, k & 1, sub, pc / CPL == grp ? pc % CPL : -1, u, k);
+    }
+    house_qr_segs<NB, SL, CPL, T, TRI1, DONE, SEG + 1>(cols, R, grp, sub, piv0, n, upd);
+  }
+}
+
+/* QR of the columns of a workgroup: pivots k = 0..n-1 on columns piv0 + k.
+ * Lane group grp holds columns grp * CPL + s, s < CPL. TRI1 and DONE as
+ * above (both off: every granule of block 1, every column updated). */
+template <int NB, int SL, int CPL, class T, bool TRI1 = false, bool DONE = false>
+__device__ __forceinline__ void house_qr(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub, int piv0,
+                                         int n, unsigned upd) {
+  house_qr_segs<NB, SL, CPL, T, TRI1, DONE>(cols, R, grp, sub, piv0, n, upd);
+  __syncthreads();
+}
+/* one column per lane group */
 template <int NB, int SL, class T>
 __device__ __forceinline__ void house_qr(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int mycol, int sub, int piv0, int n,
                                          bool upd) {
-  for (int k = 0; k < n; ++k) house_step<NB, SL, T>(col, R, k & 1, sub, mycol == piv0 + k, upd, k);
-  __syncthreads();
+  house_qr<NB, SL, 1, T>(reinterpret_cast<Col<NB, SL, T>(&)[1]>(col), R, mycol, sub, piv0, n, upd ? 1u : 0u);
 }
 
 template <int NB, int SL, int NT, class T>
@@ -364,13 +455,14 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
-template <int NB, class T, int WG = 0>
-__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG>::MINW))
+template <int NB, class T, int WG = 0, int CPL = 1>
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
-  constexpr int SL = QRGeom<NB, 2 * NB + 1, 1>::SL;
-  constexpr int NT = QRGeom<NB, 2 * NB + 1, 1>::NT;
+  using G = QRGeom<NB, 2 * NB + 1, 1, WG, CPL>;
+  constexpr int SL = G::SL;
+  constexpr int NT = G::NT;
   __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
   const int64_t u = e - lv.even_start[p];
@@ -381,51 +473,67 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG>::NT), (QRGeom<N
   const int mr = in.mr;
   const int64_t nn = (int64_t)n * n;
   const bool lone = j + 1 == L;
-  const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
-  const int qc = isP ? c : (isQ ? c - n : 0);
   refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL, T> col;
+  Col<NB, SL, T> cols[CPL];
+  unsigned upd = 0;
   const int64_t sj = s0 + j, sp = s0 + j + 1;
-  if (isP) {
-    if (lone) col.zero_block(0);
-    else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
-    col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
-  }
-  if (isQ) {
-    col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
-    col.zero_block(1);
-  }
-  if (isR1) {
-    if (lone) col.zero_block(0);
-    else col.load_block(0, in.c + sp * n, n, sub);
-    col.load_block(1, in.o + sj * n, mr, sub);
-  }
-  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isP || isQ || isR1);
-  if (isP) {
-    col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
-    for (int r = sub; r < n; r += SL) {
-      if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = T(0);
-      if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = T(0);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    Col<NB, SL, T>& col = cols[q];
+    const int c = grp * CPL + q;
+    const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
+    const int qc = isP ? c : (isQ ? c - n : 0);
+    if (isP) {
+      if (lone) col.zero_block(0);
+      else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
+      col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
     }
+    if (isQ) {
+      col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
+      col.zero_block(1);
+    }
+    if (isR1) {
+      if (lone) col.zero_block(0);
+      else col.load_block(0, in.c + sp * n, n, sub);
+      col.load_block(1, in.o + sj * n, mr, sub);
+    }
+    if (isP || isQ || isR1) upd |= 1u << q;
+    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  if (isQ) {
-    col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);     /* X_j */
-    col.store_block(1, out.C + t * nn + (int64_t)qc * mr, mr, sub);  /* D~_{j+1} */
-  }
-  if (isR1) {
-    col.store_block(0, fac.cf + e * n, n, sub);                       /* o_j */
-    if (!lone) col.store_block(1, out.o + t * n, mr, sub);           /* c~_{j+1} */
+  house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const Col<NB, SL, T>& col = cols[q];
+    const int c = grp * CPL + q;
+    const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
+    const int qc = isP ? c : (isQ ? c - n : 0);
+    if (isP) {
+      col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
+      for (int r = sub; r < n; r += SL) {
+        if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = T(0);
+        if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = T(0);
+      }
+    }
+    if (isQ) {
+      col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);     /* X_j */
+      col.store_block(1, out.C + t * nn + (int64_t)qc * mr, mr, sub);  /* D~_{j+1} */
+    }
+    if (isR1) {
+      col.store_block(0, fac.cf + e * n, n, sub);                       /* o_j */
+      if (!lone) col.store_block(1, out.o + t * n, mr, sub);           /* c~_{j+1} */
+    }
   }
 }
 
-template <int NB, class T, int WG = 0>
-__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG>::MINW))
+template <int NB, class T, int WG = 0, int CPL = 1>
+__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::MINW))
     k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
-  constexpr int SL = QRGeom<NB, 3 * NB + 1, 2>::SL;
-  constexpr int NT = QRGeom<NB, 3 * NB + 1, 2>::NT;
+  using G = QRGeom<NB, 3 * NB + 1, 2, WG, CPL>;
+  constexpr int SL = G::SL;
+  constexpr int NT = G::NT;
   __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
   const int64_t u = e - lv.even_start[p];
@@ -435,52 +543,69 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG>::NT), (QRGeom<N
   const int64_t t = lv.next_start[p] + u;
   const int64_t nn = (int64_t)n * n;
   const bool lone = j + 1 == L;
-  const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
-  const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
   refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL, T> col;
+  Col<NB, SL, T> cols[CPL];
+  unsigned upd = 0;
   const int64_t sj = lv.step_start[p] + j;
-  if (isP) {
-    col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
-    col.load_block(1, fac.R + e * nn + (int64_t)qc * n, n, sub); /* R~_j */
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    Col<NB, SL, T>& col = cols[q];
+    const int c = grp * CPL + q;
+    const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
+    const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
+    if (isP) {
+      col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
+      col.load_block(1, fac.R + e * nn + (int64_t)qc * n, n, sub); /* R~_j */
+    }
+    if (isQ) {
+      col.zero_block(0);
+      col.load_block(1, fac.Y + e * nn + (int64_t)qc * n, n, sub); /* X_j (zero for a lone step) */
+    }
+    if (isR1) {
+      col.load_block(0, in.c + sj * n, n, sub);
+      col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
+    }
+    if (isBl) {
+      col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
+      col.zero_block(1);
+    }
+    if (isP || isQ || isR1 || isBl) upd |= 1u << q;
+    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  if (isQ) {
-    col.zero_block(0);
-    col.load_block(1, fac.Y + e * nn + (int64_t)qc * n, n, sub); /* X_j (zero for a lone step) */
-  }
-  if (isR1) {
-    col.load_block(0, in.c + sj * n, n, sub);
-    col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
-  }
-  if (isBl) {
-    col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
-    col.zero_block(1);
-  }
-  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isP || isQ || isR1 || isBl);
+  /* block 1 of the pivot columns is R~_j, upper triangular */
+  house_qr<NB, SL, CPL, T, true, true>(cols, refl, grp, sub, 0, n, upd);
   T* fz = fix + (int64_t)p * (nn + n);
-  if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
-  if (isBl) {
-    col.store_block(0, fac.Bt + e * nn + (int64_t)qc * n, n, sub);
-    col.store_block(1, (lone ? fz : out.B + t * nn) + (int64_t)qc * n, n, sub); /* Z_j */
-  }
-  if (isQ) {
-    col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);
-    if (!lone) col.store_block(1, out.D + t * nn + (int64_t)qc * n, n, sub); /* X~_j */
-  }
-  if (isR1) {
-    col.store_block(0, fac.cf + e * n, n, sub);
-    col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const Col<NB, SL, T>& col = cols[q];
+    const int c = grp * CPL + q;
+    const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
+    const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
+    if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
+    if (isBl) {
+      col.store_block(0, fac.Bt + e * nn + (int64_t)qc * n, n, sub);
+      col.store_block(1, (lone ? fz : out.B + t * nn) + (int64_t)qc * n, n, sub); /* Z_j */
+    }
+    if (isQ) {
+      col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);
+      if (!lone) col.store_block(1, out.D + t * nn + (int64_t)qc * n, n, sub); /* X~_j */
+    }
+    if (isR1) {
+      col.store_block(0, fac.cf + e * n, n, sub);
+      col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
+    }
   }
 }
 
-template <int NB, class T, int WG = 0>
-__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG>::NT), (QRGeom<NB, NB + 1, 3, WG>::MINW))
+template <int NB, class T, int WG = 0, int CPL = 1>
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<NB, NB + 1, 3, WG, CPL>::MINW))
     k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
-  constexpr int SL = QRGeom<NB, NB + 1, 3>::SL;
-  constexpr int NT = QRGeom<NB, NB + 1, 3>::NT;
+  using G = QRGeom<NB, NB + 1, 3, WG, CPL>;
+  constexpr int SL = G::SL;
+  constexpr int NT = G::NT;
   __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t unit = blockIdx.x;
   const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
   const int64_t L = lv.L[p];
@@ -491,29 +616,48 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG>::NT), (QRGeom<NB, N
   const int mr = in.mr;
   const int64_t nn = (int64_t)n * n;
   const bool do_fix = (L & 1) && (u == L / 2 - 1);
-  const bool isQ = c < n, isR1 = c == n; /* Q columns numbered from 0 here */
-  const int qc = isQ ? c : 0;
   refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL, T> col;
+  /* Q columns numbered from 0 here */
+  Col<NB, SL, T> cols[CPL];
+  unsigned upd = 0;
   const int64_t sp = lv.step_start[p] + j + 1;
-  if (isQ) {
-    col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
-    col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    Col<NB, SL, T>& col = cols[q];
+    const int c = grp * CPL + q;
+    const bool isQ = c < n, isR1 = c == n;
+    const int qc = isQ ? c : 0;
+    if (isQ) {
+      col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
+      col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
+    }
+    if (isR1) {
+      col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
+      col.load_block(1, in.o + sp * n, mr, sub);
+    }
+    if (isQ || isR1) upd |= 1u << q;
+    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  if (isR1) {
-    col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
-    col.load_block(1, in.o + sp * n, mr, sub);
-  }
-  house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isQ || isR1);
+  house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
   if (do_fix) {
     const T* fz = fix + (int64_t)p * (nn + n);
-    if (isQ) col.load_block(1, fz + (int64_t)qc * n, n, sub);
-    if (isR1) col.load_block(1, fz + nn, n, sub);
-    house_qr<NB, SL, T>(col, refl, c, sub, 0, n, isQ || isR1);
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int c = grp * CPL + q;
+      const int qc = c < n ? c : 0;
+      if (c < n) cols[q].load_block(1, fz + (int64_t)qc * n, n, sub);
+      if (c == n) cols[q].load_block(1, fz + nn, n, sub);
+    }
+    house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
   }
-  if (isQ) col.store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
-  if (isR1) col.store_block(0, out.o + t * n, n, sub);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    const int qc = c < n ? c : 0;
+    if (c < n) cols[q].store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
+    if (c == n) cols[q].store_block(0, out.o + t * n, n, sub);
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1513,52 +1657,77 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
 }
 
 /* Run-time launch table of the phase kernels at NB >= 32 (FP64, and the FP32
- * elimination of the mixed-precision path): workgroups per CU asked of the
- * register allocator, per precision, phase and bucket (32, 48, 64); starts
- * from the measured table, rewritten by ukhip_autotune. */
-static int g_qr_wg[2][3][3] = {{{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
-                                {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
-                                {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}},
-                               {{qr_wgs(1, 32), qr_wgs(1, 48), qr_wgs(1, 64)},
-                                {qr_wgs(2, 32), qr_wgs(2, 48), qr_wgs(2, 64)},
-                                {qr_wgs(3, 32), qr_wgs(3, 48), qr_wgs(3, 64)}}};
+ * elimination of the mixed-precision path), per precision, phase and bucket
+ * (32, 48, 64): the columns each lane group holds (cpl) and the workgroups per
+ * CU asked of the register allocator (wg). Starts from variants measured
+ * fastest on MI355X (n = 48 and 64: k = 200000; n = 32: k = 16384), rewritten
+ * by ukhip_autotune. */
+struct QrLaunch {
+  int cpl, wg;
+};
+static QrLaunch g_qr[2][3][3] = {
+    /* FP64:  bucket 32   48        64 */
+    {/* phase 1 */ {{1, 4}, {2, 2}, {1, 1}},
+     /* phase 2 */ {{1, 4}, {1, 1}, {1, 1}},
+     /* phase 3 */ {{1, 2}, {2, 2}, {2, 3}}},
+    /* FP32 */
+    {/* phase 1 */ {{1, 1}, {1, 4}, {1, 4}},
+     /* phase 2 */ {{1, 1}, {1, 4}, {2, 2}},
+     /* phase 3 */ {{1, 1}, {1, 1}, {2, 2}}}};
 static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
+/* the compiled variants: one column per group at 1, 2 or 4 workgroups per CU,
+ * two columns per group at 2, 3 or 4 */
+static bool qr_variant(int cpl, int wg) {
+  return (cpl == 1 && (wg == 1 || wg == 2 || wg == 4)) || (cpl == 2 && (wg == 2 || wg == 3 || wg == 4));
+}
 
 /* fp32: 0 the FP64 kernels, 1 the FP32 elimination of the mixed-precision path */
-extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n) {
+extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n, int* columns_per_group) {
   const int bi = qr_bucket_index(uk_bucket(n));
-  return (phase < 1 || phase > 3 || bi < 0) ? 0 : g_qr_wg[fp32 ? 1 : 0][phase - 1][bi];
+  if (phase < 1 || phase > 3 || bi < 0) return 0;
+  const QrLaunch v = g_qr[fp32 ? 1 : 0][phase - 1][bi];
+  if (columns_per_group) *columns_per_group = v.cpl;
+  return v.wg;
 }
-extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int wg) {
+extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_group, int wg) {
   const int bi = qr_bucket_index(uk_bucket(n));
-  if (phase < 1 || phase > 3 || bi < 0 || (wg != 1 && wg != 2 && wg != 4)) return -1;
-  g_qr_wg[fp32 ? 1 : 0][phase - 1][bi] = wg;
+  if (phase < 1 || phase > 3 || bi < 0 || !qr_variant(columns_per_group, wg)) return -1;
+  g_qr[fp32 ? 1 : 0][phase - 1][bi] = QrLaunch{columns_per_group, wg};
   return 0;
 }
 
 namespace {
-template <int NB, class T, int WG>
+template <int NB, class T, int WG, int CPL = 1>
 void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
                hipStream_t st, int phase) {
   if (phase == 1)
-    k_phase1<NB, T, WG><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 2 * NB + 1, 1, WG>::NT), 0, st>>>(*lv, in, out,
-                                                                                                      fac, n);
+    k_phase1<NB, T, WG, CPL><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), 0, st>>>(
+        *lv, in, out, fac, n);
   else if (phase == 2)
-    k_phase2<NB, T, WG><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 3 * NB + 1, 2, WG>::NT), 0, st>>>(*lv, in, out,
-                                                                                                      fac, fix, n);
+    k_phase2<NB, T, WG, CPL><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), 0, st>>>(
+        *lv, in, out, fac, fix, n);
   else
-    k_phase3<NB, T, WG><<<dim3((unsigned)lv->units), dim3(QRGeom<NB, NB + 1, 3, WG>::NT), 0, st>>>(*lv, in, out, fix,
-                                                                                                  n);
+    k_phase3<NB, T, WG, CPL><<<dim3((unsigned)lv->units), dim3(QRGeom<NB, NB + 1, 3, WG, CPL>::NT), 0, st>>>(
+        *lv, in, out, fix, n);
 }
 
 template <int NB, class T>
 void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                   int n, hipStream_t st, int phase) {
   if constexpr (NB >= 32) {
-    switch (g_qr_wg[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)]) {
-      case 1: phases_nb<NB, T, 1>(lv, in, out, fac, fix, n, st, phase); break;
-      case 2: phases_nb<NB, T, 2>(lv, in, out, fac, fix, n, st, phase); break;
-      default: phases_nb<NB, T, 4>(lv, in, out, fac, fix, n, st, phase); break;
+    const QrLaunch v = g_qr[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)];
+    if (v.cpl == 2) {
+      switch (v.wg) {
+        case 2: phases_nb<NB, T, 2, 2>(lv, in, out, fac, fix, n, st, phase); break;
+        case 3: phases_nb<NB, T, 3, 2>(lv, in, out, fac, fix, n, st, phase); break;
+        default: phases_nb<NB, T, 4, 2>(lv, in, out, fac, fix, n, st, phase); break;
+      }
+    } else {
+      switch (v.wg) {
+        case 1: phases_nb<NB, T, 1>(lv, in, out, fac, fix, n, st, phase); break;
+        case 2: phases_nb<NB, T, 2>(lv, in, out, fac, fix, n, st, phase); break;
+        default: phases_nb<NB, T, 4>(lv, in, out, fac, fix, n, st, phase); break;
+      }
     }
   } else {
     phases_nb<NB, T, 0>(lv, in, out, fac, fix, n, st, phase);

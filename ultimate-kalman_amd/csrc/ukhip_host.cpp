@@ -2074,8 +2074,7 @@ extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
 /* run-time autotuning of the QR launch table                                 */
 /* ------------------------------------------------------------------------ */
 
-extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n);
-extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int wg);
+
 
 /* For every bucket (32, 48, 64) up to n_max: a synthetic problem (n = bucket,
  * m = n/2, k steps, F = 0.98 times a Householder reflection, G uniform with
@@ -2114,40 +2113,39 @@ extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
     p.G = G.data(); p.o = o.data(); p.C = Call.data(); p.C_type = 'w';
     ukhip_batch_t* b = ukhip_batch_create(1, &p);
     if (!b) return -1;
-    for (int phase = 1; phase <= 3; ++phase) {
-      int best = ukhip_qr_launch_get(0, phase, n);
-      double best_ms = 1e30;
-      for (int wg : {1, 2, 4}) {
-        ukhip_qr_launch_set(0, phase, n, wg);
-        double ms[UKHIP_OE_SLOTS];
-        double t = 0.0;
-        for (int rep = 0; rep < 3; ++rep) {
-          if (ukhip_batch_smooth_oddeven_timed(b, 0, ms, nullptr) != 0) { ukhip_batch_free(b); return -1; }
-          if (rep > 0) t += ms[UKHIP_OE_PHASE1 + phase - 1];
-        }
-        if (t < best_ms) { best_ms = t; best = wg; }
-      }
-      ukhip_qr_launch_set(0, phase, n, best);
-    }
-    /* the FP32 elimination: whole mixed-precision smoothings (no refinement
-     * step) timed on the host, one phase varied at a time */
-    for (int phase = 1; phase <= 3; ++phase) {
-      int best = ukhip_qr_launch_get(1, phase, n);
-      double best_ms = 1e30;
-      for (int wg : {1, 2, 4}) {
-        ukhip_qr_launch_set(1, phase, n, wg);
-        double t = 0.0;
-        for (int rep = 0; rep < 3; ++rep) {
-          const auto t0 = std::chrono::steady_clock::now();
-          if (ukhip_batch_smooth_oddeven_mixed(b, 0) != 0 || ukhip_synchronize() != 0) {
-            ukhip_batch_free(b);
-            return -1;
+    static const int kVariants[6][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 2}, {2, 3}, {2, 4}}; /* {cpl, wg} */
+    /* FP64: the phase's HIP events; FP32 (the mixed-precision elimination):
+     * whole mixed-precision smoothings without a refinement step, timed on
+     * the host. One phase varied at a time. */
+    for (int fp32 = 0; fp32 <= 1; ++fp32) {
+      for (int phase = 1; phase <= 3; ++phase) {
+        int best_c = 1;
+        int best_w = ukhip_qr_launch_get(fp32, phase, n, &best_c);
+        double best_ms = 1e30;
+        for (const auto& var : kVariants) {
+          ukhip_qr_launch_set(fp32, phase, n, var[0], var[1]);
+          double t = 0.0;
+          for (int rep = 0; rep < 3; ++rep) {
+            double ms[UKHIP_OE_SLOTS];
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = fp32 ? (ukhip_batch_smooth_oddeven_mixed(b, 0) != 0 || ukhip_synchronize() != 0)
+                                : ukhip_batch_smooth_oddeven_timed(b, 0, ms, nullptr);
+            if (rc != 0) {
+              ukhip_batch_free(b);
+              return -1;
+            }
+            if (rep == 0) continue;
+            t += fp32 ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count()
+                      : ms[UKHIP_OE_PHASE1 + phase - 1];
           }
-          if (rep > 0) t += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+          if (t < best_ms) {
+            best_ms = t;
+            best_c = var[0];
+            best_w = var[1];
+          }
         }
-        if (t < best_ms) { best_ms = t; best = wg; }
+        ukhip_qr_launch_set(fp32, phase, n, best_c, best_w);
       }
-      ukhip_qr_launch_set(1, phase, n, best);
     }
     ukhip_batch_free(b);
   }

@@ -70,8 +70,8 @@ def lib():
         L.ukhip_server_destroy.argtypes = [C.c_void_p]
         L.ukhip_autotune.argtypes = [C.c_int32, C.c_int32]
         L.ukhip_autotune.restype = C.c_int
-        L.ukhip_qr_launch_get.argtypes = [C.c_int, C.c_int, C.c_int]
-        L.ukhip_qr_launch_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ukhip_qr_launch_get.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.ukhip_qr_launch_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ukhip_filter_create.restype = C.c_void_p
         L.ukhip_filter_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
         L.ukhip_filter_step.argtypes = [C.c_void_p, C.c_void_p]
@@ -85,29 +85,39 @@ def lib():
 
 
 def autotune(n_max=64, k=4096):
-    """ukhip_autotune; returns the launch table {(phase, bucket): workgroups per CU}."""
+    """ukhip_autotune; returns the launch table (launch_table())."""
     L = lib()
     _check(L.ukhip_autotune(n_max, k), "ukhip_autotune")
     return launch_table()
 
 
 def launch_table():
+    """The QR launch table: {"phase<p><<bucket>, double|float>": "c<columns per group>w<workgroups per CU>"}."""
     L = lib()
-    return {f"phase{p}<{nb}, {t}>": L.ukhip_qr_launch_get(f, p, nb)
-            for f, t in ((0, "double"), (1, "float")) for nb in (32, 48, 64) for p in (1, 2, 3)}
+    out = {}
+    for f, t in ((0, "double"), (1, "float")):
+        for nb in (32, 48, 64):
+            for p in (1, 2, 3):
+                cpl = C.c_int(0)
+                wg = L.ukhip_qr_launch_get(f, p, nb, C.byref(cpl))
+                out[f"phase{p}<{nb}, {t}>"] = f"c{cpl.value}w{wg}"
+    return out
 
 
 def set_launch_table(table):
     """Sets launch-table entries from a mapping in launch_table()'s form
-    ({"phase2<48, double>": 1, ...}), e.g. one saved from an earlier autotune."""
+    ({"phase2<48, double>": "c2w3", ...}), e.g. one saved from an earlier autotune."""
     import re
     L = lib()
-    for key, wg in table.items():
+    for key, val in table.items():
         mt = re.fullmatch(r"phase([123])<(\d+), (double|float)>", key.strip())
-        if mt is None:
-            raise ValueError(f"launch table key {key!r}: expected 'phase<1..3><<bucket>, double|float>'")
-        _check(L.ukhip_qr_launch_set(int(mt.group(3) == "float"), int(mt.group(1)), int(mt.group(2)), int(wg)),
-               f"ukhip_qr_launch_set {key}={wg}")
+        mv = re.fullmatch(r"c(\d)w(\d)", str(val).strip())
+        if mt is None or mv is None:
+            raise ValueError(f"launch table entry {key!r}: {val!r}: expected 'phase<1..3><<bucket>, double|float>': "
+                             "'c<columns per group>w<workgroups per CU>'")
+        _check(L.ukhip_qr_launch_set(int(mt.group(3) == "float"), int(mt.group(1)), int(mt.group(2)),
+                                     int(mv.group(1)), int(mv.group(2))),
+               f"ukhip_qr_launch_set {key}={val}")
 
 
 def last_error():
