@@ -75,7 +75,30 @@ def kernel_label(slot, pr, cov=True):
         if nb >= 32:
             return f"k_backsub_mfma<{nb}>" if cov else f"k_estimate<{nb}>"
         return f"k_backsub<{nb}>"
-    return f"{slot}<{nb}, double>"
+    # the phase kernels' template arguments: bucket, scalar, workgroups per CU
+    # and columns per lane group of the launch variant, and for phases 1 and 3
+    # whether the level's observation rows fill at most half a block (level 0:
+    # m rows, the levels above: n)
+    phase = int(slot[-1])
+    if nb >= 32:
+        v = device.launch_table()[f"phase{phase}<{nb}, double>"]
+        cpl, wg = int(v[1]), int(v[3])
+    else:
+        cpl, wg = 1, 0
+    args = f"{nb}, double, {wg}, {cpl}"
+    if phase != 2:
+        half0 = 2 * max(m_list_of(pr)) <= nb
+        half1 = 2 * n <= nb
+        args += ", " + ("true" if half0 else "false") + ("" if half1 == half0 else "|" + ("true" if half1 else "false"))
+    return f"{slot}<{args}>"
+
+
+def base_label(label):
+    """A kernel label without its launch-variant arguments (the key of the
+    traffic summary)."""
+    import re
+    m = re.match(r"^(k_phase\d<\d+, (?:double|float))(?:, [^>]*)?>$", label)
+    return m.group(1) + ">" if m else label
 
 
 def traffic_of(kernel, workload):
@@ -351,7 +374,7 @@ def run_batch(args):
         device.set_launch_table(json.loads(args.launch_table))
     elif not args.no_autotune:
         t = time.perf_counter()
-        table = device.autotune(32, 16384)
+        table = device.autotune(32, 65536)
         print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
               flush=True)
     specs = P.batch_specs(count=args.batch_count)
@@ -494,7 +517,7 @@ def main():
         device.set_launch_table(json.loads(args.launch_table))
     elif not args.no_autotune:
         n_tune = WORKLOADS[args.workload]["n"] if args.no_precision else max(WORKLOADS[args.workload]["n"], 64)
-        table = device.autotune(n_tune, 16384)
+        table = device.autotune(n_tune, 65536)
         print(f"[bench] autotuned the QR launch table in {time.perf_counter() - t:.1f}s: {table}", file=sys.stderr,
               flush=True)
     t = time.perf_counter()
@@ -533,7 +556,7 @@ def main():
     dom = max(fl_of, key=lambda k: kern[k])
     label = kernel_label(dom, pr)
     achieved = fl_of[dom] / (kern[dom] * 1e-3) / 1e12
-    traffic = traffic_of(label, args.workload if args.k is None else None)
+    traffic = traffic_of(base_label(label), args.workload if args.k is None else None)
     if traffic is not None and dom == "k_backsub":
         traffic["algorithmic_bytes_per_step"] = flops.oddeven_backsub_bytes(n, pr.k)
 

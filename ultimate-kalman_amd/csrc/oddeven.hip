@@ -9,7 +9,9 @@
  * k_weigh      whitening of one step per workgroup (c/kalman_base.c:61-175).
  * k_phase1..3  the reference's three elimination phases of one level, one
  *              workgroup per step pair (or even step), writing the
- *              back-substitution factors and the next level's chain.
+ *              back-substitution factors and the next level's chain
+ *              (oddeven_qr.h; the buckets above 16 are compiled in
+ *              oddeven_qr{32,48,64}.hip).
  * k_backsub    one workgroup per even step: estimate and, optionally, the
  *              selected-inversion covariance and the two cross covariances.
  *
@@ -28,637 +30,12 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "oddeven_qr.h"
 #include "ukhip_internal.h"
 
 namespace {
-
-/* x + (x of the lane with index ^1) + ... over groups of SL = 2 or 4 adjacent
- * lanes (DPP quad permutations, no LDS). */
-__device__ __forceinline__ double dpp_xor1(double v) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false); /* quad_perm [1,0,3,2] */
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double dpp_xor2(double v) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false); /* quad_perm [2,3,0,1] */
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float dpp_xor1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float dpp_xor2(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
-}
-template <int SL, class T>
-__device__ __forceinline__ T group_sum(T v) {
-  v += dpp_xor1(v);
-  if (SL == 4) v += dpp_xor2(v);
-  return v;
-}
-
-/* Device views of a whitened chain and of the back-substitution factors in
- * the working precision T (double: the layouts of uk_chain_t and
- * uk_factors_t; float: the same layouts in single precision). */
-template <class T>
-struct ChainT {
-  T *B, *D, *c, *C, *o;
-  int32_t mr;
-};
-template <class T>
-struct FactorsT {
-  T *R, *Bt, *Y, *cf;
-};
-/* 2-vector of T for paired LDS reads */
-template <class T>
-struct Vec2;
-template <>
-struct Vec2<double> {
-  using type = double2;
-};
-template <>
-struct Vec2<float> {
-  using type = float2;
-};
-
-__device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= x) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-/* One matrix column as held by one of its SL lanes: rows r = SL*l + sub of
- * block 0 in locals [0,H) and of block 1 in [H,2H), H = NB/SL. */
-template <int NB, int SL, class T = double>
-struct Col {
-  static constexpr int H = NB / SL;
-  T a[2 * H];
-
-  __device__ __forceinline__ void zero_block(int blk) {
-#pragma unroll
-    for (int l = 0; l < H; ++l) a[blk * H + l] = T(0);
-  }
-  /* block blk <- rows [0, rows) of a column at src (unit stride), zero elsewhere */
-  __device__ __forceinline__ void load_block(int blk, const T* __restrict__ src, int rows, int sub) {
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      const int r = SL * l + sub;
-      a[blk * H + l] = (r < rows) ? src[r] : T(0);
-    }
-  }
-  __device__ __forceinline__ void store_block(int blk, T* __restrict__ dst, int rows, int sub) const {
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      const int r = SL * l + sub;
-      if (r < rows) dst[r] = a[blk * H + l];
-    }
-  }
-  /* upper-triangular store of block 0: rows > col are written as zero */
-  __device__ __forceinline__ void store_triu(T* __restrict__ dst, int rows, int col, int sub) const {
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      const int r = SL * l + sub;
-      if (r < rows) dst[r] = (r <= col) ? a[l] : T(0);
-    }
-  }
-  __device__ __forceinline__ void move_0_to_1() {
-#pragma unroll
-    for (int l = 0; l < H; ++l) a[H + l] = a[l];
-  }
-  /* block 0 <- block 1, block 1 <- 0 */
-  __device__ __forceinline__ void move_1_to_0() {
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      a[l] = a[H + l];
-      a[H + l] = T(0);
-    }
-  }
-  /* rows r < rows of block blk scaled by sign * s[r] */
-  __device__ __forceinline__ void scale_block(int blk, const T* __restrict__ s, int rows, int sub, T sign) {
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      const int r = SL * l + sub;
-      if (r < rows) a[blk * H + l] *= sign * s[r];
-    }
-  }
-};
-
-/* LDS reflector buffers: [buffer][sub][2H + 4] (the +4 skews the sub-lane
- * arrays onto different banks), then [buffer][2] scalars. */
-template <int NB, int SL, class T = double>
-struct Refl {
-  static constexpr int SUBSTRIDE = 2 * (NB / SL) + 4;
-  T v[2][SL * SUBSTRIDE];
-  T sc[2][2];
-};
-
-/* Reflector prefetch: with one column per lane group the whole reflector is
- * held in registers when a lane's share has at most UK_QR_FULL entries, else
- * it streams in chunks of UK_QR_CH; with two columns (whose registers leave
- * less room) it streams in chunks of at most UK_QR_CH2 whenever it is longer,
- * read once for the dot products and once for the updates. */
-#ifndef UK_QR_FULL
-#define UK_QR_FULL 24
-#endif
-#ifndef UK_QR_CH
-#define UK_QR_CH 16
-#endif
-#ifndef UK_QR_CH2
-#define UK_QR_CH2 12
-#endif
-/* the largest even divisor of nl not above cap (chunks tile the share exactly) */
-__host__ __device__ constexpr int qr_chunk(int nl, int cap) {
-  int c = cap < nl ? cap : nl;
-  c -= c & 1;
-  while (c > 2 && nl % c != 0) c -= 2;
-  return c;
-}
-
-/*
- * Step-dependent row ranges. A column's locals are cut into granules of GL
- * locals (GL * SL rows of one block); the pivot steps k in segment
- * SEG = k / (GL * SL) share which granules can hold a nonzero of the
- * Householder vector:
- *   block 0: rows >= k only (u_i = 0 above the pivot), so granules >= SEG;
- *   block 1: all rows, or with TRI1 (the stacked block starts upper
- *   triangular, as R~ does in phase 2, and every reflector so far kept it so)
- *   rows <= k only, so granules <= SEG.
- * The other granules of u are exact zeros: skipping them changes no result
- * bit (their products are exact zeros in every dot product and update), and
- * the step code is specialised per segment, so the skipping costs no branch.
- */
-__host__ __device__ constexpr int qr_gl(int H) { return H <= 16 ? 2 : 4; }
-__host__ __device__ constexpr bool qr_need(int l, int H, int SEG, bool TRI1) {
-  return l < H ? (l / qr_gl(H) >= SEG) : (!TRI1 || (l - H) / qr_gl(H) <= SEG);
-}
-
-/*
- * One Householder step for pivot k (target row k of block 0), over the CPL
- * columns a lane group holds; k lies in segment SEG.
- *   piv_slot: which of this group's columns is pivot column k (-1: none);
- *   upd: bit s set when column s is updated (uniform over the group's lanes).
- * The pivot lanes publish the reflector (u_i = 0 for i < k and its scalar;
- * the two forms below); then every updated column applies H = I - tau u u^T.
- * The dot and the update run over the segment's granules without branches,
- * so the LDS reads of v issue back to back; the reflector read into registers
- * serves all CPL columns. Each column's operations and their order do not
- * depend on CPL. One barrier.
- */
-template <int NB, int SL, class T, int SEG, bool TRI1>
-__device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, int k) {
-  constexpr int H = NB / SL;
-  T* vv = &R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE];
-  T s0 = T(0), s1 = T(0), alpha = T(0);
-#pragma unroll
-  for (int l = 0; l < H; ++l) {
-    if (!qr_need(l, H, SEG, TRI1)) continue;
-    const int r = SL * l + sub;
-    const T x = col.a[l];
-    alpha = (r == k) ? x : alpha;
-    const T xm = (r > k) ? x : T(0);
-    if (l & 1) s1 = fma(xm, xm, s1);
-    else s0 = fma(xm, xm, s0);
-  }
-#pragma unroll
-  for (int l = H; l < 2 * H; ++l) {
-    if (!qr_need(l, H, SEG, TRI1)) continue;
-    if (l & 1) s1 = fma(col.a[l], col.a[l], s1);
-    else s0 = fma(col.a[l], col.a[l], s0);
-  }
-  const T s = group_sum<SL>(s0 + s1);
-  alpha = group_sum<SL>(alpha);
-  if constexpr (sizeof(T) == 8) {
-    /* H = I - tau' u u^T with u = x below the pivot and u_k = alpha - beta:
-     * tau' = -1 / (beta u_k), the unit-pivot form's tau / u_k^2, so the
-     * pivot lanes publish their column as it is (one division, no
-     * scaling). |beta u_k| >= beta^2 = ||x||^2, so a column whose squared
-     * norm is below 1e-280 counts as zero (no reflection). */
-    T taup = T(0), ukk = T(0);
-    if (s > T(1e-280)) {
-      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-      ukk = alpha - beta;
-      taup = T(-1) / (beta * ukk);
-    }
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      if (!qr_need(l, H, SEG, TRI1)) continue;
-      const int r = SL * l + sub;
-      vv[l] = (r > k) ? col.a[l] : (r == k ? ukk : T(0));
-    }
-#pragma unroll
-    for (int l = H; l < 2 * H; ++l)
-      if (qr_need(l, H, SEG, TRI1)) vv[l] = col.a[l];
-    if (sub == 0) R.sc[buf][0] = taup;
-  } else {
-    /* single precision: the unit-pivot form v = x / (alpha - beta), v_k = 1,
-     * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
-     * the form above does, would leave FP32's range on nearly dependent
-     * columns (m << n) */
-    T tau = T(0), scale = T(0);
-    if (s != T(0)) {
-      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-      tau = (beta - alpha) / beta;
-      scale = T(1) / (alpha - beta);
-    }
-#pragma unroll
-    for (int l = 0; l < H; ++l) {
-      if (!qr_need(l, H, SEG, TRI1)) continue;
-      const int r = SL * l + sub;
-      vv[l] = (r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0));
-    }
-#pragma unroll
-    for (int l = H; l < 2 * H; ++l)
-      if (qr_need(l, H, SEG, TRI1)) vv[l] = col.a[l] * scale;
-    if (sub == 0) R.sc[buf][0] = tau;
-  }
-}
-
-template <int NB, int SL, int CPL, class T, int SEG, bool TRI1>
-__device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int buf, int sub,
-                                           int piv_slot, unsigned upd, int k) {
-  constexpr int H = NB / SL;
-#pragma unroll
-  for (int s = 0; s < CPL; ++s)
-    if (s == piv_slot) house_publish<NB, SL, T, SEG, TRI1>(cols[s], R, buf, sub, k);
-  __syncthreads();
-  if (upd) {
-    const T tau = R.sc[buf][0];
-    if (tau != T(0)) {
-      /* v is read from LDS in chunks of CH locals issued back to back. With
-       * CPL > 1 every column of the group runs the same branch-free code (a
-       * column that is not updated gets the scale 0 and is left as it is), so
-       * the columns' chains interleave. */
-      constexpr int NL = 2 * H;
-      constexpr int CH = CPL == 1 ? (NL <= UK_QR_FULL ? NL : qr_chunk(NL, UK_QR_CH)) : qr_chunk(NL, UK_QR_CH2);
-      static_assert(NL % CH == 0 && CH % 2 == 0, "reflector chunks must tile a lane's share");
-      static_assert(H % qr_gl(H) == 0, "granules must tile a block");
-      using V2 = typename Vec2<T>::type;
-      const V2* v2 = reinterpret_cast<const V2*>(&R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE]);
-      T d0[CPL], d1[CPL], d2[CPL], d3[CPL];
-#pragma unroll
-      for (int s = 0; s < CPL; ++s) d0[s] = d1[s] = d2[s] = d3[s] = T(0);
-#pragma unroll
-      for (int c0 = 0; c0 < NL; c0 += CH) {
-        T vr[CH];
-#pragma unroll
-        for (int l = 0; l < CH; l += 2) {
-          if (!qr_need(c0 + l, H, SEG, TRI1)) continue;
-          const V2 t = v2[(c0 + l) / 2];
-          vr[l] = t.x;
-          vr[l + 1] = t.y;
-        }
-#pragma unroll
-        for (int s = 0; s < CPL; ++s) {
-          T* a = cols[s].a;
-#pragma unroll
-          for (int l = 0; l < CH; l += 4) {
-            if (qr_need(c0 + l, H, SEG, TRI1)) {
-              d0[s] = fma(vr[l], a[c0 + l], d0[s]);
-              d1[s] = fma(vr[l + 1], a[c0 + l + 1], d1[s]);
-            }
-            if (l + 2 < CH && qr_need(c0 + l + 2, H, SEG, TRI1)) {
-              d2[s] = fma(vr[l + 2], a[c0 + l + 2], d2[s]);
-              d3[s] = fma(vr[l + 3], a[c0 + l + 3], d3[s]);
-            }
-          }
-        }
-        if (CH == NL) { /* whole reflector in registers: update without re-reading */
-#pragma unroll
-          for (int s = 0; s < CPL; ++s) {
-            const T d = group_sum<SL>((d0[s] + d1[s]) + (d2[s] + d3[s]));
-            const T tw = (CPL == 1 || ((upd >> s) & 1u)) ? tau * d : T(0);
-#pragma unroll
-            for (int l = 0; l < NL; ++l)
-              if (qr_need(l, H, SEG, TRI1)) cols[s].a[l] = fma(-tw, vr[l], cols[s].a[l]);
-          }
-          return;
-        }
-      }
-      T tw[CPL];
-#pragma unroll
-      for (int s = 0; s < CPL; ++s) {
-        const T d = group_sum<SL>((d0[s] + d1[s]) + (d2[s] + d3[s]));
-        tw[s] = (CPL == 1 || ((upd >> s) & 1u)) ? tau * d : T(0);
-      }
-      /* compiler-only memory clobber (no instruction): the update pass reads
-       * v from LDS again instead of keeping the first pass's values live */
-      asm volatile("" ::: "memory");
-#pragma unroll
-      for (int c0 = 0; c0 < NL; c0 += CH) {
-        T vr[CH];
-#pragma unroll
-        for (int l = 0; l < CH; l += 2) {
-          if (!qr_need(c0 + l, H, SEG, TRI1)) continue;
-          const V2 t = v2[(c0 + l) / 2];
-          vr[l] = t.x;
-          vr[l + 1] = t.y;
-        }
-#pragma unroll
-        for (int s = 0; s < CPL; ++s) {
-#pragma unroll
-          for (int l = 0; l < CH; ++l)
-            if (qr_need(c0 + l, H, SEG, TRI1)) cols[s].a[c0 + l] = fma(-tw[s], vr[l], cols[s].a[c0 + l]);
-        }
-      }
-    }
-  }
-}
-
-/* Lanes per column group and threads of a QR kernel over COLS matrix
- * columns: 4 lanes per group up to 1024 threads, else 2. MINW is the waves
- * per SIMD asked of the register allocator so that W workgroups share a CU
- * (every Householder step is a latency chain; more workgroups hide it where
- * the columns and the reflector still fit in the registers they leave). */
-/* WG > 0 overrides the table's workgroups per CU (the variants the run-time
- * autotuner chooses among); CPL is the number of columns each lane group
- * holds (GRP = ceil(COLS / CPL) groups). */
-template <int NB, int COLS, int PHASE, int WG = 0, int CPL = 1>
-struct QRGeom {
-  static constexpr int GRP = (COLS + CPL - 1) / CPL;
-  static constexpr int SL = (NB <= 16 || 4 * GRP > 1024) ? 2 : 4;
-  static constexpr int NT = ((SL * GRP + 63) / 64) * 64;
-  static constexpr int W = WG > 0 ? WG : 1;
-  static constexpr int MINW_ = (W * (NT / 64) + 3) / 4;
-  static constexpr int MINW = MINW_ < 1 ? 1 : (MINW_ > 8 ? 8 : MINW_);
-};
-
-/* Pivot steps of segment SEG and the segments after it. DONE: a pivot
- * column that is finished (piv0 <= column < piv0 + k) is not updated any more;
- * the reflectors leave its rows up to its pivot as they are (u is zero
- * there), and the rest of it is not stored. */
-template <int NB, int SL, int CPL, class T, bool TRI1, bool DONE, int SEG = 0>
-__device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub,
-                                              int piv0, int n, unsigned upd) {
-  constexpr int H = NB / SL, ROWS = qr_gl(H) * SL;
-  if constexpr (SEG * ROWS < NB) {
-    const int k1 = n < (SEG + 1) * ROWS ? n : (SEG + 1) * ROWS;
-    for (int k = SEG * ROWS; k < k1; ++k) {
-      const int pc = piv0 + k;
-      unsigned u = upd;
-      if (DONE) {
-#pragma unroll
-        for (int s = 0; s < CPL; ++s) {
-          const int cc = grp * CPL + s;
-          if (cc >= piv0 && cc < pc) u &= ~(1u << s);
-        }
-      }
-      house_step<NB, SL, CPL, T, SEG, TRI1>(cols, R, k & 1, sub, pc / CPL == grp ? pc % CPL : -1, u, k);
-    }
-    house_qr_segs<NB, SL, CPL, T, TRI1, DONE, SEG + 1>(cols, R, grp, sub, piv0, n, upd);
-  }
-}
-
-/* QR of the columns of a workgroup: pivots k = 0..n-1 on columns piv0 + k.
- * Lane group grp holds columns grp * CPL + s, s < CPL. TRI1 and DONE as
- * above (both off: every granule of block 1, every column updated). */
-template <int NB, int SL, int CPL, class T, bool TRI1 = false, bool DONE = false>
-__device__ __forceinline__ void house_qr(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub, int piv0,
-                                         int n, unsigned upd) {
-  house_qr_segs<NB, SL, CPL, T, TRI1, DONE>(cols, R, grp, sub, piv0, n, upd);
-  __syncthreads();
-}
-/* one column per lane group */
-template <int NB, int SL, class T>
-__device__ __forceinline__ void house_qr(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int mycol, int sub, int piv0, int n,
-                                         bool upd) {
-  house_qr<NB, SL, 1, T>(reinterpret_cast<Col<NB, SL, T>(&)[1]>(col), R, mycol, sub, piv0, n, upd ? 1u : 0u);
-}
-
-template <int NB, int SL, int NT, class T>
-__device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
-  for (int i = tid; i < (int)(sizeof(R) / sizeof(T)); i += NT) reinterpret_cast<T*>(&R)[i] = T(0);
-  __syncthreads();
-}
-
-/*
- * Elimination of one level, three launches (the reference's three phases):
- *
- * k_phase1, one workgroup per even step j (:379-457). Pair (j, j+1): QR of
- *   [B_{j+1}; C_j] applied to [D_{j+1}; 0] and [c_{j+1}; o_j]. A lone last
- *   step (j = L-1) or a one-step chain: QR of [0; C_j] applied to [0; o_j].
- *   R~_j, X_j, o_j go to the factor slot of j (R, Y, cf), D~_{j+1} and
- *   c~_{j+1} to the next level's C and o slots of the new step.
- * k_phase2, one workgroup per even step j > 0 (:460-523): QR of [D_j; R~_j]
- *   applied to [B_j; 0], [c_j; o_j], [0; X_j]: final R, B~, Y, cf; the rest
- *   rows Z_j, X~_j, o_j become the new step's B, D, c (a lone last step's Z
- *   and o go to a per-problem scratch for the fix-last fold).
- * k_phase3, one workgroup per pair (:526-560): QR of [D~_{j+1}; C_{j+1}]
- *   applied to [c~_{j+1}; o_{j+1}]; in the last pair of an odd chain it then
- *   folds the lone step's [Z_{L-1}, o_{L-1}] in (:1452-1476). G~, c give the
- *   new step's C and o.
- * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
- * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
- */
-template <int NB, class T, int WG = 0, int CPL = 1>
-__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::MINW))
-    k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
-  using G = QRGeom<NB, 2 * NB + 1, 1, WG, CPL>;
-  constexpr int SL = G::SL;
-  constexpr int NT = G::NT;
-  __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
-  const int64_t e = blockIdx.x;
-  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
-  const int64_t u = e - lv.even_start[p];
-  const int64_t j = 2 * u;
-  const int64_t L = lv.L[p];
-  const int64_t s0 = lv.step_start[p];
-  const int64_t t = lv.next_start[p] + u;
-  const int mr = in.mr;
-  const int64_t nn = (int64_t)n * n;
-  const bool lone = j + 1 == L;
-  refl_clear<NB, SL, NT, T>(refl, tid);
-
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
-  const int64_t sj = s0 + j, sp = s0 + j + 1;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
-    const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
-    const int qc = isP ? c : (isQ ? c - n : 0);
-    if (isP) {
-      if (lone) col.zero_block(0);
-      else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
-      col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
-    }
-    if (isQ) {
-      col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
-      col.zero_block(1);
-    }
-    if (isR1) {
-      if (lone) col.zero_block(0);
-      else col.load_block(0, in.c + sp * n, n, sub);
-      col.load_block(1, in.o + sj * n, mr, sub);
-    }
-    if (isP || isQ || isR1) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
-  }
-  house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
-    const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
-    const int qc = isP ? c : (isQ ? c - n : 0);
-    if (isP) {
-      col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
-      for (int r = sub; r < n; r += SL) {
-        if (lone) fac.Y[e * nn + (int64_t)qc * n + r] = T(0);
-        if (j == 0) fac.Bt[e * nn + (int64_t)qc * n + r] = T(0);
-      }
-    }
-    if (isQ) {
-      col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);     /* X_j */
-      col.store_block(1, out.C + t * nn + (int64_t)qc * mr, mr, sub);  /* D~_{j+1} */
-    }
-    if (isR1) {
-      col.store_block(0, fac.cf + e * n, n, sub);                       /* o_j */
-      if (!lone) col.store_block(1, out.o + t * n, mr, sub);           /* c~_{j+1} */
-    }
-  }
-}
-
-template <int NB, class T, int WG = 0, int CPL = 1>
-__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::MINW))
-    k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
-  using G = QRGeom<NB, 3 * NB + 1, 2, WG, CPL>;
-  constexpr int SL = G::SL;
-  constexpr int NT = G::NT;
-  __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
-  const int64_t e = blockIdx.x;
-  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
-  const int64_t u = e - lv.even_start[p];
-  const int64_t j = 2 * u;
-  if (j == 0) return; /* the chain's first step has no [D; R~] (uniform per block) */
-  const int64_t L = lv.L[p];
-  const int64_t t = lv.next_start[p] + u;
-  const int64_t nn = (int64_t)n * n;
-  const bool lone = j + 1 == L;
-  refl_clear<NB, SL, NT, T>(refl, tid);
-
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
-  const int64_t sj = lv.step_start[p] + j;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
-    const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
-    const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
-    if (isP) {
-      col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
-      col.load_block(1, fac.R + e * nn + (int64_t)qc * n, n, sub); /* R~_j */
-    }
-    if (isQ) {
-      col.zero_block(0);
-      col.load_block(1, fac.Y + e * nn + (int64_t)qc * n, n, sub); /* X_j (zero for a lone step) */
-    }
-    if (isR1) {
-      col.load_block(0, in.c + sj * n, n, sub);
-      col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
-    }
-    if (isBl) {
-      col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
-      col.zero_block(1);
-    }
-    if (isP || isQ || isR1 || isBl) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
-  }
-  /* block 1 of the pivot columns is R~_j, upper triangular */
-  house_qr<NB, SL, CPL, T, true, true>(cols, refl, grp, sub, 0, n, upd);
-  T* fz = fix + (int64_t)p * (nn + n);
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
-    const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
-    const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
-    if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
-    if (isBl) {
-      col.store_block(0, fac.Bt + e * nn + (int64_t)qc * n, n, sub);
-      col.store_block(1, (lone ? fz : out.B + t * nn) + (int64_t)qc * n, n, sub); /* Z_j */
-    }
-    if (isQ) {
-      col.store_block(0, fac.Y + e * nn + (int64_t)qc * n, n, sub);
-      if (!lone) col.store_block(1, out.D + t * nn + (int64_t)qc * n, n, sub); /* X~_j */
-    }
-    if (isR1) {
-      col.store_block(0, fac.cf + e * n, n, sub);
-      col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
-    }
-  }
-}
-
-template <int NB, class T, int WG = 0, int CPL = 1>
-__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<NB, NB + 1, 3, WG, CPL>::MINW))
-    k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
-  using G = QRGeom<NB, NB + 1, 3, WG, CPL>;
-  constexpr int SL = G::SL;
-  constexpr int NT = G::NT;
-  __shared__ Refl<NB, SL, T> refl;
-  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
-  const int64_t unit = blockIdx.x;
-  const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
-  const int64_t L = lv.L[p];
-  if (L == 1) return; /* uniform per block */
-  const int64_t u = unit - lv.unit_start[p];
-  const int64_t j = 2 * u;
-  const int64_t t = lv.next_start[p] + u;
-  const int mr = in.mr;
-  const int64_t nn = (int64_t)n * n;
-  const bool do_fix = (L & 1) && (u == L / 2 - 1);
-  refl_clear<NB, SL, NT, T>(refl, tid);
-
-  /* Q columns numbered from 0 here */
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
-  const int64_t sp = lv.step_start[p] + j + 1;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
-    const bool isQ = c < n, isR1 = c == n;
-    const int qc = isQ ? c : 0;
-    if (isQ) {
-      col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
-      col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
-    }
-    if (isR1) {
-      col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
-      col.load_block(1, in.o + sp * n, mr, sub);
-    }
-    if (isQ || isR1) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
-  }
-  house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
-  if (do_fix) {
-    const T* fz = fix + (int64_t)p * (nn + n);
-#pragma unroll
-    for (int q = 0; q < CPL; ++q) {
-      const int c = grp * CPL + q;
-      const int qc = c < n ? c : 0;
-      if (c < n) cols[q].load_block(1, fz + (int64_t)qc * n, n, sub);
-      if (c == n) cols[q].load_block(1, fz + nn, n, sub);
-    }
-    house_qr<NB, SL, CPL, T, false, true>(cols, refl, grp, sub, 0, n, upd);
-  }
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const int c = grp * CPL + q;
-    const int qc = c < n ? c : 0;
-    if (c < n) cols[q].store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
-    if (c == n) cols[q].store_block(0, out.o + t * n, n, sub);
-  }
-}
+using ukqr::ChainT;
+using ukqr::FactorsT;
 
 /* ------------------------------------------------------------------------ */
 /* back substitution                                                          */
@@ -1671,7 +1048,7 @@ static QrLaunch g_qr[2][3][3] = {
      /* phase 2 */ {{1, 4}, {1, 1}, {1, 1}},
      /* phase 3 */ {{1, 2}, {2, 2}, {2, 3}}},
     /* FP32 */
-    {/* phase 1 */ {{1, 1}, {1, 4}, {1, 4}},
+    {/* phase 1 */ {{1, 1}, {2, 2}, {1, 4}},
      /* phase 2 */ {{1, 1}, {1, 4}, {2, 2}},
      /* phase 3 */ {{1, 1}, {1, 1}, {2, 2}}}};
 static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
@@ -1697,40 +1074,17 @@ extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_g
 }
 
 namespace {
-template <int NB, class T, int WG, int CPL = 1>
-void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
-               hipStream_t st, int phase) {
-  if (phase == 1)
-    k_phase1<NB, T, WG, CPL><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), 0, st>>>(
-        *lv, in, out, fac, n);
-  else if (phase == 2)
-    k_phase2<NB, T, WG, CPL><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), 0, st>>>(
-        *lv, in, out, fac, fix, n);
-  else
-    k_phase3<NB, T, WG, CPL><<<dim3((unsigned)lv->units), dim3(QRGeom<NB, NB + 1, 3, WG, CPL>::NT), 0, st>>>(
-        *lv, in, out, fix, n);
-}
-
 template <int NB, class T>
 void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                   int n, hipStream_t st, int phase) {
   if constexpr (NB >= 32) {
-    const QrLaunch v = g_qr[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)];
-    if (v.cpl == 2) {
-      switch (v.wg) {
-        case 2: phases_nb<NB, T, 2, 2>(lv, in, out, fac, fix, n, st, phase); break;
-        case 3: phases_nb<NB, T, 3, 2>(lv, in, out, fac, fix, n, st, phase); break;
-        default: phases_nb<NB, T, 4, 2>(lv, in, out, fac, fix, n, st, phase); break;
-      }
-    } else {
-      switch (v.wg) {
-        case 1: phases_nb<NB, T, 1>(lv, in, out, fac, fix, n, st, phase); break;
-        case 2: phases_nb<NB, T, 2>(lv, in, out, fac, fix, n, st, phase); break;
-        default: phases_nb<NB, T, 4>(lv, in, out, fac, fix, n, st, phase); break;
-      }
-    }
+    const QrLaunch q = g_qr[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)];
+    const ukqr::Launch v{q.cpl, q.wg};
+    if constexpr (NB == 32) ukqr::phase_launch_b32(lv, in, out, fac, fix, n, st, phase, v);
+    else if constexpr (NB == 48) ukqr::phase_launch_b48(lv, in, out, fac, fix, n, st, phase, v);
+    else ukqr::phase_launch_b64(lv, in, out, fac, fix, n, st, phase, v);
   } else {
-    phases_nb<NB, T, 0>(lv, in, out, fac, fix, n, st, phase);
+    phase_launch_impl<NB, T>(lv, in, out, fac, fix, n, st, phase, ukqr::Launch{1, 0});
   }
 }
 

@@ -30,6 +30,14 @@ def short_name(name):
     return re.sub(r"\(.*", "", name).strip()
 
 
+def base_name(name):
+    """The kernel without its launch-variant template arguments
+    ("k_phase1<48, double, 2, 2, true>" -> "k_phase1<48, double>"): the
+    variants move the same bytes, so traffic is keyed by the base name."""
+    m = re.match(r"^(k_phase\d<\d+, (?:double|float))(?:, [^>]*)?>$", name)
+    return m.group(1) + ">" if m else name
+
+
 def kernel_stats(d):
     rows = []
     with open(os.path.join(d, "run_kernel_stats.csv")) as f:
@@ -43,7 +51,7 @@ def counter_sums(d):
     s = collections.defaultdict(lambda: [0, 0.0])
     with open(os.path.join(d, "run_counter_collection.csv")) as f:
         for r in csv.DictReader(f):
-            e = s[short_name(r["Kernel_Name"])]
+            e = s[base_name(short_name(r["Kernel_Name"]))]
             e[0] += 1
             e[1] += float(r["Counter_Value"])
     return s
@@ -79,9 +87,9 @@ def main(argv=None):
     t.add_argument("--algorithmic", action="append", default=[], help="KERNEL=BYTES per step")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
-        print(f"{'kernel':28s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'ms/step':>9s} {'%':>6s}")
+        print(f"{'kernel':40s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'ms/step':>9s} {'%':>6s}")
         for name, calls, tot, avg, pct in kernel_stats(args.dir):
-            print(f"{name[:28]:28s} {calls:6d} {tot / 1e6:10.3f} {avg / 1e6:9.4f} {tot / 1e6 / args.per:9.3f} {pct:6.2f}")
+            print(f"{name[:40]:40s} {calls:6d} {tot / 1e6:10.3f} {avg / 1e6:9.4f} {tot / 1e6 / args.per:9.3f} {pct:6.2f}")
         return 0
     res, cf, cw = traffic(args.fetch, args.write, args.copy_bytes, args.per)
     alg = dict(x.split("=") for x in args.algorithmic)
