@@ -35,6 +35,13 @@ struct FactorPtrs {
   const T* Y;
 };
 
+/* v of lane `lane`, broadcast to the wave (two readlanes, no LDS) */
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -138,13 +145,15 @@ __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ gA, c
 
 /*
  * Forward sweep, even steps of a level (one wave per even step):
- * y_j = R_j^{-T} g_j. R is staged through LDS; lane i holds row i.
+ * y_j = R_j^{-T} g_j. R is staged through LDS in its own precision; lane i
+ * holds entry i, the solved y_k reach every lane by readlane, and the pivots'
+ * reciprocals are formed in parallel before the substitution.
  */
 template <int NB, class T>
 __global__ void __launch_bounds__(64) k_fwd_even(uk_level_t lv, const T* __restrict__ R, const double* __restrict__ g,
                                                  double* __restrict__ y, int n) {
   constexpr int L = NB + 1;
-  __shared__ double Rs[NB * L];
+  __shared__ T Rs[NB * L];
   const int lane = threadIdx.x;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
@@ -162,16 +171,20 @@ __global__ void __launch_bounds__(64) k_fwd_even(uk_level_t lv, const T* __restr
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int idx = lane + 64 * t;
-      if (idx < n * n) Rs[idx % n + (idx / n) * L] = (double)rr[t];
+      if (idx < n * n) Rs[idx % n + (idx / n) * L] = rr[t];
     }
   }
   double v = lane < n ? g[sj * n + lane] : 0.0;
   __syncthreads();
+  const double rinv = lane < n ? 1.0 / (double)Rs[lane + lane * L] : 0.0;
   /* R^T is lower triangular: (R^T)_{ik} = R_{ki} */
-  for (int k = 0; k < n; ++k) {
-    const double yk = __shfl(v, k) / Rs[k + k * L];
-    if (lane == k) v = yk;
-    else if (lane > k && lane < n) v = fma(-Rs[k + lane * L], yk, v);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (k < n) {
+      const double yk = lane_bcast(v * rinv, k);
+      if (lane == k) v = yk;
+      else if (lane > k && lane < n) v = fma(-(double)Rs[k + lane * L], yk, v);
+    }
   }
   if (lane < n) y[e * n + lane] = v;
 }
@@ -235,16 +248,15 @@ __global__ void __launch_bounds__(64) k_fwd_odd(uk_level_t lv, const T* __restri
 /*
  * Backward sweep (one wave per even step): x_j = R^{-1} (y_j - B~ x_{j-1} - Y x_{j+1})
  * with the neighbours' x from the levels above, in the original step order.
- * The three blocks are staged through LDS with coalesced loads first.
+ * Lane i owns row i: B~, Y and R stream column by column (coalesced, no
+ * LDS), lane i keeps row i of R in registers, and the neighbours' estimates
+ * and each solved x_q reach every lane by readlane.
  */
 template <int NB, class T, class TR>
 __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> fac, const TR* __restrict__ rhs,
                                                   double* __restrict__ x, int n) {
-  constexpr int L = NB + 1;
-  constexpr int PER = (NB * NB + 63) / 64;
-  __shared__ T Rs[NB * L], Bs[NB * L], Ys[NB * L];
-  __shared__ double xm[NB], xp[NB];
-  const int lane = threadIdx.x;
+  const int i = threadIdx.x;
+  const bool row = i < n;
   const int64_t e = blockIdx.x;
   const int p = (int)find_problem(lv.even_start, lv.nprob, e);
   const int64_t j = 2 * (e - lv.even_start[p]);
@@ -256,48 +268,34 @@ __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> f
   const int64_t oj = ob + (j + 1) * stride - 1;
   const int64_t om = ob + j * stride - 1;
   const int64_t op = ob + (j + 2) * stride - 1;
-  {
-    T rr[PER], rb[PER], ry[PER];
+  const T* Rg = fac.R + e * nn;
+  const T* Bg = fac.Bt + e * nn;
+  const T* Yg = fac.Y + e * nn;
+  const double xm = (has_m && row) ? x[om * n + i] : 0.0;
+  const double xp = (has_p && row) ? x[op * n + i] : 0.0;
+  double v = row ? (double)rhs[e * n + i] : 0.0, v1 = 0.0;
+  T rrow[NB];
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int idx = lane + 64 * t;
-      const bool in = idx < n * n;
-      rr[t] = in ? fac.R[e * nn + idx] : T(0);
-      rb[t] = (in && has_m) ? fac.Bt[e * nn + idx] : T(0);
-      ry[t] = (in && has_p) ? fac.Y[e * nn + idx] : T(0);
+  for (int q = 0; q < NB; ++q) {
+    rrow[q] = T(0);
+    if (q < n) {
+      const double b = (has_m && row) ? (double)Bg[i + (int64_t)q * n] : 0.0;
+      const double yv = (has_p && row) ? (double)Yg[i + (int64_t)q * n] : 0.0;
+      if (row) rrow[q] = Rg[i + (int64_t)q * n];
+      v = fma(-b, lane_bcast(xm, q), v);
+      v1 = fma(-yv, lane_bcast(xp, q), v1);
     }
-    if (lane < n) {
-      xm[lane] = has_m ? x[om * n + lane] : 0.0;
-      xp[lane] = has_p ? x[op * n + lane] : 0.0;
-    }
+  }
+  v += v1;
+  const double rinv = row ? 1.0 / (double)Rg[i + (int64_t)i * n] : 0.0;
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int idx = lane + 64 * t;
-      if (idx < n * n) {
-        const int r = idx % n, c = idx / n;
-        Rs[r + c * L] = rr[t];
-        Bs[r + c * L] = rb[t];
-        Ys[r + c * L] = ry[t];
-      }
+  for (int q = NB - 1; q >= 0; --q) {
+    if (q < n) {
+      const double xq = lane_bcast(v * rinv, q);
+      v = (i == q) ? xq : (i < q ? fma(-(double)rrow[q], xq, v) : v);
     }
   }
-  __syncthreads();
-  double v = 0.0;
-  if (lane < n) {
-    double v1 = 0.0;
-    v = (double)rhs[e * n + lane];
-    for (int q = 0; q < n; ++q) {
-      v = fma(-(double)Bs[lane + q * L], xm[q], v);
-      v1 = fma(-(double)Ys[lane + q * L], xp[q], v1);
-    }
-    v += v1;
-  }
-  for (int q = n - 1; q >= 0; --q) {
-    const double xq = __shfl(v, q) / (double)Rs[q + q * L];
-    if (lane == q) v = xq;
-    else if (lane < q) v = fma(-(double)Rs[lane + q * L], xq, v);
-  }
-  if (lane < n) x[oj * n + lane] = v;
+  if (row) x[oj * n + i] = v;
 }
 
 }  // namespace
