@@ -22,6 +22,7 @@ template <class T>
 struct ChainT {
   T *B, *D, *c, *C, *o;
   int32_t mr;
+  int32_t d_diag; /* 1: every D is diagonal */
 };
 template <class T>
 struct FactorsT {
@@ -198,6 +199,10 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  *     phase 2), and every reflector keeps it so: rows <= k, granules <= SEG;
  *     QF_HALF1: rows >= NB / 2 of block 1 are zero in every column and stay
  *     so: granules < NG / 2.
+ *     QF_DIAG0: block 0 of the pivot columns starts diagonal (D_j in phase 2
+ *     at level 0 when every H is the identity and every K is given as 'w'):
+ *     reflector i reaches block 0 only at row i, so u_k's block-0 part is its
+ *     pivot row alone: granule SEG.
  *   QF_DONE: a finished pivot column (piv0 <= column < piv0 + k) is not
  *     updated any more; the reflectors leave its rows up to its pivot as they
  *     are (u is zero there), and the rest of it is not stored.
@@ -205,13 +210,13 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  * bit (their products are exact zeros in every dot product and update), and
  * the step code is specialised per segment, so the skipping costs no branch.
  */
-enum { QF_TRI1 = 1, QF_DONE = 2, QF_HALF0 = 4, QF_HALF1 = 8 };
+enum { QF_TRI1 = 1, QF_DONE = 2, QF_HALF0 = 4, QF_HALF1 = 8, QF_DIAG0 = 16 };
 __host__ __device__ constexpr int qr_gl(int H) { return H <= 16 ? 2 : 4; }
 __host__ __device__ constexpr bool qr_need(int l, int H, int SEG, int QF) {
   const int ng = H / qr_gl(H);
   if (l < H) {
     const int g = l / qr_gl(H);
-    return g >= SEG && (!(QF & QF_HALF0) || g < ng / 2 || g == SEG);
+    return g >= SEG && (!(QF & QF_HALF0) || g < ng / 2 || g == SEG) && (!(QF & QF_DIAG0) || g == SEG);
   }
   const int g = (l - H) / qr_gl(H);
   return (!(QF & QF_TRI1) || g <= SEG) && (!(QF & QF_HALF1) || g < ng / 2);
@@ -543,7 +548,7 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
   }
 }
 
-template <int NB, class T, int WG = 0, int CPL = 1>
+template <int NB, class T, int WG = 0, int CPL = 1, bool DIAG = false>
 __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::MINW))
     k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
   using G = QRGeom<NB, 3 * NB + 1, 2, WG, CPL>;
@@ -591,7 +596,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
     else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
   /* block 1 of the pivot columns is R~_j, upper triangular */
-  house_qr<NB, SL, CPL, T, QF_TRI1 | QF_DONE>(cols, refl, grp, sub, 0, n, upd);
+  /* DIAG: D_j is diagonal (the chain's d_diag) */
+  house_qr<NB, SL, CPL, T, QF_TRI1 | QF_DONE | (DIAG ? QF_DIAG0 : 0)>(cols, refl, grp, sub, 0, n, upd);
   T* fz = fix + (int64_t)p * (nn + n);
 #pragma unroll
   for (int q = 0; q < CPL; ++q) {
@@ -681,7 +687,7 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
 
 /* Phases of one level in bucket NB with launch variant v (NB <= 16: the one
  * variant). HALF is chosen per level: the level's observation rows fill at
- * most half a block. */
+ * most half a block; DIAG: the level's D blocks are diagonal. */
 template <int NB, class T, int WG, int CPL>
 void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
                hipStream_t st, int phase) {
@@ -691,8 +697,9 @@ void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, 
     if (half) k_phase1<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fac, n);
     else k_phase1<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fac, n);
   } else if (phase == 2) {
-    k_phase2<NB, T, WG, CPL><<<dim3((unsigned)lv->evens), dim3(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), 0, st>>>(
-        *lv, in, out, fac, fix, n);
+    const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT);
+    if (in.d_diag) k_phase2<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
+    else k_phase2<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
   } else {
     const dim3 g((unsigned)lv->units), b(QRGeom<NB, NB + 1, 3, WG, CPL>::NT);
     if (half) k_phase3<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fix, n);

@@ -814,6 +814,11 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
  * 2 + 4 L + 2 l. */
 static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
 
+/* The level-0 chain's D = V H is diagonal when every H is the identity and
+ * every K is given as 'w' (V diagonal); the phase-2 kernels then skip D's
+ * exact zeros. */
+static bool level0_d_diagonal(const ukhip_batch* b) { return b->raw.H_ident && !b->raw.K_full; }
+
 static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
                         hipStream_t st) {
   const int nl = (int)b->levels.size();
@@ -823,8 +828,10 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(
   for (int l = 0; l < nl; ++l) {
     uk_chain_t in = b->chain[l & 1];
     in.mr = l == 0 ? b->mr : b->n;
+    in.d_diag = level0_d_diagonal(b) && l == 0;
     uk_chain_t out = b->chain[(l + 1) & 1];
     out.mr = b->n;
+    out.d_diag = 0;
     uk_factors_t f = b->fac;
     const int64_t fb = b->levels[l].fac_base;
     const size_t nn = (size_t)b->n * b->n;
@@ -1131,8 +1138,10 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
   for (int l = 0; l < nl; ++l) {
     uk_chain32_t in = b->chain32[l & 1];
     in.mr = l == 0 ? b->mr : n;
+    in.d_diag = level0_d_diagonal(b) && l == 0;
     uk_chain32_t out = b->chain32[(l + 1) & 1];
     out.mr = n;
+    out.d_diag = 0;
     const uk_factors32_t f = fac32_at(b, l);
     uk_level_t lv = b->levels[l].dev;
     if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;

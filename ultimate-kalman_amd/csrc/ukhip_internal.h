@@ -50,6 +50,7 @@ typedef struct {
   double* C;
   double* o;
   int32_t mr;
+  int32_t d_diag; /* 1: every D is diagonal (level 0 with H = I and all K 'w') */
 } uk_chain_t;
 
 /* The same chain in single precision (the FP32 elimination). */
@@ -60,6 +61,7 @@ typedef struct {
   float* C;
   float* o;
   int32_t mr;
+  int32_t d_diag;
 } uk_chain32_t;
 
 /* Per-level factors kept for the back substitution (one slot per even step):
