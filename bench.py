@@ -86,10 +86,18 @@ def kernel_label(slot, pr, cov=True):
     else:
         cpl, wg = 1, 0
     args = f"{nb}, double, {wg}, {cpl}"
-    if phase != 2:
-        half0 = 2 * max(m_list_of(pr)) <= nb
-        half1 = 2 * n <= nb
-        args += ", " + ("true" if half0 else "false") + ("" if half1 == half0 else "|" + ("true" if half1 else "false"))
+    if phase != 2:  # HALF: the level's observation rows fill at most half a block
+        lv0 = 2 * max(m_list_of(pr)) <= nb
+        above = 2 * n <= nb
+    else:  # DIAG: level 0's D blocks are diagonal (every H the identity, every K 'w')
+        pk = getattr(pr, "packed", None)
+        if pk:
+            lv0 = pk.get("H") is None and pk["K_type"] == "w"
+        else:
+            lv0 = (all(h is None or np.array_equal(h, np.eye(n)) for h in pr.H)
+                   and all(t in (None, "w") for t in pr.K_type))
+        above = False
+    args += ", " + ("true" if lv0 else "false") + ("" if above == lv0 else "|" + ("true" if above else "false"))
     return f"{slot}<{args}>"
 
 

@@ -88,8 +88,25 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(f"{'kernel':40s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'ms/step':>9s} {'%':>6s}")
-        for name, calls, tot, avg, pct in kernel_stats(args.dir):
+        rows = kernel_stats(args.dir)
+        for name, calls, tot, avg, pct in rows:
             print(f"{name[:40]:40s} {calls:6d} {tot / 1e6:10.3f} {avg / 1e6:9.4f} {tot / 1e6 / args.per:9.3f} {pct:6.2f}")
+        # a phase kernel launched with different per-level flags appears once per
+        # flag; its launches together are what a per-kernel HIP-event average covers
+        groups = collections.defaultdict(lambda: [0, 0.0, 0.0, set()])
+        for name, calls, tot, avg, pct in rows:
+            m = re.match(r"^(k_phase\d<\d+, (?:double|float), \d+, \d+)(, (?:true|false))?>$", name)
+            if m and m.group(2):
+                g = groups[m.group(1) + ", *>"]
+                g[0] += calls
+                g[1] += tot
+                g[2] += pct
+                g[3].add(name)
+        multi = {k: v for k, v in groups.items() if len(v[3]) > 1}
+        if multi:
+            print("\nall launches of a kernel over its per-level flags:")
+            for k, (calls, tot, pct, _) in sorted(multi.items(), key=lambda kv: -kv[1][1]):
+                print(f"{k[:40]:40s} {calls:6d} {tot / 1e6:10.3f} {tot / calls / 1e6:9.4f} {tot / 1e6 / args.per:9.3f} {pct:6.2f}")
         return 0
     res, cf, cw = traffic(args.fetch, args.write, args.copy_bytes, args.per)
     alg = dict(x.split("=") for x in args.algorithmic)
