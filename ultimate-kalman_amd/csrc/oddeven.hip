@@ -806,36 +806,68 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_
  * Element-parallel over spw consecutive steps per workgroup: consecutive
  * threads touch consecutive doubles of every array.
  */
-__global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out, int spw) {
+template <bool F32>
+__global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out, uk_chain32_t out32, int spw) {
   const int n = raw.n, mr = raw.mr, nn = n * n, mn = mr * n;
   const int64_t s0 = (int64_t)blockIdx.x * spw;
   const int cnt = (int)((raw.steps - s0) < spw ? (raw.steps - s0) : spw);
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < cnt * nn; idx += 256) {
-    const int ls = idx / nn, e = idx - ls * nn;
-    const int64_t s = s0 + ls;
-    if (raw.first[s]) continue; /* no evolution equation on a chain's first step */
-    const int i = e % n;
-    const int64_t g = s * nn + e;
-    const double kw = raw.K[s * n + i];
-    out.B[g] = -kw * raw.F[raw.Fidx[s] * nn + e];
-    out.D[g] = raw.H_ident ? ((e / n == i) ? kw : 0.0) : kw * raw.H[g];
+  {
+    /* element e = c n + i of step ls; the indices advance by 256 elements
+     * per iteration without a division */
+    const int dls = 256 / nn, de = 256 - dls * nn, dc = de / n, di = de - dc * n;
+    int ls = tid / nn, e = tid - ls * nn, c = e / n, i = e - c * n;
+    for (int idx = tid; idx < cnt * nn; idx += 256) {
+      const int64_t s = s0 + ls;
+      if (!raw.first[s]) { /* no evolution equation on a chain's first step */
+        const int64_t g = s * nn + e;
+        const double kw = raw.K[s * n + i];
+        const double bv = -kw * raw.F[raw.Fidx[s] * nn + e];
+        const double dv = raw.H_ident ? ((c == i) ? kw : 0.0) : kw * raw.H[g];
+        out.B[g] = bv;
+        out.D[g] = dv;
+        if (F32) {
+          out32.B[g] = (float)bv;
+          out32.D[g] = (float)dv;
+        }
+      }
+      ls += dls;
+      e += de;
+      c += dc;
+      i += di;
+      if (i >= n) i -= n, ++c;
+      if (e >= nn) e -= nn, ++ls, c -= n;
+    }
   }
-  for (int idx = tid; idx < cnt * mn; idx += 256) {
-    const int ls = idx / mn, e = idx - ls * mn;
-    const int64_t s = s0 + ls;
-    const int i = e % mr;
-    const int m = raw.m[s];
-    out.C[s * nn + e] = (i < m) ? raw.C[s * mr + i] * raw.G[s * mn + e] : 0.0;
+  {
+    const int dls = 256 / mn, de = 256 - dls * mn, di = de % mr;
+    int ls = tid / mn, e = tid - ls * mn, i = e % mr;
+    for (int idx = tid; idx < cnt * mn; idx += 256) {
+      const int64_t s = s0 + ls;
+      const double v = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.G[s * mn + e] : 0.0;
+      out.C[s * nn + e] = v;
+      if (F32) out32.C[s * nn + e] = (float)v;
+      ls += dls;
+      e += de;
+      i += di;
+      if (i >= mr) i -= mr;
+      if (e >= mn) e -= mn, ++ls;
+    }
   }
   for (int idx = tid; idx < cnt * n; idx += 256) {
     const int64_t s = s0 + idx / n;
-    if (!raw.first[s]) out.c[s0 * n + idx] = raw.K[s0 * n + idx] * raw.c[s0 * n + idx];
+    if (!raw.first[s]) {
+      const double v = raw.K[s0 * n + idx] * raw.c[s0 * n + idx];
+      out.c[s0 * n + idx] = v;
+      if (F32) out32.c[s0 * n + idx] = (float)v;
+    }
   }
   for (int idx = tid; idx < cnt * mr; idx += 256) {
     const int ls = idx / mr, i = idx - ls * mr;
     const int64_t s = s0 + ls;
-    out.o[s * n + i] = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.o[s * mr + i] : 0.0;
+    const double v = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.o[s * mr + i] : 0.0;
+    out.o[s * n + i] = v;
+    if (F32) out32.o[s * n + i] = (float)v;
   }
 }
 
@@ -1020,7 +1052,7 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
     const int nn = raw->n * raw->n;
     const int spw = nn >= 1024 ? 1 : 1024 / nn;
     const int64_t blocks = (raw->steps + spw - 1) / spw;
-    k_weigh_diag<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, spw);
+    k_weigh_diag<false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, uk_chain32_t{}, spw);
     return hipGetLastError();
   }
   const int nb = uk_bucket(raw->n > raw->mr ? raw->n : raw->mr);
@@ -1030,6 +1062,19 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
     default: return hipErrorInvalidValue;
   }
 #undef L_
+  return hipGetLastError();
+}
+
+/* The level-0 chain in FP64 and, written by the same pass, its FP32 copy
+ * (the mixed-precision path) when every K and C is given as 'w'; returns
+ * hipErrorNotSupported otherwise (then the FP64 chain is converted afterwards). */
+extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32,
+                                          hipStream_t st) {
+  if (raw->K_full || raw->C_full) return hipErrorNotSupported;
+  const int nn = raw->n * raw->n;
+  const int spw = nn >= 1024 ? 1 : 1024 / nn;
+  const int64_t blocks = (raw->steps + spw - 1) / spw;
+  k_weigh_diag<true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, *out32, spw);
   return hipGetLastError();
 }
 

@@ -45,6 +45,7 @@ void uk_set_error(const char* fmt, ...);
 hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                  const uk_factors32_t* fac, float* fix, int n, hipStream_t st);
 hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st);
+hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32, hipStream_t st);
 hipError_t uk_launch_add(double* x, const double* d, int64_t count, hipStream_t st);
 hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first, const int32_t* m_obs, const double* x,
                             double* gA, double* gB, double* g, int64_t steps, int n, hipStream_t st);
@@ -1125,11 +1126,15 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
   if (!mixed_workspace(b)) return false;
   const int n = b->n, nl = (int)b->levels.size();
   const size_t nn = (size_t)n * n, S = (size_t)b->steps;
-  if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
-  /* the level-0 chain in FP32 (the FP64 chain stays for the residuals) */
+  /* the level-0 chain in FP32 (the FP64 chain stays for the residuals):
+   * written beside the FP64 one when the whitening is diagonal, else converted */
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
-  if (!hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
+  const hipError_t fused = uk_launch_weigh_f32(&b->raw, &b->chain[0], &c32, st);
+  if (fused != hipErrorNotSupported) {
+    if (!hip_ok(fused, "k_weigh launch")) return false;
+  } else if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch") ||
+      !hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.D, c64.D, S * nn, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.c, c64.c, S * n, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.C, c64.C, S * nn, st), "to32") ||
