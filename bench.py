@@ -70,7 +70,7 @@ def kernel_label(slot, pr, cov=True):
             all_w = pk["K_type"] == "w" and pk["C_type"] == "w"
         else:
             all_w = all(t in (None, "w") for t in pr.K_type) and all(t in (None, "w") for t in pr.C_type)
-        return "k_weigh_diag" if all_w else f"k_weigh<{bucket(max(n, max(m_list_of(pr))))}>"
+        return "k_weigh_diag<false>" if all_w else f"k_weigh<{bucket(max(n, max(m_list_of(pr))))}>"
     if slot == "k_backsub":
         if nb >= 32:
             return f"k_backsub_mfma<{nb}>" if cov else f"k_estimate<{nb}>"
