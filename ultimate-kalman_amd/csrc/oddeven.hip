@@ -1082,20 +1082,20 @@ extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t*
  * elimination of the mixed-precision path), per precision, phase and bucket
  * (32, 48, 64): the columns each lane group holds (cpl) and the workgroups per
  * CU asked of the register allocator (wg). Starts from variants measured
- * fastest on MI355X (n = 48 and 64: k = 200000; n = 32: k = 16384), rewritten
- * by ukhip_autotune. */
+ * fastest on MI355X (n = 24, 32, 48 and 64, k = 200000), rewritten by
+ * ukhip_autotune. */
 struct QrLaunch {
   int cpl, wg;
 };
 static QrLaunch g_qr[2][3][3] = {
     /* FP64:  bucket 32   48        64 */
-    {/* phase 1 */ {{1, 4}, {2, 2}, {1, 1}},
-     /* phase 2 */ {{1, 4}, {1, 1}, {1, 1}},
+    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 1}},
+     /* phase 2 */ {{2, 2}, {1, 1}, {1, 1}},
      /* phase 3 */ {{1, 2}, {2, 2}, {2, 3}}},
     /* FP32 */
-    {/* phase 1 */ {{1, 1}, {2, 2}, {1, 4}},
-     /* phase 2 */ {{1, 1}, {1, 4}, {2, 2}},
-     /* phase 3 */ {{1, 1}, {1, 1}, {2, 2}}}};
+    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 4}},
+     /* phase 2 */ {{2, 3}, {1, 4}, {2, 2}},
+     /* phase 3 */ {{1, 2}, {1, 1}, {2, 2}}}};
 static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
 /* the compiled variants: one column per group at 1, 2 or 4 workgroups per CU,
  * two columns per group at 2, 3 or 4 */
