@@ -190,3 +190,21 @@ def test_kalman_t_api(dev, oracle):
         L.matrix_free(cvm)
         assert rel_err(x, ox[i]) < TOL and rel_err(cv, oc[i]) < TOL, i
     L.kalman_free(kal)
+
+
+@pytest.mark.parametrize("H_kind", ["I", "near-I"])
+def test_diagonal_weights_with_and_without_identity_H(dev, oracle, H_kind):
+    """Every K and C given as 'w' at n = 40 (bucket 48), m = 20: with H = I the
+    level-0 D blocks are diagonal and phase 2 skips their zeros; with a dense
+    H (the identity plus a small random part, well conditioned) they are dense
+    and it must not. Both against the oracle."""
+    _oracle_threads(oracle)
+    pr = P.random_problem(91, 65, 40, 20, K_kind="w", C_kind="w", H_kind="I", unobserved=(7, 30))
+    if H_kind != "I":
+        rng = P.rng_for(94)
+        for i in range(1, pr.k):
+            pr.H[i] = np.eye(40) + 0.05 * rng.standard_normal((40, 40))
+    say(f"n=40 m=20 'w' covariances, H={H_kind}")
+    xs, cs = dev.smooth_equations(pr, "oddeven")
+    ox, oc = oracle.oddeven(pr)
+    assert max_rel_err(xs, ox) < TOL and max_rel_err(cs, oc) < TOL
