@@ -86,9 +86,8 @@ def kernel_label(slot, pr, cov=True):
     else:
         cpl, wg = 1, 0
     args = f"{nb}, double, {wg}, {cpl}"
-    if phase != 2:  # HALF: the level's observation rows fill at most half a block
-        lv0 = 2 * max(m_list_of(pr)) <= nb
-        above = 2 * n <= nb
+    if phase != 2:  # the level's C blocks: 1 at most half a block of rows (level 0), 2 triangular (above)
+        args += f", {1 if 2 * max(m_list_of(pr)) <= nb else 0}|2"
     else:  # DIAG: level 0's D blocks are diagonal (every H the identity, every K 'w')
         pk = getattr(pr, "packed", None)
         if pk:
@@ -96,8 +95,7 @@ def kernel_label(slot, pr, cov=True):
         else:
             lv0 = (all(h is None or np.array_equal(h, np.eye(n)) for h in pr.H)
                    and all(t in (None, "w") for t in pr.K_type))
-        above = False
-    args += ", " + ("true" if lv0 else "false") + ("" if above == lv0 else "|" + ("true" if above else "false"))
+        args += ", true|false" if lv0 else ", false"
     return f"{slot}<{args}>"
 
 

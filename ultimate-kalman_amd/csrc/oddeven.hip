@@ -1159,8 +1159,8 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
 extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                           const uk_factors_t* fac, double* fix, int n, hipStream_t st,
                                           hipEvent_t* ev) {
-  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag};
-  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag};
+  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri};
+  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri};
   const FactorsT<double> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<double>(lv, ci, co, f, fix, n, st, ev);
 }
@@ -1168,8 +1168,8 @@ extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t
 /* The same elimination in single precision (chains, factors and arithmetic). */
 extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                             const uk_factors32_t* fac, float* fix, int n, hipStream_t st) {
-  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag};
-  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag};
+  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri};
+  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri};
   const FactorsT<float> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<float>(lv, ci, co, f, fix, n, st, nullptr);
 }

@@ -23,6 +23,7 @@ struct ChainT {
   T *B, *D, *c, *C, *o;
   int32_t mr;
   int32_t d_diag; /* 1: every D is diagonal */
+  int32_t c_tri;  /* 1: every C is upper triangular n x n */
 };
 template <class T>
 struct FactorsT {
@@ -196,7 +197,8 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  *     < NG / 2, or SEG.
  *   block 1: all rows;
  *     QF_TRI1: block 1 starts upper triangular in the pivot columns (R~ in
- *     phase 2), and every reflector keeps it so: rows <= k, granules <= SEG;
+ *     phase 2; C = G~ of the levels above 0 in phases 1 and 3), and every
+ *     reflector keeps it so: rows <= k, granules <= SEG;
  *     QF_HALF1: rows >= NB / 2 of block 1 are zero in every column and stay
  *     so: granules < NG / 2.
  *     QF_DIAG0: block 0 of the pivot columns starts diagonal (D_j in phase 2
@@ -476,7 +478,9 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
-template <int NB, class T, int WG = 0, int CPL = 1, bool HALF = false>
+/* LV: the level's stacked rows (C) as structure: 0 dense, 1 at most NB / 2
+ * rows (level 0 with m <= NB / 2), 2 upper triangular (the levels above 0) */
+template <int NB, class T, int WG = 0, int CPL = 1, int LV = 0>
 __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
   using G = QRGeom<NB, 2 * NB + 1, 1, WG, CPL>;
@@ -522,8 +526,10 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
     if (isP || isQ || isR1) upd |= 1u << q;
     else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  /* HALF: the stacked rows C_j, o_j fill at most NB / 2 rows */
-  house_qr<NB, SL, CPL, T, QF_DONE | (HALF ? QF_HALF1 : 0)>(cols, refl, grp, sub, 0, n, upd);
+  /* LV 1: the stacked rows C_j, o_j fill at most NB / 2 rows; LV 2: C_j is
+   * upper triangular (the pivot columns' block 1) */
+  house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp, sub, 0, n,
+                                                                                        upd);
 #pragma unroll
   for (int q = 0; q < CPL; ++q) {
     const Col<NB, SL, T>& col = cols[q];
@@ -621,7 +627,7 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
   }
 }
 
-template <int NB, class T, int WG = 0, int CPL = 1, bool HALF = false>
+template <int NB, class T, int WG = 0, int CPL = 1, int LV = 0>
 __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<NB, NB + 1, 3, WG, CPL>::MINW))
     k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
   using G = QRGeom<NB, NB + 1, 3, WG, CPL>;
@@ -662,9 +668,11 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
     if (isQ || isR1) upd |= 1u << q;
     else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  /* HALF: both blocks (D~_{j+1}, C_{j+1} and the right-hand sides) fill at
-   * most NB / 2 rows */
-  house_qr<NB, SL, CPL, T, QF_DONE | (HALF ? QF_HALF0 | QF_HALF1 : 0)>(cols, refl, grp, sub, 0, n, upd);
+  /* LV 1: both blocks (D~_{j+1}, C_{j+1} and the right-hand sides) fill at
+   * most NB / 2 rows; LV 2: C_{j+1} (block 1 of the pivot columns) is upper
+   * triangular */
+  house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF0 | QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp,
+                                                                                                  sub, 0, n, upd);
   if (do_fix) {
     const T* fz = fix + (int64_t)p * (nn + n);
 #pragma unroll
@@ -686,24 +694,27 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
 }
 
 /* Phases of one level in bucket NB with launch variant v (NB <= 16: the one
- * variant). HALF is chosen per level: the level's observation rows fill at
- * most half a block; DIAG: the level's D blocks are diagonal. */
+ * variant). Per level: the structure of its C blocks (LV of phases 1 and 3:
+ * at most half a block of rows at level 0, upper triangular above), and DIAG:
+ * its D blocks are diagonal. */
 template <int NB, class T, int WG, int CPL>
 void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
                hipStream_t st, int phase) {
-  const bool half = 2 * in.mr <= NB;
+  const int lvk = in.c_tri ? 2 : (2 * in.mr <= NB ? 1 : 0);
   if (phase == 1) {
     const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT);
-    if (half) k_phase1<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fac, n);
-    else k_phase1<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fac, n);
+    if (lvk == 2) k_phase1<NB, T, WG, CPL, 2><<<g, b, 0, st>>>(*lv, in, out, fac, n);
+    else if (lvk == 1) k_phase1<NB, T, WG, CPL, 1><<<g, b, 0, st>>>(*lv, in, out, fac, n);
+    else k_phase1<NB, T, WG, CPL, 0><<<g, b, 0, st>>>(*lv, in, out, fac, n);
   } else if (phase == 2) {
     const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT);
     if (in.d_diag) k_phase2<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
     else k_phase2<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
   } else {
     const dim3 g((unsigned)lv->units), b(QRGeom<NB, NB + 1, 3, WG, CPL>::NT);
-    if (half) k_phase3<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fix, n);
-    else k_phase3<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fix, n);
+    if (lvk == 2) k_phase3<NB, T, WG, CPL, 2><<<g, b, 0, st>>>(*lv, in, out, fix, n);
+    else if (lvk == 1) k_phase3<NB, T, WG, CPL, 1><<<g, b, 0, st>>>(*lv, in, out, fix, n);
+    else k_phase3<NB, T, WG, CPL, 0><<<g, b, 0, st>>>(*lv, in, out, fix, n);
   }
 }
 

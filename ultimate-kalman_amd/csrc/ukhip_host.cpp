@@ -830,9 +830,11 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(
     uk_chain_t in = b->chain[l & 1];
     in.mr = l == 0 ? b->mr : b->n;
     in.d_diag = level0_d_diagonal(b) && l == 0;
+    in.c_tri = l > 0;
     uk_chain_t out = b->chain[(l + 1) & 1];
     out.mr = b->n;
     out.d_diag = 0;
+    out.c_tri = 0;
     uk_factors_t f = b->fac;
     const int64_t fb = b->levels[l].fac_base;
     const size_t nn = (size_t)b->n * b->n;
@@ -1144,9 +1146,11 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
     uk_chain32_t in = b->chain32[l & 1];
     in.mr = l == 0 ? b->mr : n;
     in.d_diag = level0_d_diagonal(b) && l == 0;
+    in.c_tri = l > 0;
     uk_chain32_t out = b->chain32[(l + 1) & 1];
     out.mr = n;
     out.d_diag = 0;
+    out.c_tri = 0;
     const uk_factors32_t f = fac32_at(b, l);
     uk_level_t lv = b->levels[l].dev;
     if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;

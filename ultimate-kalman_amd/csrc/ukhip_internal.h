@@ -51,6 +51,7 @@ typedef struct {
   double* o;
   int32_t mr;
   int32_t d_diag; /* 1: every D is diagonal (level 0 with H = I and all K 'w') */
+  int32_t c_tri;  /* 1: every C is upper triangular n x n (the levels above 0: phase 3's G~) */
 } uk_chain_t;
 
 /* The same chain in single precision (the FP32 elimination). */
@@ -62,6 +63,7 @@ typedef struct {
   float* o;
   int32_t mr;
   int32_t d_diag;
+  int32_t c_tri;
 } uk_chain32_t;
 
 /* Per-level factors kept for the back substitution (one slot per even step):

@@ -95,7 +95,7 @@ def main(argv=None):
         # flag; its launches together are what a per-kernel HIP-event average covers
         groups = collections.defaultdict(lambda: [0, 0.0, 0.0, set()])
         for name, calls, tot, avg, pct in rows:
-            m = re.match(r"^(k_phase\d<\d+, (?:double|float), \d+, \d+)(, (?:true|false))?>$", name)
+            m = re.match(r"^(k_phase\d<\d+, (?:double|float), \d+, \d+)(, (?:true|false|\d))?>$", name)
             if m and m.group(2):
                 g = groups[m.group(1) + ", *>"]
                 g[0] += calls
