@@ -823,12 +823,17 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
         const int64_t g = s * nn + e;
         const double kw = raw.K[s * n + i];
         const double bv = -kw * raw.F[raw.Fidx[s] * nn + e];
-        const double dv = raw.H_ident ? ((c == i) ? kw : 0.0) : kw * raw.H[g];
         out.B[g] = bv;
-        out.D[g] = dv;
-        if (F32) {
-          out32.B[g] = (float)bv;
-          out32.D[g] = (float)dv;
+        if (F32) out32.B[g] = (float)bv;
+        if (out.d_diag) { /* D = diag(k): its diagonal in the slot's first n entries (then c == 0, i == e) */
+          if (e < n) {
+            out.D[g] = kw;
+            if (F32) out32.D[g] = (float)kw;
+          }
+        } else {
+          const double dv = raw.H_ident ? ((c == i) ? kw : 0.0) : kw * raw.H[g];
+          out.D[g] = dv;
+          if (F32) out32.D[g] = (float)dv;
         }
       }
       ls += dls;

@@ -22,7 +22,7 @@ template <class T>
 struct ChainT {
   T *B, *D, *c, *C, *o;
   int32_t mr;
-  int32_t d_diag; /* 1: every D is diagonal */
+  int32_t d_diag; /* 1: every D is diagonal, stored as its diagonal (the first n entries of its slot) */
   int32_t c_tri;  /* 1: every C is upper triangular n x n */
 };
 template <class T>
@@ -124,6 +124,12 @@ struct Col {
       const int r = SL * l + sub;
       if (r < rows) dst[r] = a[blk * H + l];
     }
+  }
+  /* block blk <- column col of a diagonal matrix given by its diagonal d */
+  __device__ __forceinline__ void load_diag(int blk, const T* __restrict__ d, int col, int sub) {
+    const T v = d[col];
+#pragma unroll
+    for (int l = 0; l < H; ++l) a[blk * H + l] = (SL * l + sub == col) ? v : T(0);
   }
   /* upper-triangular store of block 0: rows > col are written as zero */
   __device__ __forceinline__ void store_triu(T* __restrict__ dst, int rows, int col, int sub) const {
@@ -515,7 +521,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
       col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
     }
     if (isQ) {
-      col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
+      if (in.d_diag) col.load_diag(0, in.D + sp * nn, qc, sub); /* stored as its diagonal */
+      else col.load_block(0, in.D + sp * nn + (int64_t)qc * n, n, sub);
       col.zero_block(1);
     }
     if (isR1) {
@@ -583,7 +590,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
     const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
     const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
     if (isP) {
-      col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
+      if (DIAG) col.load_diag(0, in.D + sj * nn, qc, sub); /* stored as its diagonal */
+      else col.load_block(0, in.D + sj * nn + (int64_t)qc * n, n, sub);
       col.load_block(1, fac.R + e * nn + (int64_t)qc * n, n, sub); /* R~_j */
     }
     if (isQ) {

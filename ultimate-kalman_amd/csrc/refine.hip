@@ -88,9 +88,10 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
     xm[i] = evo ? x[(s - 1) * n + i] : 0.0;
   }
   if (evo) {
+    /* a chain with d_diag holds D as its diagonal (the slot's first n entries) */
     for (int idx = tid; idx < n * n; idx += 256) {
       const int i = idx % n, j = idx / n;
-      M1[i + j * L] = ch.D[s * nn + idx];
+      M1[i + j * L] = ch.d_diag ? (i == j ? ch.D[s * nn + i] : 0.0) : ch.D[s * nn + idx];
       M2[i + j * L] = ch.B[s * nn + idx];
     }
   }

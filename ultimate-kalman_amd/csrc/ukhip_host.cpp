@@ -816,15 +816,18 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
 static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
 
 /* The level-0 chain's D = V H is diagonal when every H is the identity and
- * every K is given as 'w' (V diagonal); the phase-2 kernels then skip D's
- * exact zeros. */
-static bool level0_d_diagonal(const ukhip_batch* b) { return b->raw.H_ident && !b->raw.K_full; }
+ * every K is given as 'w' (V diagonal). When C is 'w' too, k_weigh_diag writes
+ * the chain and stores D as its diagonal; the phase kernels and k_normal read
+ * it so, and phase 2 skips D's exact zeros. */
+static bool level0_d_diagonal(const ukhip_batch* b) { return b->raw.H_ident && !b->raw.K_full && !b->raw.C_full; }
 
 static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
                         hipStream_t st) {
   const int nl = (int)b->levels.size();
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
-  if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch")) return false;
+  uk_chain_t w0 = b->chain[0];
+  w0.d_diag = level0_d_diagonal(b);
+  if (!hip_ok(uk_launch_weigh(&b->raw, &w0, st), "k_weigh launch")) return false;
   if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
   for (int l = 0; l < nl; ++l) {
     uk_chain_t in = b->chain[l & 1];
@@ -1132,7 +1135,9 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
    * written beside the FP64 one when the whitening is diagonal, else converted */
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
-  const hipError_t fused = uk_launch_weigh_f32(&b->raw, &b->chain[0], &c32, st);
+  uk_chain_t w0 = b->chain[0];
+  w0.d_diag = level0_d_diagonal(b); /* implies the fused path below */
+  const hipError_t fused = uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
   if (fused != hipErrorNotSupported) {
     if (!hip_ok(fused, "k_weigh launch")) return false;
   } else if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch") ||
@@ -1162,6 +1167,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
     if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
   }
   uk_chain_t r64 = b->chain[0];
+  r64.d_diag = level0_d_diagonal(b);
   r64.mr = b->mr;
   for (int it = 0; it < refinements; ++it) {
     if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->raw.m, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),

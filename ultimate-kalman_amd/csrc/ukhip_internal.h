@@ -50,7 +50,8 @@ typedef struct {
   double* C;
   double* o;
   int32_t mr;
-  int32_t d_diag; /* 1: every D is diagonal (level 0 with H = I and all K 'w') */
+  int32_t d_diag; /* 1: every D is diagonal and stored as its diagonal, the first n entries of its
+                     slot (level 0 with H = I and every K and C given as 'w') */
   int32_t c_tri;  /* 1: every C is upper triangular n x n (the levels above 0: phase 3's G~) */
 } uk_chain_t;
 
