@@ -822,9 +822,11 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
       if (!raw.first[s]) { /* no evolution equation on a chain's first step */
         const int64_t g = s * nn + e;
         const double kw = raw.K[s * n + i];
-        const double bv = -kw * raw.F[raw.Fidx[s] * nn + e];
-        out.B[g] = bv;
-        if (F32) out32.B[g] = (float)bv;
+        if (!out.b_synth) { /* else B is formed where it is read */
+          const double bv = -kw * raw.F[raw.Fidx[s] * nn + e];
+          out.B[g] = bv;
+          if (F32) out32.B[g] = (float)bv;
+        }
         if (out.d_diag) { /* D = diag(k): its diagonal in the slot's first n entries (then c == 0, i == e) */
           if (e < n) {
             out.D[g] = kw;
@@ -1164,8 +1166,8 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
 extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                           const uk_factors_t* fac, double* fix, int n, hipStream_t st,
                                           hipEvent_t* ev) {
-  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri};
-  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri};
+  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri, in->b_synth, in->wk, in->wF, in->wFidx};
+  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri, out->b_synth, out->wk, out->wF, out->wFidx};
   const FactorsT<double> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<double>(lv, ci, co, f, fix, n, st, ev);
 }
@@ -1173,8 +1175,8 @@ extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t
 /* The same elimination in single precision (chains, factors and arithmetic). */
 extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                             const uk_factors32_t* fac, float* fix, int n, hipStream_t st) {
-  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri};
-  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri};
+  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri, in->b_synth, in->wk, in->wF, in->wFidx};
+  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri, out->b_synth, out->wk, out->wF, out->wFidx};
   const FactorsT<float> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<float>(lv, ci, co, f, fix, n, st, nullptr);
 }

@@ -24,6 +24,10 @@ struct ChainT {
   int32_t mr;
   int32_t d_diag; /* 1: every D is diagonal, stored as its diagonal (the first n entries of its slot) */
   int32_t c_tri;  /* 1: every C is upper triangular n x n */
+  int32_t b_synth; /* 1: B_s = -diag(wk_s) wF[wFidx[s]] (wFidx NULL: block 0), formed on load */
+  const double* wk;
+  const double* wF;
+  const int64_t* wFidx;
 };
 template <class T>
 struct FactorsT {
@@ -123,6 +127,16 @@ struct Col {
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
       if (r < rows) dst[r] = a[blk * H + l];
+    }
+  }
+  /* block blk <- column col of B = -diag(k) F (n x n, F column-major), the
+   * whitened evolution block formed as the whitening pass forms it */
+  __device__ __forceinline__ void load_bsynth(int blk, const double* __restrict__ k, const double* __restrict__ F,
+                                              int col, int n, int sub) {
+#pragma unroll
+    for (int l = 0; l < H; ++l) {
+      const int r = SL * l + sub;
+      a[blk * H + l] = (r < n) ? T(-k[r] * F[r + (int64_t)col * n]) : T(0);
     }
   }
   /* block blk <- column col of a diagonal matrix given by its diagonal d */
@@ -517,6 +531,7 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
     const int qc = isP ? c : (isQ ? c - n : 0);
     if (isP) {
       if (lone) col.zero_block(0);
+      else if (in.b_synth) col.load_bsynth(0, in.wk + sp * n, in.wF + (in.wFidx ? in.wFidx[sp] * nn : 0), qc, n, sub);
       else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
       col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
     }
@@ -603,7 +618,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
       col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
     }
     if (isBl) {
-      col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
+      if (in.b_synth) col.load_bsynth(0, in.wk + sj * n, in.wF + (in.wFidx ? in.wFidx[sj] * nn : 0), qc, n, sub);
+      else col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
       col.zero_block(1);
     }
     if (isP || isQ || isR1 || isBl) upd |= 1u << q;

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
     for (int idx = tid; idx < n * n; idx += 256) {
       const int i = idx % n, j = idx / n;
       M1[i + j * L] = ch.d_diag ? (i == j ? ch.D[s * nn + i] : 0.0) : ch.D[s * nn + idx];
-      M2[i + j * L] = ch.B[s * nn + idx];
+      M2[i + j * L] = ch.b_synth ? -ch.wk[s * n + i] * ch.wF[(ch.wFidx ? ch.wFidx[s] * nn : 0) + idx] : ch.B[s * nn + idx];
     }
   }
   __syncthreads();

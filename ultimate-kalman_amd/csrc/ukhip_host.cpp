@@ -173,6 +173,7 @@ struct DevArena {
 };
 
 struct ukhip_batch {
+  int64_t nF = 0; /* F blocks in raw.F (1: every step shares one) */
   int n = 0, mr = 0, nprob = 0;
   int64_t steps = 0;
   std::vector<int64_t> k, orig_start;
@@ -439,6 +440,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->raw.C_full = st.C_full;
   b->raw.F = (const double*)(r + st.offF);
   b->raw.Fidx = (const int64_t*)(r + st.offFidx);
+  b->nF = st.nF;
   b->raw.H = st.H_ident ? nullptr : (const double*)(r + st.offH);
   b->raw.c = (const double*)(r + st.offc);
   b->raw.K = (const double*)(r + st.offK);
@@ -821,23 +823,31 @@ static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
  * it so, and phase 2 skips D's exact zeros. */
 static bool level0_d_diagonal(const ukhip_batch* b) { return b->raw.H_ident && !b->raw.K_full && !b->raw.C_full; }
 
+/* With every K and C given as 'w', the level-0 chain's B_s = -diag(k_s) F is
+ * not stored: its readers form it from the raw K and F. */
+template <class Chain>
+static Chain level0_chain(const ukhip_batch* b, Chain c) {
+  c.d_diag = level0_d_diagonal(b);
+  c.b_synth = !b->raw.K_full && !b->raw.C_full;
+  c.wk = b->raw.K;
+  c.wF = b->raw.F;
+  c.wFidx = b->nF == 1 ? nullptr : b->raw.Fidx; /* one F block: no per-step lookup */
+  return c;
+}
+
 static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
                         hipStream_t st) {
   const int nl = (int)b->levels.size();
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
-  uk_chain_t w0 = b->chain[0];
-  w0.d_diag = level0_d_diagonal(b);
+  const uk_chain_t w0 = level0_chain(b, b->chain[0]);
   if (!hip_ok(uk_launch_weigh(&b->raw, &w0, st), "k_weigh launch")) return false;
   if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
   for (int l = 0; l < nl; ++l) {
-    uk_chain_t in = b->chain[l & 1];
+    uk_chain_t in = l == 0 ? level0_chain(b, b->chain[0]) : b->chain[l & 1];
     in.mr = l == 0 ? b->mr : b->n;
-    in.d_diag = level0_d_diagonal(b) && l == 0;
     in.c_tri = l > 0;
-    uk_chain_t out = b->chain[(l + 1) & 1];
+    uk_chain_t out = b->chain[(l + 1) & 1]; /* its flags stay 0: a dense chain */
     out.mr = b->n;
-    out.d_diag = 0;
-    out.c_tri = 0;
     uk_factors_t f = b->fac;
     const int64_t fb = b->levels[l].fac_base;
     const size_t nn = (size_t)b->n * b->n;
@@ -1135,8 +1145,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
    * written beside the FP64 one when the whitening is diagonal, else converted */
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
-  uk_chain_t w0 = b->chain[0];
-  w0.d_diag = level0_d_diagonal(b); /* implies the fused path below */
+  const uk_chain_t w0 = level0_chain(b, b->chain[0]); /* its flags imply the fused path below */
   const hipError_t fused = uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
   if (fused != hipErrorNotSupported) {
     if (!hip_ok(fused, "k_weigh launch")) return false;
@@ -1148,14 +1157,11 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
       !hip_ok(uk_launch_to32(c32.o, c64.o, S * n, st), "to32"))
     return false;
   for (int l = 0; l < nl; ++l) {
-    uk_chain32_t in = b->chain32[l & 1];
+    uk_chain32_t in = l == 0 ? level0_chain(b, b->chain32[0]) : b->chain32[l & 1];
     in.mr = l == 0 ? b->mr : n;
-    in.d_diag = level0_d_diagonal(b) && l == 0;
     in.c_tri = l > 0;
-    uk_chain32_t out = b->chain32[(l + 1) & 1];
+    uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
     out.mr = n;
-    out.d_diag = 0;
-    out.c_tri = 0;
     const uk_factors32_t f = fac32_at(b, l);
     uk_level_t lv = b->levels[l].dev;
     if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;
@@ -1166,8 +1172,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
     uk_level_t lv = b->levels[l].dev;
     if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
   }
-  uk_chain_t r64 = b->chain[0];
-  r64.d_diag = level0_d_diagonal(b);
+  uk_chain_t r64 = level0_chain(b, b->chain[0]);
   r64.mr = b->mr;
   for (int it = 0; it < refinements; ++it) {
     if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->raw.m, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),

@@ -53,6 +53,13 @@ typedef struct {
   int32_t d_diag; /* 1: every D is diagonal and stored as its diagonal, the first n entries of its
                      slot (level 0 with H = I and every K and C given as 'w') */
   int32_t c_tri;  /* 1: every C is upper triangular n x n (the levels above 0: phase 3's G~) */
+  /* b_synth = 1: B is not stored; B_s = -diag(wk_s) wF[wFidx[s]] (wFidx NULL:
+   * every step uses block 0) is formed on load (level 0 with every K and C
+   * given as 'w': wk = raw K, wF = raw F) */
+  int32_t b_synth;
+  const double* wk;
+  const double* wF;
+  const int64_t* wFidx;
 } uk_chain_t;
 
 /* The same chain in single precision (the FP32 elimination). */
@@ -65,6 +72,10 @@ typedef struct {
   int32_t mr;
   int32_t d_diag;
   int32_t c_tri;
+  int32_t b_synth;
+  const double* wk;
+  const double* wF;
+  const int64_t* wFidx;
 } uk_chain32_t;
 
 /* Per-level factors kept for the back substitution (one slot per even step):
