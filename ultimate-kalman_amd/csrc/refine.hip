@@ -70,13 +70,18 @@ __global__ void __launch_bounds__(256) k_add(double* __restrict__ x, const doubl
  * Blocks are staged through LDS (odd leading dimension) so that HBM is read
  * along columns and the transposed products read LDS.
  */
-template <int NB>
+template <int NB, bool DD>
 __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* first, const int32_t* m_obs,
                                                 const double* __restrict__ x, double* __restrict__ gA,
                                                 double* __restrict__ gB, int n) {
   constexpr int L = NB + 1;
-  __shared__ double M1[NB * L], M2[NB * L];
-  __shared__ double xs[NB], xm[NB], re[NB], ro[NB];
+  /* DD (the chain stores D as its diagonal): one block, B and then C; the
+   * diagonal's zeros are formed in the products, so the arithmetic is the
+   * same as with D staged */
+  __shared__ double M2[NB * L];
+  __shared__ double M1s[DD ? 1 : NB * L];
+  double* const M1 = DD ? M2 : M1s;
+  __shared__ double xs[NB], xm[NB], re[NB], ro[NB], dd[NB];
   const int tid = threadIdx.x;
   const int64_t s = blockIdx.x;
   const int64_t nn = (int64_t)n * n;
@@ -88,19 +93,22 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
     xm[i] = evo ? x[(s - 1) * n + i] : 0.0;
   }
   if (evo) {
-    /* a chain with d_diag holds D as its diagonal (the slot's first n entries) */
     for (int idx = tid; idx < n * n; idx += 256) {
       const int i = idx % n, j = idx / n;
-      M1[i + j * L] = ch.d_diag ? (i == j ? ch.D[s * nn + i] : 0.0) : ch.D[s * nn + idx];
+      if (!DD) M1[i + j * L] = ch.D[s * nn + idx];
       M2[i + j * L] = ch.b_synth ? -ch.wk[s * n + i] * ch.wF[(ch.wFidx ? ch.wFidx[s] * nn : 0) + idx] : ch.B[s * nn + idx];
     }
+    if (DD)
+      for (int i = tid; i < n; i += 256) dd[i] = ch.D[s * nn + i];
   }
   __syncthreads();
+  /* D's entry (i, j): from the staged block, or the diagonal */
+  auto Dij = [&](int i, int j) -> double { return DD ? (i == j ? dd[i] : 0.0) : M1[i + j * L]; };
   if (tid < n) {
     double r = 0.0;
     if (evo) {
       r = ch.c[s * n + tid];
-      for (int j = 0; j < n; ++j) r -= M1[tid + j * L] * xs[j] + M2[tid + j * L] * xm[j];
+      for (int j = 0; j < n; ++j) r -= Dij(tid, j) * xs[j] + M2[tid + j * L] * xm[j];
     }
     re[tid] = r;
   }
@@ -108,10 +116,10 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
   double ga = 0.0, gb = 0.0;
   if (tid < n && evo)
     for (int r = 0; r < n; ++r) {
-      ga = fma(M1[r + tid * L], re[r], ga);
+      ga = fma(Dij(r, tid), re[r], ga);
       gb = fma(M2[r + tid * L], re[r], gb);
     }
-  __syncthreads();
+  __syncthreads(); /* B consumed: with DD its block takes C */
   if (m > 0) {
     for (int idx = tid; idx < mr * n; idx += 256) {
       const int i = idx % mr, j = idx / mr;
@@ -328,7 +336,9 @@ extern "C" hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first
                                        const double* x, double* gA, double* gB, double* g, int64_t steps, int n,
                                        hipStream_t st) {
   const int nb = uk_bucket(n > ch->mr ? n : ch->mr);
-#define L_(NBV) k_normal<NBV><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n)
+#define L_(NBV)                                                                                       \
+  if (ch->d_diag) k_normal<NBV, true><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n); \
+  else k_normal<NBV, false><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
