@@ -283,17 +283,24 @@ __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> f
   const double xm = (has_m && row) ? x[om * n + i] : 0.0;
   const double xp = (has_p && row) ? x[op * n + i] : 0.0;
   double v = row ? (double)rhs[e * n + i] : 0.0, v1 = 0.0;
+  /* The column loads take no branch: row and column indices are clamped into
+   * the step's n x n block (every even step of the level has its R, B~ and Y
+   * slots) and the values outside it are replaced by zeros, so the compiler
+   * issues every load of the loop before the first wait. A replaced term adds
+   * -0 * 0 to v, which leaves v as it is. */
+  const int ic = row ? i : 0;
+  const bool lm = has_m && row, lp = has_p && row;
   T rrow[NB];
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    rrow[q] = T(0);
-    if (q < n) {
-      const double b = (has_m && row) ? (double)Bg[i + (int64_t)q * n] : 0.0;
-      const double yv = (has_p && row) ? (double)Yg[i + (int64_t)q * n] : 0.0;
-      if (row) rrow[q] = Rg[i + (int64_t)q * n];
-      v = fma(-b, lane_bcast(xm, q), v);
-      v1 = fma(-yv, lane_bcast(xp, q), v1);
-    }
+    const bool col = q < n;
+    const int64_t idx = ic + (int64_t)(col ? q : n - 1) * n;
+    const T rv = Rg[idx], bv = Bg[idx], yw = Yg[idx];
+    rrow[q] = (row && col) ? rv : T(0);
+    const double b = (lm && col) ? (double)bv : 0.0;
+    const double yv = (lp && col) ? (double)yw : 0.0;
+    v = fma(-b, lane_bcast(xm, q), v);
+    v1 = fma(-yv, lane_bcast(xp, q), v1);
   }
   v += v1;
   const double rinv = row ? 1.0 / (double)Rg[i + (int64_t)i * n] : 0.0;
