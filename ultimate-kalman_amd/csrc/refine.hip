@@ -88,18 +88,59 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
   const int mr = ch.mr;
   const bool evo = first[s] == 0;
   const int m = m_obs[s];
-  for (int i = tid; i < n; i += 256) {
-    xs[i] = x[s * n + i];
-    xm[i] = evo ? x[(s - 1) * n + i] : 0.0;
+  /* Every block element of the step is read into registers first, at indices
+   * clamped into the step's slots (an element outside the block is read but
+   * never stored), so the loads go out together instead of one loop
+   * iteration's latency after another. C is read before it is staged. */
+  constexpr int PS = (NB * NB + 255) / 256;
+  const int nb2 = n * n, cn = mr * n;
+  double tcv[PS], tbv[PS], tdv[DD ? 1 : PS];
+#pragma unroll
+  for (int t = 0; t < PS; ++t) {
+    const int idx = tid + 256 * t;
+    tcv[t] = ch.C[s * nn + (idx < cn ? idx : 0)];
+  }
+  double ddv = 0.0;
+  if (evo) {
+    if (ch.b_synth) {
+      const double* wFs = ch.wF + (ch.wFidx ? ch.wFidx[s] * nn : 0);
+#pragma unroll
+      for (int t = 0; t < PS; ++t) {
+        const int idx = tid + 256 * t, ic = idx < nb2 ? idx : 0;
+        tbv[t] = -ch.wk[s * n + ic % n] * wFs[ic];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < PS; ++t) {
+        const int idx = tid + 256 * t;
+        tbv[t] = ch.B[s * nn + (idx < nb2 ? idx : 0)];
+      }
+    }
+    if (DD) {
+      ddv = ch.D[s * nn + (tid < n ? tid : 0)];
+    } else {
+#pragma unroll
+      for (int t = 0; t < (DD ? 1 : PS); ++t) {
+        const int idx = tid + 256 * t;
+        tdv[t] = ch.D[s * nn + (idx < nb2 ? idx : 0)];
+      }
+    }
+  }
+  if (tid < n) {
+    xs[tid] = x[s * n + tid];
+    xm[tid] = evo ? x[(s - 1) * n + tid] : 0.0;
   }
   if (evo) {
-    for (int idx = tid; idx < n * n; idx += 256) {
-      const int i = idx % n, j = idx / n;
-      if (!DD) M1[i + j * L] = ch.D[s * nn + idx];
-      M2[i + j * L] = ch.b_synth ? -ch.wk[s * n + i] * ch.wF[(ch.wFidx ? ch.wFidx[s] * nn : 0) + idx] : ch.B[s * nn + idx];
+#pragma unroll
+    for (int t = 0; t < PS; ++t) {
+      const int idx = tid + 256 * t;
+      if (idx < nb2) {
+        const int i = idx % n, j = idx / n;
+        if (!DD) M1[i + j * L] = tdv[DD ? 0 : t];
+        M2[i + j * L] = tbv[t];
+      }
     }
-    if (DD)
-      for (int i = tid; i < n; i += 256) dd[i] = ch.D[s * nn + i];
+    if (DD && tid < n) dd[tid] = ddv;
   }
   __syncthreads();
   /* D's entry (i, j): from the staged block, or the diagonal */
@@ -121,9 +162,13 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
     }
   __syncthreads(); /* B consumed: with DD its block takes C */
   if (m > 0) {
-    for (int idx = tid; idx < mr * n; idx += 256) {
-      const int i = idx % mr, j = idx / mr;
-      M1[i + j * L] = ch.C[s * nn + idx]; /* C is mr x n, ld mr; rows >= m are zero */
+#pragma unroll
+    for (int t = 0; t < PS; ++t) {
+      const int idx = tid + 256 * t;
+      if (idx < cn) {
+        const int i = idx % mr, j = idx / mr;
+        M1[i + j * L] = tcv[t]; /* C is mr x n, ld mr; rows >= m are zero */
+      }
     }
     __syncthreads();
     if (tid < mr) {
