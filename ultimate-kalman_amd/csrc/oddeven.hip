@@ -812,7 +812,11 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
   const int64_t s0 = (int64_t)blockIdx.x * spw;
   const int cnt = (int)((raw.steps - s0) < spw ? (raw.steps - s0) : spw);
   const int tid = threadIdx.x;
-  {
+  /* B formed where it is read and D stored as its diagonal: nothing is left
+   * of the n x n pass, and D's n entries per step are written by the c pass
+   * below, which reads the same k_i */
+  const bool d_only = out.b_synth && out.d_diag;
+  if (!d_only) {
     /* element e = c n + i of step ls; the indices advance by 256 elements
      * per iteration without a division */
     const int dls = 256 / nn, de = 256 - dls * nn, dc = de / n, di = de - dc * n;
@@ -864,9 +868,15 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
   for (int idx = tid; idx < cnt * n; idx += 256) {
     const int64_t s = s0 + idx / n;
     if (!raw.first[s]) {
-      const double v = raw.K[s0 * n + idx] * raw.c[s0 * n + idx];
+      const double kw = raw.K[s0 * n + idx];
+      const double v = kw * raw.c[s0 * n + idx];
       out.c[s0 * n + idx] = v;
       if (F32) out32.c[s0 * n + idx] = (float)v;
+      if (d_only) { /* D = diag(k): the slot's first n entries */
+        const int64_t g = s * nn + (idx - (s - s0) * n);
+        out.D[g] = kw;
+        if (F32) out32.D[g] = (float)kw;
+      }
     }
   }
   for (int idx = tid; idx < cnt * mr; idx += 256) {
