@@ -3,7 +3,8 @@
 Run in a container holding the reference sources (/root/reference) after
 `make -C oracle ref` has built oracle/_ref/libukref.so from them:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 # every fixture
+    python tests/golden/make_golden.py --only-shapes   # shapes_oddeven.npz only
 
 Every reference call runs in a forked child, so a reference assertion
 (its defects D1-D5) costs one case, not the script. Each fixture holds inputs
@@ -123,10 +124,42 @@ def small_cases():
     return cases
 
 
+def shapes_fixture():
+    """Per-step shapes of the reference's test catalogue
+    (matlab/examples/kalmanTest.m:80-95: rectangular H, changing state
+    dimension, more observation rows than states) through the reference's
+    kalman_smooth_oddeven. Cases whose length is not a power of two abort in
+    the reference (defect D1) and are recorded as such, inputs kept."""
+    kinds = [("W", "W"), ("C", "w"), ("w", "C"), ("U", "W")]
+    bundle, ok, aborted = {}, [], []
+    for t, (name, rows) in enumerate(sorted(P.shape_catalogue().items())):
+        K_kind, C_kind = kinds[t % len(kinds)]
+        pr = P.shape_problem(5000 + t, rows, K_kind=K_kind, C_kind=C_kind)
+        tag = f"{name}_{K_kind}{C_kind}"
+        arrs = P.problem_to_arrays(pr)
+        r = run_ref("oddeven", pr)
+        for key, v in arrs.items():
+            bundle[tag + "__" + key] = v
+        if r is None:
+            aborted.append(tag)
+            continue
+        ok.append(tag)
+        xs, cs = r
+        # per-step dimensions differ: states and covariances flattened step after step
+        bundle[tag + "__out_state_flat"] = np.concatenate([np.ravel(x) for x in xs])
+        bundle[tag + "__out_cov_flat"] = np.concatenate([np.ravel(c, order="F") for c in cs])
+    save("shapes_oddeven", bundle, dict(cases=ok + aborted, reference_ok=ok, reference_aborted=aborted,
+                                        note="reference kalman_smooth_oddeven on kalmanTest.m shape tables"))
+
+
 def main():
     if uk_libs.reference() is None:
         sys.exit("oracle/_ref/libukref.so is missing: run `make -C oracle ref` first")
     os.makedirs(HERE, exist_ok=True)
+    if "--only-shapes" in sys.argv:
+        shapes_fixture()
+        return
+    shapes_fixture()
 
     # 1. small odd-even and associative cases, inputs stored explicitly
     oe_ok, oe_fail, as_ok, as_fail = [], [], [], []

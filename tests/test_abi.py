@@ -174,6 +174,45 @@ def test_parallel_hints_and_algorithms(lib):
     k = lib.kalman_create_options(capi.KALMAN_ALGORITHM_ASSOCIATIVE)
     assert k
     lib.kalman_free(k)
+    # kalman_create() asks for the reference's default, ULTIMATE (c/kalman_base.c:231-232): NULL and an error
+    lib.ukhip_clear_error()
+    lib.kalman_create.restype = C.c_void_p
+    assert not lib.kalman_create()
+    assert b"ODDEVEN" in lib.ukhip_last_error()
+
+
+def test_batch_inputs_checked_before_the_device():
+    """ukhip_batch_create refuses a malformed problem (a per-step observation
+    count outside 0..m, an unknown covariance type, a missing array) before
+    anything is packed, uploaded or launched, so it fails the same way with or
+    without a GPU."""
+    from ukhip import device
+    from ukhip import problems as P
+    L = device.lib()
+    pr = P.packed_from_lists(P.random_problem(7, 6, 3, 2, K_kind="w", C_kind="w"))
+    keep = device._Keep()
+    good = device.problem_struct(pr, keep)
+    cases = []
+    bad = device.problem_struct(pr, keep)
+    mi = np.array([2, 2, 3, 2, 2, 2], dtype=np.int32)  # m_i = 3 > m = 2
+    keep.arrays.append(mi)
+    bad.m_i = mi.ctypes.data
+    cases.append((bad, b"m_i = 3 outside 0..m = 2"))
+    bad2 = device.problem_struct(pr, keep)
+    bad2.C_type = b"x"
+    cases.append((bad2, b"unknown covariance type"))
+    bad3 = device.problem_struct(pr, keep)
+    bad3.G = None
+    cases.append((bad3, b"missing input array"))
+    bad4 = device.problem_struct(pr, keep)
+    bad4.m = 40  # more observation rows than the device reduces for n = 3
+    cases.append((bad4, b"outside 0..16"))
+    for b, msg in cases:
+        arr = (device.ProblemStruct * 2)(good, b)
+        L.ukhip_clear_error()
+        assert not L.ukhip_batch_create(2, arr)
+        assert msg in L.ukhip_last_error(), L.ukhip_last_error()
+        assert b"problem 1" in L.ukhip_last_error()
 
 
 def test_smoother_without_device_fails_loudly(lib):

@@ -151,3 +151,39 @@ def test_w_covariance_option(L, oracle, algo):
         assert worst < 1e-9
     finally:
         L.kalman_free(kal)
+
+
+def test_shapes_through_kalman_t(L, oracle):
+    """kalman_evolve / kalman_observe with the reference's per-step shapes
+    (kalmanTest.m:80-95): a regular stretch first, then a change of state
+    dimension, rectangular H and more observation rows than states switch the
+    ingest to its general form; smoothing, kalman_forget and kalman_rollback
+    then work on it. Estimates and covariances (of each step's own dimension)
+    against the oracle."""
+    import json
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shapes_oddeven.npz")
+    d = np.load(gold)
+    meta = json.loads(bytes(d["meta"]).decode())
+    for tag in ("grow8_wC", "rand2_k64_wC", "wide16_wC", "mid3obs_UW"):
+        assert tag in meta["cases"]
+        pre = tag + "__"
+        pr = P.problem_from_arrays({k[len(pre):]: d[k] for k in d.files if k.startswith(pre)}, tag)
+        eqs = capi.EquationSet(pr)
+        kal = L.kalman_create_options(capi.KALMAN_ALGORITHM_ODDEVEN)
+        try:
+            for i in range(pr.k):
+                _feed(L, kal, eqs, i)
+            L.kalman_smooth(kal)
+            _check(L, kal, pr, 0, oracle, f"{tag}: all steps")
+            if pr.k >= 8:
+                L.kalman_forget(kal, 2)
+                L.kalman_smooth(kal)
+                _check(L, kal, _sub(pr, 3, pr.k), 3, oracle, f"{tag}: after kalman_forget(2)")
+                last = pr.k - 1
+                L.kalman_rollback(kal, last)
+                s = eqs.structs[last]
+                L.kalman_observe(kal, s.G, s.o, s.C, s.C_type or b"W")
+                L.kalman_smooth(kal)
+                _check(L, kal, _sub(pr, 3, pr.k), 3, oracle, f"{tag}: after kalman_rollback({last}) and observe")
+        finally:
+            L.kalman_free(kal)

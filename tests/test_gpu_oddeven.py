@@ -47,8 +47,6 @@ def test_small_fixture_cases(dev, oracle):
         pre = tag + "__"
         a = {k[len(pre):]: d[k] for k in d.files if k.startswith(pre)}
         pr = P.problem_from_arrays(a, tag)
-        if any(o is not None and o.shape[0] > pr.n[0] for o in pr.o):
-            continue  # more observations than states: rejected, see test_rejects_wide_observations
         say(f"small case {tag}")
         xs, cs = dev.smooth_equations(pr, "oddeven")
         assert max_rel_err(xs, list(a["out_state"])) < TOL, tag
@@ -56,13 +54,77 @@ def test_small_fixture_cases(dev, oracle):
         ox, oc = oracle.oddeven(pr)
         assert max_rel_err(xs, ox) < TOL and max_rel_err(cs, oc) < TOL, tag
         ran += 1
-    assert ran >= 40
+    assert ran == len(meta["cases"]) and ran >= 80  # every case, m > n included
 
 
-def test_rejects_wide_observations(dev):
-    pr = P.random_problem(3, 4, 2, 3)
-    with pytest.raises(dev.UkhipError, match="exceed"):
-        dev.smooth_equations(pr, "oddeven")
+def _shape_outputs(a, pr):
+    xs, cs, p, q = [], [], 0, 0
+    for n in pr.n:
+        xs.append(a["out_state_flat"][p:p + n])
+        cs.append(a["out_cov_flat"][q:q + n * n].reshape(n, n, order="F"))
+        p += n
+        q += n * n
+    return xs, cs
+
+
+def test_shape_fixture_cases(dev, oracle):
+    """The reference's shape catalogue (kalmanTest.m:80-95 plus larger random
+    tables): rectangular H_i, state dimensions that change from step to step,
+    more observation rows than states. Against the reference's outputs where
+    it ran and against the oracle on every case, with covariances and in the
+    'W' representation."""
+    d = np.load(os.path.join(GOLD, "shapes_oddeven.npz"))
+    meta = json.loads(bytes(d["meta"]).decode())
+    for tag in meta["cases"]:
+        pre = tag + "__"
+        a = {k[len(pre):]: d[k] for k in d.files if k.startswith(pre)}
+        pr = P.problem_from_arrays(a, tag)
+        say(f"shape case {tag} k={pr.k} n={min(pr.n)}..{max(pr.n)}")
+        xs, cs = dev.smooth_equations(pr, "oddeven")
+        ox, oc = oracle.oddeven(pr)
+        assert max_rel_err(xs, ox) < TOL and max_rel_err(cs, oc) < TOL, tag
+        if tag in meta["reference_ok"]:
+            rx, rc = _shape_outputs(a, pr)
+            assert max_rel_err(xs, rx) < TOL and max_rel_err(cs, rc) < TOL, tag
+        x2, _ = dev.smooth_equations(pr, "oddeven", options=capi.KALMAN_NO_COVARIANCE)
+        assert max_rel_err(x2, ox) < TOL, tag
+        xw, ws = dev.smooth_equations(pr, "oddeven", options=capi.UKHIP_COVARIANCE_W)
+        assert max_rel_err(xw, ox) < TOL, tag
+        worst = max(rel_err(w @ c @ w.T, np.eye(len(c))) for w, c in zip(ws, oc))
+        assert worst < TOL, tag
+
+
+def test_associative_rejects_rectangular_h(dev):
+    """The associative elements need square H and one dimension (as the
+    reference's, which never reads H): such a problem fails with an error."""
+    d = np.load(os.path.join(GOLD, "shapes_oddeven.npz"))
+    tag = "u3238_UW"
+    pre = tag + "__"
+    pr = P.problem_from_arrays({k[len(pre):]: d[k] for k in d.files if k.startswith(pre)}, tag)
+    with pytest.raises(dev.UkhipError, match="associative smoother needs"):
+        dev.smooth_equations(pr, "associative")
+
+
+def test_batch_wide_observations(dev, oracle):
+    """The batch API with more observation rows than states (m = n + 3 and
+    per-step m_i up to it), FP64 with covariances and the mixed-precision
+    path, against the oracle."""
+    pr = P.packed_from_lists(P.random_problem(91, 33, 5, 8, K_kind="w", C_kind="w",
+                                              m_of=lambda i: (5, 3, 8, 0, 6)[i % 5]))
+    pr2 = P.packed_from_lists(P.random_problem(92, 20, 5, 7, K_kind="C", C_kind="W", H_kind="R"))
+    say("batch with m > n")
+    b = dev.Batch([pr, pr2])
+    b.smooth("oddeven")
+    for i, q in enumerate((pr, pr2)):
+        x, S = b.download(i)
+        ox, oc = oracle.oddeven(q)
+        assert max_rel_err(list(x), ox) < TOL and max_rel_err(list(S), oc) < TOL
+    b.smooth_mixed(2)
+    for i, q in enumerate((pr, pr2)):
+        x, _ = b.download(i, want_cov=False)
+        ox, _ = oracle.oddeven(q)
+        assert max_rel_err(list(x), ox) < 1e-8
+    b.free()
 
 
 @pytest.mark.parametrize("n,k", [(6, 1024), (48, 128)])

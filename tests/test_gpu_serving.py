@@ -102,3 +102,28 @@ def test_budget_rejection(dev):
         assert np.all(np.isfinite(x))
     finally:
         srv.close()
+
+
+def test_bad_request_leaves_others_alone(dev, oracle, requests_):
+    """A malformed request (a per-step observation count above its m) is
+    refused by ukhip_server_submit itself, so it never joins a batch; the
+    valid requests submitted around it all complete and match the oracle."""
+    import copy
+    srv = dev.Server(hbm_budget_bytes=2 << 30, max_batch=8, max_inflight=2)
+    try:
+        good = requests_[:6]
+        ids = [srv.submit(p) for p in good[:3]]
+        bad = copy.copy(good[3])
+        bad.packed = dict(good[3].packed)
+        mi = np.where(bad.packed["observed"], bad.packed["m"], 0).astype(np.int32)
+        mi[len(mi) // 2] = bad.packed["m"] + 1
+        bad.packed["m_i"] = mi
+        with pytest.raises(dev.UkhipError, match="outside 0..m"):
+            srv.submit(bad)
+        ids += [srv.submit(p) for p in good[3:]]
+        outs = [srv.wait(i) for i in ids]
+    finally:
+        srv.close()
+    for p, (x, c, _) in zip(good, outs):
+        ox, oc = oracle.oddeven(p)
+        assert max_rel_err(list(x), ox) < TOL and max_rel_err(list(c), oc) < TOL, p.name

@@ -207,3 +207,94 @@ def test_checker_detects_planted_error(oracle):
     j = len(bad) // 2
     bad[j][0] += 1e-7 * np.max(np.abs(bad[j]))
     assert max_rel_err(bad, list(a["out_state"])) > TOL_OE
+
+
+def shape_outputs(a, pr):
+    """Reference outputs of a shapes_oddeven case, split per step."""
+    xs, cs, p, q = [], [], 0, 0
+    for n in pr.n:
+        xs.append(a["out_state_flat"][p:p + n])
+        cs.append(a["out_cov_flat"][q:q + n * n].reshape(n, n, order="F"))
+        p += n
+        q += n * n
+    return xs, cs
+
+
+def test_shapes_fixture(oracle):
+    """Per-step shapes of the reference's own catalogue (kalmanTest.m:80-95):
+    rectangular H, changing state dimension, more observations than states.
+    The oracle matches the reference where it ran (power-of-two lengths) and
+    the global least-squares definition everywhere."""
+    d, meta = load("shapes_oddeven")
+    assert len(meta["reference_ok"]) >= 15
+    kinds = set()
+    for tag in meta["cases"]:
+        a = sub_arrays(d, tag)
+        pr = P.problem_from_arrays(a, tag)
+        kinds |= {"rect_H" for h in pr.H if h is not None and h.shape[0] < h.shape[1]}
+        kinds |= {"n_changes" for i in range(1, pr.k) if pr.n[i] != pr.n[i - 1]}
+        kinds |= {"m>n" for i in range(pr.k) if pr.o[i] is not None and pr.o[i].shape[0] > pr.n[i]}
+        xs, cs = oracle.oddeven(pr)
+        lx, lc = uk_libs.dense_ls(pr)
+        assert max(max_rel_err(xs, lx), max_rel_err(cs, lc)) < 1e-12, tag
+        if tag in meta["reference_ok"]:
+            rx, rc = shape_outputs(a, pr)
+            assert max(max_rel_err(xs, rx), max_rel_err(cs, rc)) < TOL_OE, tag
+    assert kinds == {"rect_H", "n_changes", "m>n"}
+
+
+def pad_problem(pr):
+    """The padded form libukhip's kalman_smooth_oddeven packs (every step N =
+    max n_i components: [H_i 0], [F_i 0] and c_i in the first l_i of N rows,
+    zero rows below; G_i as [G_i 0] plus N - n_i unit-weight rows z_i = 0;
+    K_i, C_i extended block-diagonally with I), written here in numpy as an
+    independent statement of that construction."""
+    N = max(pr.n)
+    H, F, c, K, Kt, G, o, Cc, Ct = ([None] * pr.k for _ in range(9))
+    for i in range(pr.k):
+        n = pr.n[i]
+        if i > 0:
+            l = pr.c[i].shape[0]
+            H[i] = np.zeros((N, N)); H[i][:l, :n] = pr.H[i]
+            F[i] = np.zeros((N, N)); F[i][:l, :pr.n[i - 1]] = pr.F[i]
+            c[i] = np.zeros((N, 1)); c[i][:l] = pr.c[i]
+            Kt[i] = pr.K_type[i]
+            if Kt[i] == "w":
+                K[i] = np.ones((N, 1)); K[i][:l] = pr.K[i]
+            else:
+                K[i] = np.eye(N); K[i][:l, :l] = pr.K[i]
+        m = 0 if pr.o[i] is None else pr.o[i].shape[0]
+        mp = m + N - n
+        if mp == 0:
+            continue
+        G[i] = np.zeros((mp, N)); o[i] = np.zeros((mp, 1))
+        if m:
+            G[i][:m, :n] = pr.G[i]
+            o[i][:m] = pr.o[i]
+        for r in range(N - n):
+            G[i][m + r, n + r] = 1.0
+        Ct[i] = pr.C_type[i] if m else "w"
+        if Ct[i] == "w":
+            Cc[i] = np.ones((mp, 1))
+            if m:
+                Cc[i][:m] = pr.C[i]
+        else:
+            Cc[i] = np.eye(mp); Cc[i][:m, :m] = pr.C[i]
+    return P.Problem(k=pr.k, n=[N] * pr.k, H=H, F=F, c=c, K=K, K_type=Kt, G=G, o=o, C=Cc, C_type=Ct,
+                     name=pr.name + "_padded")
+
+
+def test_padding_equivalence(oracle):
+    """The padded problem's estimates and covariance blocks, restricted to each
+    step's own n_i components, are the unpadded problem's (the padding
+    components are decoupled and determined as zero)."""
+    d, meta = load("shapes_oddeven")
+    for tag in meta["cases"]:
+        pr = P.problem_from_arrays(sub_arrays(d, tag), tag)
+        xs, cs = oracle.oddeven(pr)
+        px, pc = oracle.oddeven(pad_problem(pr))
+        n = pr.n
+        assert max_rel_err([px[i][:n[i]] for i in range(pr.k)], xs) < 1e-12, tag
+        assert max_rel_err([pc[i][:n[i], :n[i]] for i in range(pr.k)], cs) < 1e-12, tag
+        pad = [px[i][n[i]:] for i in range(pr.k) if n[i] < max(n)]
+        assert all(np.max(np.abs(v)) < 1e-12 for v in pad), tag

@@ -42,7 +42,6 @@
 
 extern "C" void uk_set_error(const char* fmt, ...);
 extern "C" int uk_bucket(int n);
-extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 extern "C" void* ukhip_stream(void);
 
 namespace {
@@ -972,7 +971,8 @@ struct ukhip_batch;
 extern "C" void uk_batch_view(ukhip_batch* b, const uk_raw_t** raw, double** xout, double** Sout, int64_t* steps,
                               int* nprob, const int64_t** k, void** assoc_ws, size_t* assoc_bytes);
 extern "C" void uk_batch_set_assoc_ws(ukhip_batch* b, void* p, size_t bytes);
-extern "C" void uk_batch_chain0(ukhip_batch* b, uk_chain_t* ch);
+extern "C" void uk_batch_chain0(ukhip_batch* b, uk_chain_t* ch, const int32_t** m0);
+extern "C" hipError_t uk_batch_weigh_dense0(ukhip_batch* b, hipStream_t st);
 
 namespace {
 
@@ -1015,14 +1015,15 @@ int run_assoc(ukhip_batch* b) {
   void* ws;
   size_t ws_bytes;
   uk_batch_view(b, &raw, &xout, &Sout, &S, &nprob, &kk, &ws, &ws_bytes);
-  const int n = raw->n, mr = raw->mr;
+  const int n = raw->n;
   uk_chain_t wch;
-  uk_batch_chain0(b, &wch);
-  wch.mr = mr;
+  const int32_t* m_obs = nullptr;
+  uk_batch_chain0(b, &wch, &m_obs);
+  const int mr = wch.mr;
 
   /* plan: p of every problem from the observation counts */
   std::vector<int32_t> m_h(S);
-  if (!ok(hipMemcpy(m_h.data(), raw->m, S * sizeof(int32_t), hipMemcpyDeviceToHost), "download m")) return -1;
+  if (!ok(hipMemcpy(m_h.data(), m_obs, S * sizeof(int32_t), hipMemcpyDeviceToHost), "download m")) return -1;
   std::vector<AsProb> probs(nprob);
   std::vector<int32_t> elem_prob, step_prob(S);
   int64_t s0 = 0, ne = 0;
@@ -1129,12 +1130,12 @@ int run_assoc(ukhip_batch* b) {
   if (!ok(hipMemcpyAsync(W + o_tab, tab.data(), to, hipMemcpyHostToDevice, st), "upload scan tables")) return -1;
 
   const dim3 blk(AT);
-  if (!ok(uk_launch_weigh(raw, &wch, st), "k_weigh")) return -1;
+  if (!ok(uk_batch_weigh_dense0(b, st), "k_weigh")) return -1;
   ka_model<NB><<<dim3((unsigned)S), blk, 0, st>>>(wch, raw->first, md, n);
-  ka_prior<NB><<<dim3((unsigned)nprob), blk, 0, st>>>(d_probs, wch, raw->m, md, n, mr);
+  ka_prior<NB><<<dim3((unsigned)nprob), blk, 0, st>>>(d_probs, wch, m_obs, md, n, mr);
   if (ne > 0) {
     const unsigned nc = (unsigned)fch.size();
-    ka_felem<NB><<<dim3((unsigned)ne), blk, 0, st>>>(d_probs, d_elem_prob, wch, raw->m, md, fe, n, mr);
+    ka_felem<NB><<<dim3((unsigned)ne), blk, 0, st>>>(d_probs, d_elem_prob, wch, m_obs, md, fe, n, mr);
     ka_scan_reduce<NB, true><<<dim3(nc), blk, 0, st>>>(d_fch, fe, ft[0], n, gw);
     int cur = 0;
     for (int64_t d = 1; d < maxr_f; d <<= 1) {
@@ -1176,7 +1177,10 @@ extern "C" int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options) {
   void* ws;
   size_t wsb;
   uk_batch_view(b, &raw, &xout, &Sout, &S, &nprob, &k, &ws, &wsb);
-  switch (uk_bucket(raw->n > raw->mr ? raw->n : raw->mr)) {
+  uk_chain_t ch0;
+  const int32_t* m0;
+  uk_batch_chain0(b, &ch0, &m0);
+  switch (uk_bucket(raw->n > ch0.mr ? raw->n : ch0.mr)) {
     case 8: return run_assoc<8>(b);
     case 16: return run_assoc<16>(b);
     case 32: return run_assoc<32>(b);

@@ -10,8 +10,10 @@
  * Differences: smooth passes the steps from the first live one after
  * kalman_forget (the reference passes the array from physical slot 0,
  * c/kalman_explicit_representation.c:250-251); ULTIMATE and CONVENTIONAL
- * (sequential filters, not part of this library) are rejected with an error;
- * kalman_create() therefore defaults to ODDEVEN.
+ * (sequential filters, not part of this library) are rejected with an error.
+ * kalman_create() asks for ULTIMATE, as the reference's does
+ * (c/kalman_base.c:231-232), so it too returns NULL with an error: a client
+ * written for the default filter never silently gets a smoother.
  */
 #include <math.h>
 #include <stdint.h>
@@ -112,6 +114,7 @@ void uk_ingest_free(uk_ingest_t* g);
 int64_t uk_ingest_count(const uk_ingest_t* g);
 int64_t uk_ingest_first(const uk_ingest_t* g);
 int32_t uk_ingest_dimension(const uk_ingest_t* g);
+int32_t uk_ingest_step_dimension(const uk_ingest_t* g, int64_t si);
 int uk_ingest_evolve(uk_ingest_t* g, int32_t n, const kalman_matrix_t* H, const kalman_matrix_t* F,
                      const kalman_matrix_t* c, const kalman_matrix_t* K, char K_type);
 int uk_ingest_observe(uk_ingest_t* g, const kalman_matrix_t* G, const kalman_matrix_t* o, const kalman_matrix_t* C,
@@ -142,7 +145,7 @@ kalman_t* kalman_create_options(uint32_t options) {
   return k;
 }
 
-kalman_t* kalman_create(void) { return kalman_create_options(KALMAN_ALGORITHM_ODDEVEN); }
+kalman_t* kalman_create(void) { return kalman_create_options(KALMAN_ALGORITHM_ULTIMATE); }
 
 void kalman_free(kalman_t* k) {
   if (!k) return;
@@ -180,7 +183,7 @@ void kalman_smooth(kalman_t* k) {
 
 kalman_matrix_t* kalman_estimate(kalman_t* k, kalman_step_index_t si) {
   if (!k || !uk_ingest_valid(k->steps, si)) return NULL;
-  const int n = uk_ingest_dimension(k->steps);
+  const int n = uk_ingest_step_dimension(k->steps, si);
   const double* x = uk_ingest_state(k->steps, si);
   if (!x) return matrix_create_constant(n, 1, kalman_nan);
   kalman_matrix_t* m = matrix_create(n, 1);
@@ -190,7 +193,7 @@ kalman_matrix_t* kalman_estimate(kalman_t* k, kalman_step_index_t si) {
 
 kalman_matrix_t* kalman_covariance(kalman_t* k, kalman_step_index_t si) {
   if (!k || !uk_ingest_valid(k->steps, si)) return NULL;
-  const int n = uk_ingest_dimension(k->steps);
+  const int n = uk_ingest_step_dimension(k->steps, si);
   const double* S = uk_ingest_covariance(k->steps, si, NULL);
   if (!S) return matrix_create_constant(n, n, kalman_nan);
   kalman_matrix_t* m = matrix_create(n, n);

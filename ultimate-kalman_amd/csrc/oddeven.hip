@@ -711,10 +711,14 @@ __device__ __forceinline__ void apply_weight(double (&x)[NB], int m, int code, c
 
 /*
  * One step per workgroup. Lanes: [0,n) columns of F, [n,2n) columns of H,
- * 2n: c, [2n+1, 3n+1) columns of G, 3n+1: o.
+ * 2n: c, [2n+1, 3n+1) columns of G, 3n+1: o. The whitened observation block
+ * W G (mr rows, ld mr) and W o of step s go to out.C + s * cst and
+ * out.o + s * ost: the chain's slots (cst = n * n, ost = n), or a staging
+ * area that k_obs_compress reduces to n rows when mr > n.
  */
 template <int NB>
-__global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_t raw, uk_chain_t out) {
+__global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64)
+    k_weigh(uk_raw_t raw, uk_chain_t out, int64_t cst, int64_t ost) {
   __shared__ WeighShared<NB> sh;
   constexpr int NT = ((3 * NB + 2 + 63) / 64) * 64;
   const int tid = threadIdx.x;
@@ -792,7 +796,7 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64) k_weigh(uk_raw_
 #pragma unroll
       for (int i = 0; i < NB; ++i) x[i] = 0.0;
     }
-    double* dst = is_o ? out.o + s * n : out.C + s * nn + (int64_t)qq * mr;
+    double* dst = is_o ? out.o + s * ost : out.C + s * cst + (int64_t)qq * mr;
 #pragma unroll
     for (int i = 0; i < NB; ++i)
       if (i < mr) dst[i] = (i < m) ? x[i] : 0.0;
@@ -886,6 +890,45 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
     out.o[s * n + i] = v;
     if (F32) out32.o[s * n + i] = (float)v;
   }
+}
+
+/*
+ * Observation blocks taller than the state (mr > n rows after whitening,
+ * at most 2 NB): one step per workgroup, QR of [W G_s | W o_s] (mr x (n+1),
+ * staged by k_weigh at wC + s * mr * n, ld mr, and wo + s * mr); R (n x n,
+ * upper triangular) and the first n entries of Q^T W o become the chain's C
+ * (ld n) and o. The least-squares problem is unchanged: the dropped rows of
+ * Q^T [W G | W o] are zero in G's columns, so they add a constant to the
+ * residual and nothing to the normal equations. This lets the elimination
+ * keep one n-row observation slot per step, where the reference stacks the
+ * m_i rows into its phase-1 and phase-3 blocks
+ * (c/kalman_oddeven_smoother.c:422-440, :535-546).
+ */
+template <int NB>
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3>::NT))
+    k_obs_compress(const double* __restrict__ wC, const double* __restrict__ wo, int mr, int n,
+                   double* __restrict__ C, double* __restrict__ o) {
+  using G = QRGeom<NB, NB + 1, 3>;
+  constexpr int SL = G::SL;
+  constexpr int NT = G::NT;
+  __shared__ Refl<NB, SL, double> refl;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t s = blockIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  const int hi = mr > NB ? mr - NB : 0; /* rows in block 1 */
+  refl_clear<NB, SL, NT, double>(refl, tid);
+  Col<NB, SL, double> col;
+  const double* src = c < n ? wC + s * (int64_t)mr * n + (int64_t)c * mr : wo + s * mr;
+  if (c <= n) {
+    col.load_block(0, src, mr < NB ? mr : NB, sub);
+    col.load_block(1, src + NB, hi, sub);
+  } else {
+    col.zero_block(0);
+    col.zero_block(1);
+  }
+  house_qr<NB, SL, double>(col, refl, c, sub, 0, n, c <= n);
+  if (c < n) col.store_triu(C + s * nn + (int64_t)c * n, n, c, sub);
+  if (c == n) col.store_block(0, o + s * n, n, sub);
 }
 
 /*
@@ -1073,8 +1116,41 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
     return hipGetLastError();
   }
   const int nb = uk_bucket(raw->n > raw->mr ? raw->n : raw->mr);
-#define L_(NBV) k_weigh<NBV><<<dim3((unsigned)raw->steps), dim3(((3 * NBV + 2 + 63) / 64) * 64), 0, st>>>(*raw, *out)
+  const int64_t nn = (int64_t)raw->n * raw->n;
+#define L_(NBV)                                                                                            \
+  k_weigh<NBV><<<dim3((unsigned)raw->steps), dim3(((3 * NBV + 2 + 63) / 64) * 64), 0, st>>>(*raw, *out, nn, \
+                                                                                             (int64_t)raw->n)
   switch (nb) {
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    default: return hipErrorInvalidValue;
+  }
+#undef L_
+  return hipGetLastError();
+}
+
+/* Whitening when observation blocks have more rows than the state (raw->mr >
+ * n): k_weigh writes the evolution part into the chain and the whitened
+ * observations to wobs (mr * n + mr doubles per step), then k_obs_compress
+ * reduces every step's observation block to n rows in the chain's C (ld n)
+ * and o. raw->mr must not exceed 2 * uk_bucket(n) or 64. */
+extern "C" hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t* out, double* wobs, hipStream_t st) {
+  const int n = raw->n, mr = raw->mr;
+  const int nbn = uk_bucket(n), nb = uk_bucket(n > mr ? n : mr);
+  if (mr <= n || nbn < 0 || nb < 0 || mr > 2 * nbn) return hipErrorInvalidValue;
+  uk_chain_t w = *out;
+  w.C = wobs;
+  w.o = wobs + raw->steps * (int64_t)mr * n;
+  const unsigned g = (unsigned)raw->steps;
+#define L_(NBV) k_weigh<NBV><<<dim3(g), dim3(((3 * NBV + 2 + 63) / 64) * 64), 0, st>>>(*raw, w, (int64_t)mr * n, (int64_t)mr)
+  switch (nb) {
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    default: return hipErrorInvalidValue;
+  }
+#undef L_
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+#define L_(NBV) k_obs_compress<NBV><<<dim3(g), dim3(QRGeom<NBV, NBV + 1, 3>::NT), 0, st>>>(w.C, w.o, mr, n, out->C, out->o)
+  switch (nbn) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }

@@ -32,6 +32,7 @@
 extern "C" {
 int uk_bucket(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
+hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t* out, double* wobs, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                const uk_factors_t* fac, double* fix, int n, hipStream_t st, hipEvent_t* ev);
 hipError_t uk_launch_filter_step(const uk_filter_t* st, const uk_filter_in_t* in, int nf, int n, int first,
@@ -175,6 +176,14 @@ struct DevArena {
 struct ukhip_batch {
   int64_t nF = 0; /* F blocks in raw.F (1: every step shares one) */
   int n = 0, mr = 0, nprob = 0;
+  /* Level-0 observation slots: compress = observation blocks taller than n
+   * are reduced to n rows after whitening (staged in wobs, mr * n + mr doubles
+   * per step); the level-0 chain's C then has ld mr0 = n and step s holds
+   * m0[s] = min(m_s, n) rows. Otherwise mr0 = mr and m0 = raw.m. */
+  bool compress = false;
+  int mr0 = 0;
+  const int32_t* m0 = nullptr;
+  double* wobs = nullptr;
   int64_t steps = 0;
   std::vector<int64_t> k, orig_start;
   std::vector<LevelHost> levels;
@@ -297,9 +306,14 @@ struct Staging {
   int n = 0, mr = 0;
   int64_t steps = 0;
   int H_ident = 1, K_full = 0, C_full = 0;
+  /* compress = 1: some step has more observation rows than states (mr > n);
+   * the whitened observation blocks are then reduced to n rows on the device
+   * (k_obs_compress) and meff holds each step's row count after that,
+   * min(m_i, n) */
+  int compress = 0;
   int64_t nF = 0; /* F blocks (0: one per step) */
   size_t offFidx = 0, offF = 0, offH = 0, offc = 0, offK = 0, offG = 0, offo = 0, offC = 0, offKt = 0, offCt = 0, offm = 0,
-         offfirst = 0, bytes = 0;
+         offfirst = 0, offme = 0, bytes = 0;
   char* host = nullptr;
 
   void layout() {
@@ -323,6 +337,7 @@ struct Staging {
     offCt = take(S);
     offm = take(S * 4);
     offfirst = take(S);
+    offme = compress ? take(S * 4) : 0;
     bytes = o;
   }
   double* F() { return (double*)(host + offF); }
@@ -337,7 +352,20 @@ struct Staging {
   int8_t* Ct() { return (int8_t*)(host + offCt); }
   int32_t* m() { return (int32_t*)(host + offm); }
   int8_t* first() { return (int8_t*)(host + offfirst); }
+  int32_t* meff() { return (int32_t*)(host + offme); }
+  /* sets compress from mr and n, before layout() */
+  void decide_compress() { compress = mr > n ? 1 : 0; }
+  /* meff from m (compress only) */
+  void fill_meff() {
+    if (!compress) return;
+    for (int64_t s = 0; s < steps; ++s) meff()[s] = std::min(m()[s], n);
+  }
 };
+
+/* Largest observation row count (after whitening) a step may have for state
+ * dimension n: k_obs_compress holds the rows in two blocks of the n bucket,
+ * k_weigh in one block of at most 64 rows. */
+static int max_obs_rows(int n) { return std::min(2 * uk_bucket(n), UKHIP_MAX_N); }
 
 static void plan_levels(ukhip_batch* b) {
   b->levels.clear();
@@ -451,6 +479,9 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->raw.Ct = (const int8_t*)(r + st.offCt);
   b->raw.m = (const int32_t*)(r + st.offm);
   b->raw.first = (const int8_t*)(r + st.offfirst);
+  b->compress = st.compress != 0;
+  b->mr0 = b->compress ? n : st.mr;
+  b->m0 = b->compress ? (const int32_t*)(r + st.offme) : b->raw.m;
 
   /* workspace: chains (2), factors, cross covariances (2 x 2), outputs */
   int64_t L1 = b->levels.size() > 1 ? b->levels[1].steps : 1;
@@ -482,6 +513,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   }
   const size_t xoo = take(S * n * 8), Soo = take(S * nn * 8);
   const size_t fxo = take((size_t)b->nprob * (nn + n) * 8);
+  const size_t wbo = b->compress ? take(S * (size_t)st.mr * (n + 1) * 8) : 0;
   b->work_bytes = o;
   b->d_work = dev_alloc(b, o, "device memory for the workspace");
   if (!b->d_work) { batch_release(b); return nullptr; }
@@ -502,6 +534,7 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->xout = (double*)(w + xoo);
   b->Sout = (double*)(w + Soo);
   b->fix = (double*)(w + fxo);
+  b->wobs = b->compress ? (double*)(w + wbo) : nullptr;
 
   /* level tables: one int64 array; prob indices stored widened */
   std::vector<int64_t> tab;
@@ -613,32 +646,62 @@ static bool cov_shape_ok(const kalman_matrix_t* A, char t, int dim) {
   return A->row_dim == dim && A->col_dim == dim;
 }
 
-/* Validates and packs kalman_step_equations_t (c/kalman.h:56-75). */
+/*
+ * Validates and packs kalman_step_equations_t (c/kalman.h:56-75) into the
+ * device layout, with every step padded to N = max_i n_i state components.
+ *
+ * The reference lets step i carry its own state dimension n_i, an evolution
+ * equation of l_i rows (H_i l_i x n_i, F_i l_i x n_{i-1}) and m_i observation
+ * rows, and sizes every block of the elimination from them
+ * (c/kalman_oddeven_smoother.c:422-440 ll = MIN(nn, rows(QR)), :480-494
+ * ll = rows(H_i), nn = cols(H_i); matlab/KalmanExplicitRepresentation.m:95-101
+ * pads H_i as [I 0]). Here each step has N state components, u'_i =
+ * [u_i; z_i], and the same least-squares problem in that form:
+ *   - evolution: [H_i 0] u'_i = [F_i 0] u'_{i-1} + c_i in the first l_i of N
+ *     rows; the remaining rows are zero (zero rows leave every Householder
+ *     factorization unchanged);
+ *   - observation: the m_i rows [G_i 0] u'_i = o_i, plus N - n_i rows
+ *     z_i = 0 of unit weight.
+ * The padding components z_i appear in no other equation, so they are
+ * determined as zero, the normal equations are block diagonal between u and
+ * z, and u's estimates and covariance blocks equal the unpadded problem's.
+ * K_i and C_i are padded as block-diag(K_i, I) in their own representation
+ * ('w': the vector extended with ones). A step with more observation rows
+ * than N (after padding) is reduced to N rows on the device (k_obs_compress).
+ */
 static bool pack_equations(kalman_step_equations_t** eq, int64_t k, Staging& st) {
   if (k < 1) { uk_set_error("length must be >= 1"); return false; }
-  const int n = eq[0]->dimension;
-  if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("state dimension %d outside 1..%d", n, UKHIP_MAX_N); return false; }
+  int N = 0;
+  for (int64_t i = 0; i < k; ++i) {
+    if (!eq[i]) { uk_set_error("step %lld: NULL step equations", (long long)i); return false; }
+    N = std::max(N, (int)eq[i]->dimension);
+  }
+  if (N < 1 || N > UKHIP_MAX_N) { uk_set_error("state dimension %d outside 1..%d", N, UKHIP_MAX_N); return false; }
   int mr = 0;
   st.H_ident = 1;
   st.K_full = 0;
   st.C_full = 0;
   for (int64_t i = 0; i < k; ++i) {
     kalman_step_equations_t* e = eq[i];
-    if (e->dimension != n) { uk_set_error("step %lld: dimension %d differs from %d", (long long)i, e->dimension, n); return false; }
+    const int n = e->dimension;
+    if (n < 1) { uk_set_error("step %lld: state dimension %d < 1", (long long)i, n); return false; }
     if (i > 0) {
-      CHECK_SHAPE(e->H, n, n, "H");
-      CHECK_SHAPE(e->F, n, n, "F");
-      CHECK_SHAPE(e->c, n, 1, "c");
-      if (cov_code(e->K_type) < 0 || !cov_shape_ok(e->K, e->K_type, n)) {
+      const int np = eq[i - 1]->dimension;
+      if (!e->c || e->c->col_dim != 1) { uk_set_error("step %lld: c must be a column vector", (long long)i); return false; }
+      const int l = e->c->row_dim;
+      if (l > N) { uk_set_error("step %lld: %d evolution rows exceed the state dimension %d", (long long)i, l, N); return false; }
+      CHECK_SHAPE(e->H, l, n, "H");
+      CHECK_SHAPE(e->F, l, np, "F");
+      if (cov_code(e->K_type) < 0 || !cov_shape_ok(e->K, e->K_type, l)) {
         uk_set_error("step %lld: K of type '%c' has the wrong shape or type", (long long)i, e->K_type);
         return false;
       }
       if (e->K_type != 'w') st.K_full = 1;
-      if (st.H_ident && !is_identity(e->H)) st.H_ident = 0;
+      if (st.H_ident && (l != N || n != N || !is_identity(e->H))) st.H_ident = 0;
     }
+    int m = 0;
     if (e->o != NULL) {
-      const int m = e->o->row_dim;
-      if (m > n) { uk_set_error("step %lld: %d observations exceed the state dimension %d", (long long)i, m, n); return false; }
+      m = e->o->row_dim;
       CHECK_SHAPE(e->o, m, 1, "o");
       CHECK_SHAPE(e->G, m, n, "G");
       if (cov_code(e->C_type) < 0 || !cov_shape_ok(e->C, e->C_type, m)) {
@@ -646,36 +709,88 @@ static bool pack_equations(kalman_step_equations_t** eq, int64_t k, Staging& st)
         return false;
       }
       if (e->C_type != 'w') st.C_full = 1;
-      mr = std::max(mr, m);
     }
+    const int mp = m + (N - n); /* with the padding components' unit rows */
+    if (mp > max_obs_rows(N)) {
+      uk_set_error("step %lld: %d observation rows (with %d padding rows for state dimension %d of %d) exceed "
+                   "the %d this build reduces on the device",
+                   (long long)i, mp, N - n, n, N, max_obs_rows(N));
+      return false;
+    }
+    mr = std::max(mr, mp);
   }
-  st.n = n;
+  st.n = N;
   st.mr = std::max(mr, 1);
   st.steps = k;
+  st.decide_compress();
   if (!staging_alloc(st)) return false;
-  const size_t nn = (size_t)n * n;
-  const int Kw = st.K_full ? n * n : n, Cw = st.C_full ? st.mr * st.mr : st.mr;
+  const size_t nn = (size_t)N * N;
+  const int Kw = st.K_full ? N * N : N, Cw = st.C_full ? st.mr * st.mr : st.mr;
   std::memset(st.host, 0, st.bytes);
   for (int64_t i = 0; i < k; ++i) {
     kalman_step_equations_t* e = eq[i];
+    const int n = e->dimension;
     st.first()[i] = i == 0;
     st.Fidx()[i] = i;
     if (i > 0) {
-      copy_cols(st.F() + i * nn, n, e->F, n, n);
-      if (!st.H_ident) copy_cols(st.H() + i * nn, n, e->H, n, n);
-      copy_cols(st.c() + i * n, n, e->c, n, 1);
-      copy_cols(st.K() + i * Kw, n, e->K, e->K->row_dim, e->K->col_dim);
+      const int l = e->c->row_dim, np = eq[i - 1]->dimension;
+      copy_cols(st.F() + i * nn, N, e->F, l, np);
+      if (!st.H_ident) copy_cols(st.H() + i * nn, N, e->H, l, n);
+      copy_cols(st.c() + i * N, N, e->c, l, 1);
+      double* K = st.K() + i * Kw;
+      if (e->K_type == 'w') {
+        copy_cols(K, N, e->K, l, 1);
+        for (int r = l; r < N; ++r) K[r] = 1.0;
+      } else {
+        copy_cols(K, N, e->K, l, l);
+        for (int r = l; r < N; ++r) K[(size_t)r * N + r] = 1.0;
+      }
       st.Kt()[i] = (int8_t)cov_code(e->K_type);
     }
-    if (e->o != NULL) {
-      const int m = e->o->row_dim;
-      st.m()[i] = m;
-      copy_cols(st.G() + i * (size_t)st.mr * n, st.mr, e->G, m, n);
-      copy_cols(st.o() + i * (size_t)st.mr, st.mr, e->o, m, 1);
-      if (e->C_type == 'w') copy_cols(st.C() + i * (size_t)Cw, st.mr, e->C, m, 1);
-      else copy_cols(st.C() + i * (size_t)Cw, st.mr, e->C, m, m);
-      st.Ct()[i] = (int8_t)cov_code(e->C_type);
+    const int m = e->o ? e->o->row_dim : 0, pad = N - n, mp = m + pad;
+    st.m()[i] = mp;
+    if (mp == 0) continue;
+    double* G = st.G() + i * (size_t)st.mr * N;
+    double* o = st.o() + i * (size_t)st.mr;
+    double* C = st.C() + i * (size_t)Cw;
+    const char ct = m > 0 ? e->C_type : 'w';
+    if (m > 0) {
+      copy_cols(G, st.mr, e->G, m, n);
+      copy_cols(o, st.mr, e->o, m, 1);
     }
+    for (int r = 0; r < pad; ++r) G[(size_t)(n + r) * st.mr + m + r] = 1.0; /* z_i = 0, unit weight */
+    if (ct == 'w') {
+      if (m > 0) copy_cols(C, st.mr, e->C, m, 1);
+      for (int r = m; r < mp; ++r) C[r] = 1.0;
+    } else {
+      copy_cols(C, st.mr, e->C, m, m);
+      for (int r = m; r < mp; ++r) C[(size_t)r * st.mr + r] = 1.0;
+    }
+    st.Ct()[i] = (int8_t)cov_code(ct);
+  }
+  st.fill_meff();
+  return true;
+}
+
+/* Checks one ukhip_problem_t before anything is packed or launched: sizes,
+ * covariance types, required arrays and every per-step observation count
+ * (0 <= m_i <= m, so that a step never reaches past its m-row slot). */
+static bool problem_ok(const ukhip_problem_t& q, int p) {
+  if (q.n < 1 || q.n > UKHIP_MAX_N) { uk_set_error("problem %d: state dimension %d outside 1..%d", p, q.n, UKHIP_MAX_N); return false; }
+  if (q.k < 1) { uk_set_error("problem %d: need k >= 1", p); return false; }
+  if (q.m < 0 || q.m > max_obs_rows(q.n)) {
+    uk_set_error("problem %d: m = %d outside 0..%d (observation rows the device reduces for n = %d)", p, q.m,
+                 max_obs_rows(q.n), q.n);
+    return false;
+  }
+  if (cov_code(q.K_type) < 0 || cov_code(q.C_type) < 0) { uk_set_error("problem %d: unknown covariance type", p); return false; }
+  if (!q.F || !q.K || (q.m > 0 && (!q.G || !q.o || !q.C))) { uk_set_error("problem %d: missing input array", p); return false; }
+  if (q.m_i) {
+    for (int64_t i = 0; i < q.k; ++i)
+      if (q.m_i[i] < 0 || q.m_i[i] > q.m) {
+        uk_set_error("problem %d: step %lld has m_i = %d outside 0..m = %d", p, (long long)i, q.m_i[i], q.m);
+        return false;
+      }
   }
   return true;
 }
@@ -693,9 +808,7 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
   for (int p = 0; p < nprob; ++p) {
     const ukhip_problem_t& q = P[p];
     if (q.n != n) { uk_set_error("problem %d: n = %d differs from %d (one n per batch)", p, q.n, n); return false; }
-    if (q.k < 1 || q.m < 0 || q.m > n) { uk_set_error("problem %d: need k >= 1 and 0 <= m <= n", p); return false; }
-    if (cov_code(q.K_type) < 0 || cov_code(q.C_type) < 0) { uk_set_error("problem %d: unknown covariance type", p); return false; }
-    if (!q.F || !q.K || (q.m > 0 && (!q.G || !q.o || !q.C))) { uk_set_error("problem %d: missing input array", p); return false; }
+    if (!problem_ok(q, p)) return false;
     if (q.H) st.H_ident = 0;
     if (q.K_type != 'w') st.K_full = 1;
     if (q.C_type != 'w') st.C_full = 1;
@@ -708,6 +821,7 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
   st.mr = mr;
   st.steps = S;
   st.nF = nF;
+  st.decide_compress();
   if (!staging_alloc(st)) return false;
   /* Every field a kernel reads is written below (no blanket clearing): rows
    * of G, o, C at or beyond a step's m_i and the evolution fields of a
@@ -805,6 +919,7 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
       });
     for (auto& th : pool) th.join();
   }
+  st.fill_meff();
   return true;
 }
 
@@ -821,31 +936,45 @@ static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
  * every K is given as 'w' (V diagonal). When C is 'w' too, k_weigh_diag writes
  * the chain and stores D as its diagonal; the phase kernels and k_normal read
  * it so, and phase 2 skips D's exact zeros. */
-static bool level0_d_diagonal(const ukhip_batch* b) { return b->raw.H_ident && !b->raw.K_full && !b->raw.C_full; }
+static bool level0_d_diagonal(const ukhip_batch* b) {
+  return b->raw.H_ident && !b->raw.K_full && !b->raw.C_full && !b->compress;
+}
 
 /* With every K and C given as 'w', the level-0 chain's B_s = -diag(k_s) F is
  * not stored: its readers form it from the raw K and F. */
 template <class Chain>
 static Chain level0_chain(const ukhip_batch* b, Chain c) {
   c.d_diag = level0_d_diagonal(b);
-  c.b_synth = !b->raw.K_full && !b->raw.C_full;
+  c.b_synth = !b->raw.K_full && !b->raw.C_full && !b->compress; /* compressed chains come from k_weigh */
+  c.mr = b->mr0;
+  c.c_tri = b->compress; /* k_obs_compress leaves every C upper triangular */
   c.wk = b->raw.K;
   c.wF = b->raw.F;
   c.wFidx = b->nF == 1 ? nullptr : b->raw.Fidx; /* one F block: no per-step lookup */
   return c;
 }
 
+/* The whitened level-0 chain (c/kalman_oddeven_smoother.c:1620-1659): k_weigh
+ * or the element-parallel k_weigh_diag; with observation blocks taller than n,
+ * k_weigh followed by k_obs_compress. */
+static bool weigh_level0(ukhip_batch* b, hipStream_t st) {
+  if (b->compress) return hip_ok(uk_launch_weigh_obs(&b->raw, &b->chain[0], b->wobs, st), "k_weigh launch");
+  const uk_chain_t w0 = level0_chain(b, b->chain[0]);
+  return hip_ok(uk_launch_weigh(&b->raw, &w0, st), "k_weigh launch");
+}
+
 static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
                         hipStream_t st) {
   const int nl = (int)b->levels.size();
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
-  const uk_chain_t w0 = level0_chain(b, b->chain[0]);
-  if (!hip_ok(uk_launch_weigh(&b->raw, &w0, st), "k_weigh launch")) return false;
+  if (!weigh_level0(b, st)) return false;
   if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
   for (int l = 0; l < nl; ++l) {
     uk_chain_t in = l == 0 ? level0_chain(b, b->chain[0]) : b->chain[l & 1];
-    in.mr = l == 0 ? b->mr : b->n;
-    in.c_tri = l > 0;
+    if (l > 0) {
+      in.mr = b->n;
+      in.c_tri = 1;
+    }
     uk_chain_t out = b->chain[(l + 1) & 1]; /* its flags stay 0: a dense chain */
     out.mr = b->n;
     uk_factors_t f = b->fac;
@@ -899,8 +1028,10 @@ static ukhip_batch* batch_create_one(int32_t nprob, const ukhip_problem_t* probl
 }
 
 extern "C" ukhip_batch_t* ukhip_batch_create(int32_t nprob, const ukhip_problem_t* problems) {
-  if (!ensure_stream()) return nullptr;
   if (nprob < 1 || !problems) { uk_set_error("ukhip_batch_create: no problems"); return nullptr; }
+  for (int32_t p = 0; p < nprob; ++p) /* every input is checked before the device is touched */
+    if (!problem_ok(problems[p], p)) return nullptr;
+  if (!ensure_stream()) return nullptr;
   bool mixed = false;
   for (int32_t p = 1; p < nprob; ++p) mixed |= problems[p].n != problems[0].n;
   if (!mixed) return batch_create_one(nprob, problems);
@@ -1146,10 +1277,10 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
   const uk_chain_t w0 = level0_chain(b, b->chain[0]); /* its flags imply the fused path below */
-  const hipError_t fused = uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
+  const hipError_t fused = b->compress ? hipErrorNotSupported : uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
   if (fused != hipErrorNotSupported) {
     if (!hip_ok(fused, "k_weigh launch")) return false;
-  } else if (!hip_ok(uk_launch_weigh(&b->raw, &b->chain[0], st), "k_weigh launch") ||
+  } else if (!weigh_level0(b, st) ||
       !hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.D, c64.D, S * nn, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.c, c64.c, S * n, st), "to32") ||
@@ -1158,8 +1289,10 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
     return false;
   for (int l = 0; l < nl; ++l) {
     uk_chain32_t in = l == 0 ? level0_chain(b, b->chain32[0]) : b->chain32[l & 1];
-    in.mr = l == 0 ? b->mr : n;
-    in.c_tri = l > 0;
+    if (l > 0) {
+      in.mr = n;
+      in.c_tri = 1;
+    }
     uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
     out.mr = n;
     const uk_factors32_t f = fac32_at(b, l);
@@ -1172,10 +1305,9 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
     uk_level_t lv = b->levels[l].dev;
     if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
   }
-  uk_chain_t r64 = level0_chain(b, b->chain[0]);
-  r64.mr = b->mr;
+  const uk_chain_t r64 = level0_chain(b, b->chain[0]);
   for (int it = 0; it < refinements; ++it) {
-    if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->raw.m, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),
+    if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->m0, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),
                 "residual launch"))
       return false;
     for (int l = 0; l < nl; ++l) {
@@ -1252,34 +1384,59 @@ extern "C" void uk_batch_set_assoc_ws(ukhip_batch* b, void* p, size_t bytes) {
   b->d_assoc = p;
   b->assoc_bytes = bytes;
 }
-/* the level-0 chain buffers, which hold the whitened steps of the batch */
-extern "C" void uk_batch_chain0(ukhip_batch* b, uk_chain_t* ch) { *ch = b->chain[0]; }
+/* the level-0 chain buffers, which hold the whitened steps of the batch (B
+ * and D stored in full), with the level-0 observation slots: ld mr0 and m0[s]
+ * rows per step */
+extern "C" void uk_batch_chain0(ukhip_batch* b, uk_chain_t* ch, const int32_t** m0) {
+  *ch = b->chain[0];
+  ch->mr = b->mr0;
+  *m0 = b->m0;
+}
+/* whitens the batch into that chain (every block stored in full) */
+extern "C" hipError_t uk_batch_weigh_dense0(ukhip_batch* b, hipStream_t st) {
+  if (b->compress) return uk_launch_weigh_obs(&b->raw, &b->chain[0], b->wobs, st);
+  return uk_launch_weigh(&b->raw, &b->chain[0], st);
+}
 
 /* ------------------------------------------------------------------------ */
 /* the reference's smoother entry points (c/kalman.h:148-149)                */
 /* ------------------------------------------------------------------------ */
 
-static void smooth_equations(kalman_options_t options, kalman_step_equations_t** equations, kalman_step_index_t length,
+static bool smooth_equations(kalman_options_t options, kalman_step_equations_t** equations, kalman_step_index_t length,
                              bool associative) {
-  if (!ensure_stream()) return;
+  if (!ensure_stream()) return false;
   if (!equations || length < 1) {
     uk_set_error("kalman_smooth: need length >= 1 and an equations array");
-    return;
+    return false;
   }
   Staging st;
   if (!pack_equations(equations, length, st)) {
     staging_free(st);
-    return;
+    return false;
+  }
+  if (associative) {
+    /* the associative elements need square, invertible H_i and one state
+     * dimension (c/kalman_associative_smoother.c:309-505 never reads H) */
+    for (kalman_step_index_t i = 0; i < length; ++i) {
+      const kalman_step_equations_t* e = equations[i];
+      if (e->dimension != st.n || (i > 0 && e->H->row_dim != e->H->col_dim)) {
+        uk_set_error("kalman_smooth_associative: step %lld has state dimension %d of %d or a %d x %d H; the "
+                     "associative smoother needs one dimension and square H (use kalman_smooth_oddeven)",
+                     (long long)i, e->dimension, st.n, i > 0 ? e->H->row_dim : 0, i > 0 ? e->H->col_dim : 0);
+        staging_free(st);
+        return false;
+      }
+    }
   }
   std::vector<int64_t> lengths{(int64_t)length};
   ukhip_batch* b = batch_from_staging(st, lengths);
   staging_free(st);
-  if (!b) return;
+  if (!b) return false;
   const int cov = associative ? 1 : ((options & KALMAN_NO_COVARIANCE) == 0);
   int rc = associative ? uk_assoc_smooth(b, options) : (run_oddeven(b, cov, nullptr, g_stream) ? 0 : -1);
   if (rc != 0) {
     batch_release(b);
-    return;
+    return false;
   }
   const int n = b->n;
   const bool wform = cov && (options & UKHIP_COVARIANCE_W) != 0;
@@ -1287,7 +1444,7 @@ static void smooth_equations(kalman_options_t options, kalman_step_equations_t**
   if (ukhip_batch_download(b, 0, xs.data(), (cov && !wform) ? Ss.data() : nullptr) != 0 ||
       (wform && ukhip_batch_download_cov(b, 0, 'W', Ss.data()) != 0)) {
     batch_release(b);
-    return;
+    return false;
   }
   batch_release(b);
   for (kalman_step_index_t i = 0; i < length; ++i) {
@@ -1296,16 +1453,23 @@ static void smooth_equations(kalman_options_t options, kalman_step_equations_t**
       matrix_free(e->state);
       matrix_free(e->covariance);
     }
-    e->state = matrix_create(n, 1);
-    std::memcpy(e->state->elements, &xs[(size_t)i * n], (size_t)n * 8);
+    /* the step's own n_i components: the leading part of the padded state
+     * (and the leading block of its covariance, or of W = L^{-1}: both
+     * Cholesky factor and inverse of a lower-triangular matrix keep their
+     * leading blocks) */
+    const int ni = e->dimension;
+    e->state = matrix_create(ni, 1);
+    std::memcpy(e->state->elements, &xs[(size_t)i * n], (size_t)ni * 8);
     if (cov) {
-      e->covariance = matrix_create(n, n);
-      std::memcpy(e->covariance->elements, &Ss[(size_t)i * n * n], (size_t)n * n * 8);
+      e->covariance = matrix_create(ni, ni);
+      for (int c = 0; c < ni; ++c)
+        std::memcpy(e->covariance->elements + (size_t)c * ni, &Ss[(size_t)i * n * n + (size_t)c * n], (size_t)ni * 8);
     } else {
       e->covariance = NULL;
     }
     e->covariance_type = wform ? 'W' : 'C';
   }
+  return true;
 }
 
 extern "C" void kalman_smooth_oddeven(kalman_options_t options, kalman_step_equations_t** equations,
@@ -1370,6 +1534,7 @@ struct InFlight {
 size_t batch_device_bytes(int n, int mr, int64_t steps, bool mixed) {
   const double nn = (double)n * n;
   double per = 8.0 * (14.0 * nn + 10.0 * n + 4.0 * mr + mr * (double)n + (double)mr * mr);
+  if (mr > n) per += 8.0 * mr * (n + 1.0) + 4.0; /* whitened observations staged for k_obs_compress, m0 */
   if (mixed) per += 4.0 * (7.5 * nn + 4 * n) + 8.0 * (6.0 * n);
   return (size_t)(per * (double)steps * 1.05) + (size_t)(4u << 20);
 }
@@ -1624,8 +1789,11 @@ extern "C" ukhip_server_t* ukhip_server_create(const ukhip_server_options_t* opt
 
 extern "C" int64_t ukhip_server_submit(ukhip_server_t* s, const ukhip_problem_t* p) {
   if (!s || !p) { uk_set_error("NULL server or problem"); return -1; }
-  if (p->n < 1 || p->n > UKHIP_MAX_N || p->k < 1 || p->m < 0 || p->m > p->n) {
-    uk_set_error("ukhip_server_submit: need 1 <= n <= %d, k >= 1, 0 <= m <= n", UKHIP_MAX_N);
+  /* the whole request is checked here, so that a malformed one never joins
+   * (and fails) a batch of valid requests */
+  if (!problem_ok(*p, 0)) {
+    const std::string why = ukhip_last_error();
+    uk_set_error("ukhip_server_submit: %s", why.c_str());
     return -1;
   }
   /* the request's own copy of its inputs, in one block reused from the
@@ -1855,6 +2023,15 @@ struct uk_ingest {
   int n = -1, mr = 1;
   int64_t start = 0, count = 0, step0 = 0;
   bool current = false;
+  /* General mode: once a step has another state dimension, an evolution
+   * equation of other than n rows, or more observation rows than states, the
+   * live steps (and the pending one, at the back) are kept as deep copies of
+   * their equations, as the reference's explicit representation keeps them
+   * (c/kalman_explicit_representation.c:164-244), and kalman_smooth hands
+   * them to the packing of kalman_smooth_oddeven, which pads every step to
+   * one dimension. eqs[q] is step step0 + q. */
+  bool general = false;
+  std::deque<kalman_step_equations_t*> eqs;
   bool H_any = false, K_full = false, C_full = false;
   std::vector<double> F, H, c, K, G, o, C, x, S;
   std::vector<int8_t> Kt, Ct, hasx, hasS;
@@ -1899,6 +2076,76 @@ struct uk_ingest {
     mv(Kt, 1); mv(Ct, 1); mv(hasx, 1); mv(hasS, 1); mv(m, 1);
     start = 0;
   }
+  ~uk_ingest() {
+    for (auto* e : eqs) eq_free(e);
+  }
+  static kalman_matrix_t* mat_from(const double* src, int rows, int cols, int ld) {
+    kalman_matrix_t* A = matrix_create(rows, cols);
+    for (int c = 0; c < cols; ++c) std::memcpy(A->elements + (size_t)c * rows, src + (size_t)c * ld, (size_t)rows * 8);
+    return A;
+  }
+  static void eq_clear_observation(kalman_step_equations_t* e) {
+    matrix_free(e->G);
+    matrix_free(e->o);
+    matrix_free(e->C);
+    matrix_free(e->state);
+    matrix_free(e->covariance);
+    e->G = e->o = e->C = e->state = e->covariance = nullptr;
+  }
+  static void eq_free(kalman_step_equations_t* e) {
+    if (!e) return;
+    eq_clear_observation(e);
+    matrix_free(e->H);
+    matrix_free(e->F);
+    matrix_free(e->c);
+    matrix_free(e->K);
+    std::free(e);
+  }
+  static char cov_char(int code) {
+    return code == UK_COV_W ? 'W' : code == UK_COV_w ? 'w' : code == UK_COV_C ? 'C' : 'F';
+  }
+  /* moves every live slot (and the pending one) into eqs */
+  void to_general() {
+    if (general) return;
+    const int64_t total = count + (current ? 1 : 0);
+    const size_t nn_ = nn();
+    for (int64_t q = 0; q < total; ++q) {
+      const int64_t j = start + q;
+      auto* e = (kalman_step_equations_t*)std::calloc(1, sizeof(kalman_step_equations_t));
+      e->step = (kalman_step_index_t)(step0 + q);
+      e->dimension = n;
+      e->covariance_type = 'C';
+      if (q > 0) { /* the earliest live step's evolution equation is never used */
+        e->F = mat_from(&F[j * nn_], n, n, n);
+        if (H_any) {
+          e->H = mat_from(&H[j * nn_], n, n, n);
+        } else {
+          e->H = matrix_create(n, n);
+          for (size_t t = 0; t < nn_; ++t) e->H->elements[t] = (t % (n + 1) == 0) ? 1.0 : 0.0;
+        }
+        e->c = mat_from(&c[j * n], n, 1, n);
+        e->K_type = cov_char(Kt[j]);
+        e->K = Kt[j] == UK_COV_w ? mat_from(&K[j * Kw()], n, 1, n) : mat_from(&K[j * Kw()], n, n, n);
+      }
+      if (m[j] > 0) {
+        const int mi = m[j];
+        e->G = mat_from(&G[j * (size_t)mr * n], mi, n, mr);
+        e->o = mat_from(&o[j * (size_t)mr], mi, 1, mr);
+        e->C_type = cov_char(Ct[j]);
+        e->C = Ct[j] == UK_COV_w ? mat_from(&C[j * Cw()], mi, 1, mr) : mat_from(&C[j * Cw()], mi, mi, mr);
+      }
+      if (hasx[j]) e->state = mat_from(&x[j * n], n, 1, n);
+      if (hasS[j]) {
+        e->covariance = mat_from(&S[j * nn_], n, n, n);
+        e->covariance_type = Stype;
+      }
+      eqs.push_back(e);
+    }
+    general = true;
+    for (auto* v : {&F, &H, &c, &K, &G, &o, &C, &x, &S}) std::vector<double>().swap(*v);
+    start = 0;
+    cap = 0;
+  }
   /* re-lays a field from stride a to stride b (b >= a), the first `keep` entries of each slot */
   static void restride(std::vector<double>& v, int64_t slots, size_t a, size_t b, size_t keep) {
     std::vector<double> w(slots * b, 0.0);
@@ -1913,9 +2160,42 @@ extern "C" int64_t uk_ingest_count(const uk_ingest* g) { return g ? g->count : 0
 extern "C" int64_t uk_ingest_first(const uk_ingest* g) { return g ? g->step0 : 0; }
 extern "C" int32_t uk_ingest_dimension(const uk_ingest* g) { return g ? g->n : -1; }
 
+/* general mode: the step's equations as deep copies (checked by the packing at smoothing time) */
+static int ingest_evolve_general(uk_ingest* g, int32_t n, const kalman_matrix_t* Hm, const kalman_matrix_t* Fm,
+                                 const kalman_matrix_t* cm, const kalman_matrix_t* Km, char K_type) {
+  if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("kalman_evolve: state dimension %d outside 1..%d", n, UKHIP_MAX_N); return -1; }
+  if (g->count > 0 && (!Hm || !Fm || !cm || !Km)) {
+    uk_set_error("kalman_evolve: H, F, c and K are required after the first step");
+    return -1;
+  }
+  if (g->current) { /* a second evolve replaces the pending step */
+    uk_ingest::eq_free(g->eqs.back());
+    g->eqs.pop_back();
+  }
+  auto* e = (kalman_step_equations_t*)std::calloc(1, sizeof(kalman_step_equations_t));
+  e->step = (kalman_step_index_t)(g->step0 + g->count);
+  e->dimension = n;
+  e->covariance_type = 'C';
+  if (g->count > 0) {
+    e->H = matrix_create_copy((kalman_matrix_t*)Hm);
+    e->F = matrix_create_copy((kalman_matrix_t*)Fm);
+    e->c = matrix_create_copy((kalman_matrix_t*)cm);
+    e->K = matrix_create_copy((kalman_matrix_t*)Km);
+    e->K_type = K_type;
+  }
+  g->eqs.push_back(e);
+  g->current = true;
+  return 0;
+}
+
 extern "C" int uk_ingest_evolve(uk_ingest* g, int32_t n, const kalman_matrix_t* Hm, const kalman_matrix_t* Fm,
                                 const kalman_matrix_t* cm, const kalman_matrix_t* Km, char K_type) {
   if (!g) return -1;
+  if (!g->general && g->n >= 0 &&
+      (n != g->n || (g->count > 0 && Hm && Fm && cm &&
+                     (Hm->row_dim != n || Hm->col_dim != n || Fm->row_dim != n || Fm->col_dim != n || cm->row_dim != n))))
+    g->to_general(); /* another dimension or a rectangular evolution equation */
+  if (g->general) return ingest_evolve_general(g, n, Hm, Fm, cm, Km, K_type);
   if (g->n < 0) {
     if (n < 1 || n > UKHIP_MAX_N) { uk_set_error("kalman_evolve: state dimension %d outside 1..%d", n, UKHIP_MAX_N); return -1; }
     g->n = n;
@@ -1958,6 +2238,20 @@ extern "C" int uk_ingest_evolve(uk_ingest* g, int32_t n, const kalman_matrix_t* 
 extern "C" int uk_ingest_observe(uk_ingest* g, const kalman_matrix_t* Gm, const kalman_matrix_t* om,
                                  const kalman_matrix_t* Cm, char C_type) {
   if (!g || !g->current) { uk_set_error("kalman_observe: call kalman_evolve first"); return -1; }
+  if (!g->general && om && om->row_dim > g->n) g->to_general(); /* more observation rows than states */
+  if (g->general) {
+    kalman_step_equations_t* e = g->eqs.back();
+    if (om) {
+      if (!Gm || !Cm) { uk_set_error("kalman_observe: G and C are required with o"); return -1; }
+      e->G = matrix_create_copy((kalman_matrix_t*)Gm);
+      e->o = matrix_create_copy((kalman_matrix_t*)om);
+      e->C = matrix_create_copy((kalman_matrix_t*)Cm);
+      e->C_type = C_type;
+    }
+    g->current = false;
+    g->count++;
+    return 0;
+  }
   const int n = g->n;
   const int64_t j = g->start + g->count;
   if (om) {
@@ -2008,6 +2302,15 @@ extern "C" int uk_ingest_smooth(uk_ingest* g, uint32_t options) {
   if (!g || g->count == 0) return 0;
   if (!ensure_stream()) return -1;
   const bool assoc = (options & KALMAN_ALGORITHM_ASSOCIATIVE) != 0;
+  if (g->general) {
+    std::vector<kalman_step_equations_t*> live(g->eqs.begin(), g->eqs.begin() + g->count);
+    for (auto* e : live) { /* earlier results are replaced */
+      matrix_free(e->state);
+      matrix_free(e->covariance);
+      e->state = e->covariance = nullptr;
+    }
+    return smooth_equations((kalman_options_t)options, live.data(), (kalman_step_index_t)g->count, assoc) ? 0 : -1;
+  }
   const int n = g->n;
   const size_t nn = g->nn(), S = (size_t)g->count, j0 = (size_t)g->start;
   Staging st;
@@ -2058,18 +2361,34 @@ static int64_t ingest_slot(const uk_ingest* g, int64_t si) {
     uk_set_error("step %lld is not in memory", (long long)si);
     return -1;
   }
-  return g->start + (si - g->step0);
+  return (g->general ? 0 : g->start) + (si - g->step0);
 }
 
 extern "C" const double* uk_ingest_state(const uk_ingest* g, int64_t si) {
   const int64_t j = ingest_slot(g, si);
+  if (j >= 0 && g->general) {
+    const kalman_step_equations_t* e = g->eqs[j];
+    return e->state ? e->state->elements : nullptr;
+  }
   return (j >= 0 && g->hasx[j]) ? &g->x[j * g->n] : nullptr;
 }
 
 extern "C" const double* uk_ingest_covariance(const uk_ingest* g, int64_t si, char* type) {
   const int64_t j = ingest_slot(g, si);
+  if (j >= 0 && g->general) {
+    const kalman_step_equations_t* e = g->eqs[j];
+    if (type) *type = e->covariance ? e->covariance_type : 'C';
+    return e->covariance ? e->covariance->elements : nullptr;
+  }
   if (type) *type = g ? g->Stype : 'C';
   return (j >= 0 && g->hasS[j]) ? &g->S[j * g->nn()] : nullptr;
+}
+
+/* state dimension of step si (-1: not in memory) */
+extern "C" int32_t uk_ingest_step_dimension(const uk_ingest* g, int64_t si) {
+  const int64_t j = ingest_slot(g, si);
+  if (j < 0) return -1;
+  return g->general ? g->eqs[j]->dimension : g->n;
 }
 
 extern "C" int uk_ingest_valid(const uk_ingest* g, int64_t si) { return ingest_slot(g, si) >= 0; }
@@ -2082,7 +2401,14 @@ extern "C" void uk_ingest_forget(uk_ingest* g, int64_t si) {
   if (si > latest - 1) return;
   const int64_t d = si - g->step0 + 1;
   if (d <= 0) return;
-  g->start += d;
+  if (g->general) {
+    for (int64_t q = 0; q < d; ++q) {
+      uk_ingest::eq_free(g->eqs.front());
+      g->eqs.pop_front();
+    }
+  } else {
+    g->start += d;
+  }
   g->count -= d;
   g->step0 += d;
 }
@@ -2093,6 +2419,15 @@ extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
   if (!g || g->count == 0) return;
   if (si > g->step0 + g->count - 1 || si < g->step0) return;
   g->count = si - g->step0;
+  if (g->general) {
+    while ((int64_t)g->eqs.size() > g->count + 1) {
+      uk_ingest::eq_free(g->eqs.back());
+      g->eqs.pop_back();
+    }
+    uk_ingest::eq_clear_observation(g->eqs.back());
+    g->current = true;
+    return;
+  }
   const int64_t j = g->start + g->count;
   g->m[j] = 0;
   g->hasx[j] = g->hasS[j] = 0;

@@ -173,19 +173,26 @@ def problem_struct(p, keep):
         s.F_shared = 1
     else:
         s.F = keep.ptr(pk["F"])
-    s.H = None
+    s.H = keep.ptr(pk.get("H"))
     s.c = keep.ptr(pk["c"])
-    K = pk["K"]
     s.K_type = pk["K_type"].encode()
-    s.K = keep.ptr(np.broadcast_to(np.asfortranarray(K).T.ravel(), (k, K.size)))
+    if "K_steps" in pk:  # one K per step (packed_from_lists)
+        s.K = keep.ptr(pk["K_steps"])
+    else:
+        K = pk["K"]
+        s.K = keep.ptr(np.broadcast_to(np.asfortranarray(K).T.ravel(), (k, K.size)))
     s.G = keep.ptr(pk["G"])
     s.o = keep.ptr(pk["o"])
-    mi = np.where(pk["observed"], m, 0).astype(np.int32)
+    mi = pk["m_i"] if "m_i" in pk else np.where(pk["observed"], m, 0).astype(np.int32)
+    mi = np.ascontiguousarray(mi, dtype=np.int32)
     keep.arrays.append(mi)
     s.m_i = mi.ctypes.data
-    Cm = pk["C"]
     s.C_type = pk["C_type"].encode()
-    s.C = keep.ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
+    if "C_steps" in pk:
+        s.C = keep.ptr(pk["C_steps"])
+    else:
+        Cm = pk["C"]
+        s.C = keep.ptr(np.broadcast_to(np.asfortranarray(Cm).T.ravel(), (k, Cm.size)))
     return s
 
 

@@ -371,3 +371,120 @@ def problem_from_arrays(d, name=""):
     kt = [None if x == ord("-") else chr(x) for x in t[:, 0]]
     ct = [None if x == ord("-") else chr(x) for x in t[:, 1]]
     return Problem(k=k, n=n, K_type=kt, C_type=ct, name=name, **cols)
+
+
+def shape_problem(seed, rows, *, K_kind="W", C_kind="W"):
+    """A problem with per-step shapes, as matlab/examples/compare.m:30-71 builds
+    them from a kalmanTest.m row table: step i has state dimension n_i, an
+    evolution equation of l_i rows (H_i = eye(l_i, n_i), F_i the first l_i x
+    n_{i-1} block of a random orthogonal matrix, c_i ~ N(0,1)) and m_i
+    observation rows (G_i, o_i ~ N(0,1)); `rows` lists (n_i, l_i, m_i).
+    Covariances are random SPD matrices in representation K_kind / C_kind."""
+    rng = rng_for(seed)
+    k = len(rows)
+    H, F, c, K, Kt, G, o, Cc, Ct = ([None] * k for _ in range(9))
+
+    def cov_of(kind, dim):
+        a = rng.standard_normal((dim, dim)) * 0.3 + np.eye(dim) * 1.5
+        cov = a @ a.T
+        if kind == "C":
+            return cov, "C"
+        if kind == "W":
+            return np.linalg.inv(np.linalg.cholesky(cov)), "W"
+        if kind == "w":
+            return rng.uniform(0.5, 2.0, size=(dim, 1)), "w"
+        return np.linalg.cholesky(cov), kind
+
+    for i, (n, l, m) in enumerate(rows):
+        if i > 0:
+            npv = rows[i - 1][0]
+            d = max(l, npv)
+            q, r = np.linalg.qr(rng.standard_normal((d, d)))
+            H[i] = np.eye(l, n)
+            F[i] = np.asfortranarray(q[:l, :npv])
+            c[i] = rng.standard_normal((l, 1))
+            K[i], Kt[i] = cov_of(K_kind, l)
+        if m > 0:
+            G[i] = rng.standard_normal((m, n))
+            o[i] = rng.standard_normal((m, 1))
+            Cc[i], Ct[i] = cov_of(C_kind, m)
+    return Problem(k=k, n=[r[0] for r in rows], H=H, F=F, c=c, K=K, K_type=Kt, G=G, o=o, C=Cc, C_type=Ct,
+                   name=f"shape_s{seed}_k{k}")
+
+
+def shape_catalogue():
+    """Row tables (n_i, l_i, m_i) of matlab/examples/kalmanTest.m:80-95, plus
+    power-of-two-length variants (the reference's odd-even smoother runs only
+    on power-of-two lengths, SURVEY defect D1) and larger random tables with
+    rectangular H, changing dimensions and more observations than states."""
+    def uniform(n, l, m, k):
+        return [(n, l, m)] * k
+    cat = {
+        "u1117": uniform(1, 1, 1, 7), "u2221": uniform(2, 2, 2, 1), "u2227": uniform(2, 2, 2, 7),
+        "u3331": uniform(3, 3, 3, 1), "u3334": uniform(3, 3, 3, 4), "u3335": uniform(3, 3, 3, 5),
+        "u3313": uniform(3, 3, 1, 3), "u4422": uniform(4, 4, 2, 2),
+        "u2129": uniform(2, 1, 2, 9), "u2128": uniform(2, 1, 2, 8),
+        "u3239": uniform(3, 2, 3, 9), "u3238": uniform(3, 2, 3, 8),
+        "u3139": uniform(3, 1, 3, 9), "u31316": uniform(3, 1, 3, 16),
+        "mid3obs": [(3, 0, 2), (3, 1, 2), (3, 1, 3), (3, 1, 2)],
+        "grow2": [(1, 1, 0), (1, 1, 2)],
+        "grow7": [(1, 1, 0), (1, 1, 1), (2, 1, 2), (3, 2, 3), (3, 3, 0), (3, 2, 3), (4, 2, 3)],
+        "grow8": [(1, 1, 0), (1, 1, 1), (2, 1, 2), (3, 2, 3), (3, 3, 0), (3, 2, 3), (4, 2, 3), (4, 3, 4)],
+        "wide8": uniform(2, 2, 4, 8), "wide16": uniform(3, 3, 5, 16),
+    }
+    rng = rng_for(777)
+    for t in range(4):
+        # larger tables: n_i in 4..12, l_i in [n_i - 3, n_i], m_i in [n_i - l_i, n_i + 3] (step 0: l_0 = 0), so
+        # that [H_i; G_i] has full column rank and every level-0 block the elimination inverts is nonsingular
+        k = (32, 64, 64, 128)[t]
+        rows = []
+        for i in range(k):
+            n = int(rng.integers(4, 13))
+            l = int(rng.integers(max(1, n - 3), n + 1))
+            m = int(rng.integers(n if i == 0 else n - l, n + 4))
+            rows.append((n, l, m))
+        cat[f"rand{t}_k{k}"] = rows
+    return cat
+
+
+def packed_from_lists(pr):
+    """The packed (batch API) form of a list-form Problem with one state
+    dimension and one covariance representation per kind: per-step F, H, c,
+    K, G, o, C arrays (column-major per step), observation rows m = max m_i
+    with the per-step counts in m_i. Returns pr with `packed` set."""
+    n, k = pr.n[0], pr.k
+    assert all(x == n for x in pr.n)
+    ms = [0 if o is None else o.shape[0] for o in pr.o]
+    m = max(ms)
+    kt = next((t for t in pr.K_type if t), "w")
+    ct = next((t for t in pr.C_type if t), "w")
+    assert all(t in (None, kt) for t in pr.K_type) and all(t in (None, ct) for t in pr.C_type)
+    F = np.zeros((k, n, n))
+    H = np.zeros((k, n, n))
+    c = np.zeros((k, n))
+    K = np.zeros((k, n)) if kt == "w" else np.zeros((k, n, n))
+    G = np.zeros((k, n, max(m, 1)))
+    o = np.zeros((k, max(m, 1)))
+    C = np.zeros((k, max(m, 1))) if ct == "w" else np.zeros((k, max(m, 1), max(m, 1)))
+    H_ident = True
+    for i in range(k):
+        if i > 0:
+            F[i] = np.asarray(pr.F[i]).T
+            H[i] = np.asarray(pr.H[i]).T
+            H_ident &= bool(np.array_equal(pr.H[i], np.eye(n)))
+            c[i] = np.ravel(pr.c[i])
+            K[i] = np.ravel(pr.K[i]) if kt == "w" else np.asarray(pr.K[i]).T
+        else:
+            H[i] = np.eye(n)
+            K[i] = 1.0 if kt == "w" else np.eye(n)
+        mi = ms[i]
+        if mi:
+            G[i, :, :mi] = np.asarray(pr.G[i]).T
+            o[i, :mi] = np.ravel(pr.o[i])
+            if ct == "w":
+                C[i, :mi] = np.ravel(pr.C[i])
+            else:
+                C[i, :mi, :mi] = np.asarray(pr.C[i]).T
+    pr.packed = dict(n=n, m=m, F=F, H=None if H_ident else H, c=c, o=o, G=G, observed=np.array(ms) > 0,
+                     m_i=np.array(ms, dtype=np.int32), K_steps=K, K_type=kt, C_steps=C, C_type=ct)
+    return pr
