@@ -91,6 +91,21 @@ int ukhip_synchronize(void);
  * the error by about cond^2 * 6e-8). Estimates only: the covariances of the
  * batch are left as they were. Enqueued on the library stream. */
 int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements);
+/* The same with options. fp32_levels: elimination levels 0 .. fp32_levels-1
+ * run in FP32 and the levels above in FP64, where the reduced blocks'
+ * conditioning compounds and few steps remain (<= 0: every level in FP32;
+ * level 0 is always FP32). covariances = 1: the covariances are formed from
+ * the same block factors by the FP64 selected-inversion back substitution
+ * (FP32 factors widened), into the batch's covariance output; they carry the
+ * FP32 factors' accuracy, not the refined estimates'. The first call per
+ * option set captures the launch sequence (every part of a composite on its
+ * own forked stream) into a HIP graph, later calls replay it. */
+typedef struct {
+  int32_t refinements; /* FP64 refinement steps, 0..64 */
+  int32_t fp32_levels;
+  int32_t covariances;
+} ukhip_mixed_options_t;
+int ukhip_batch_smooth_oddeven_mixed_opt(ukhip_batch_t* b, const ukhip_mixed_options_t* options);
 
 /* Copies problem p's estimates (k*n) and covariances (k*n*n, may be NULL) to the host. */
 int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covariances);

@@ -105,3 +105,57 @@ def test_few_observations_stay_finite(dev):
     say("few observations per step")
     errs = _errors(dev, probs, (0, 2))
     assert np.isfinite(errs[0]) and errs[1] < TOL, errs
+
+
+def test_precision_switch_and_covariances(dev, oracle):
+    """The precision workload's dynamics (n = 64, m = 32, F = Q
+    diag(logspace(0,-3,64)), K = 1e-2 I) at k = 515 against the oracle:
+    levels above fp32_levels eliminated in FP64, estimates after refinement
+    and the covariance diagonals formed from the same factors."""
+    F = P.random_orthogonal(P.rng_for(1004), 64) @ np.diag(np.logspace(0, -3, 64))
+    pr = P.iid_problem(515, 64, 32, seed=14, F=F, G_scale=1 / 8.0, Kvar=1e-2, Cvar=1.0, name="n64_illcond_k515")
+    oracle.lib.oracle_set_threads(min(16, os.cpu_count() or 1))
+    try:
+        ox, oc = oracle.oddeven(pr)
+    finally:
+        oracle.lib.oracle_set_threads(1)
+    od = [np.diag(c) for c in oc]
+    b = dev.Batch([pr])
+    try:
+        for fl in (0, 4, 3, 2):
+            for r in (1, 2):
+                b.smooth_mixed_opt(r, fl, covariances=True)
+                x, _ = b.download(0, want_cov=False)
+                d = b.download_cov(0, "D")
+                ex, ed = max_rel_err(list(x), ox), max_rel_err(list(d), od)
+                say(f"  fp32_levels={fl} refinements={r}: estimates {ex:.2e}, covariance diagonals {ed:.2e}")
+                if r == 2:
+                    assert ex < TOL
+                assert ed < 1e-2
+    finally:
+        b.free()
+
+
+def test_mixed_graph_replays_match_eager(dev):
+    """Mixed-precision smoothing of a batch of several state dimensions,
+    captured as one HIP graph (every part on its own forked stream) and
+    replayed: each replay, with eager work and synchronizations between them,
+    equals the eager launch sequence bit for bit."""
+    specs = P.batch_specs(count=12, seed=9, lmin=8, lmax=200)
+    probs = [P.batch_problem(s) for s in specs]
+    b = dev.Batch(probs)
+    L = dev.lib()
+    try:
+        L.ukhip_use_graphs(0)
+        b.smooth_mixed_opt(1, 3, covariances=True)
+        eager = [b.download(i) for i in range(len(probs))]
+        L.ukhip_use_graphs(1)
+        for rep in range(3):
+            b.smooth("oddeven")  # other eager work between the replays
+            b.smooth_mixed_opt(1, 3, covariances=True)
+            for i in range(len(probs)):
+                x, c = b.download(i)
+                assert np.array_equal(x, eager[i][0]) and np.array_equal(c, eager[i][1]), (rep, i)
+    finally:
+        L.ukhip_use_graphs(1)
+        b.free()

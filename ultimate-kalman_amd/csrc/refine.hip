@@ -57,6 +57,11 @@ __global__ void __launch_bounds__(256) k_to32(float* __restrict__ dst, const dou
   for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (float)src[i];
 }
 
+/* dst[i] = src[i] widened (FP32 chain blocks entering the FP64 top levels) */
+__global__ void __launch_bounds__(256) k_to64(double* __restrict__ dst, const float* __restrict__ src, int64_t count) {
+  for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (double)src[i];
+}
+
 /* x[i] += d[i] */
 __global__ void __launch_bounds__(256) k_add(double* __restrict__ x, const double* __restrict__ d, int64_t count) {
   for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) x[i] += d[i];
@@ -401,14 +406,22 @@ extern "C" hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first
   return hipGetLastError();
 }
 
+extern "C" hipError_t uk_launch_to64(double* dst, const float* src, int64_t count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 65536);
+  k_to64<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(dst, src, count);
+  return hipGetLastError();
+}
+
 /* forward sweep of one level: y (per even slot of the level) and the next
- * level's right-hand side */
-extern "C" hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* fac, const double* g, double* y,
-                                    double* gnext, int n, hipStream_t st) {
+ * level's right-hand side, reading the level's factors in precision T */
+template <class T>
+static hipError_t launch_fwd(const uk_level_t* lv, const T* R, const T* Bt, const T* Y, const double* g, double* y,
+                             double* gnext, int n, hipStream_t st) {
   const int nb = uk_bucket(n);
-#define L_(NBV)                                                                                          \
-  k_fwd_even<NBV, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, fac->R, g, y, n);          \
-  k_fwd_odd<NBV, float><<<dim3((unsigned)lv->units), dim3(64), 0, st>>>(*lv, fac->Bt, fac->Y, g, y, gnext, n)
+#define L_(NBV)                                                                                   \
+  k_fwd_even<NBV, T><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, R, g, y, n);            \
+  k_fwd_odd<NBV, T><<<dim3((unsigned)lv->units), dim3(64), 0, st>>>(*lv, Bt, Y, g, y, gnext, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
@@ -416,20 +429,37 @@ extern "C" hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* 
 #undef L_
   return hipGetLastError();
 }
+extern "C" hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* fac, const double* g, double* y,
+                                    double* gnext, int n, hipStream_t st) {
+  return launch_fwd<float>(lv, fac->R, fac->Bt, fac->Y, g, y, gnext, n, st);
+}
+extern "C" hipError_t uk_launch_fwd64(const uk_level_t* lv, const uk_factors_t* fac, const double* g, double* y,
+                                      double* gnext, int n, hipStream_t st) {
+  return launch_fwd<double>(lv, fac->R, fac->Bt, fac->Y, g, y, gnext, n, st);
+}
 
 /* backward sweep of one level; rhs32 (the FP32 transformed right-hand side)
- * or rhs64 (the forward sweep's y) */
-extern "C" hipError_t uk_launch_solve_est(const uk_level_t* lv, const uk_factors32_t* fac, const float* rhs32,
-                                          const double* rhs64, double* x, int n, hipStream_t st) {
+ * or rhs64 (the forward sweep's y, or an FP64 level's transformed right-hand
+ * side) */
+template <class T>
+static hipError_t launch_solve_est(const uk_level_t* lv, const FactorPtrs<T>& f, const float* rhs32,
+                                   const double* rhs64, double* x, int n, hipStream_t st) {
   const int nb = uk_bucket(n);
-  const FactorPtrs<float> f{fac->R, fac->Bt, fac->Y};
-#define L_(NBV)                                                                                            \
-  if (rhs32) k_solve_est<NBV, float, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
-  else k_solve_est<NBV, float, double><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
+#define L_(NBV)                                                                                                \
+  if (rhs32) k_solve_est<NBV, T, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
+  else k_solve_est<NBV, T, double><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }
 #undef L_
   return hipGetLastError();
+}
+extern "C" hipError_t uk_launch_solve_est(const uk_level_t* lv, const uk_factors32_t* fac, const float* rhs32,
+                                          const double* rhs64, double* x, int n, hipStream_t st) {
+  return launch_solve_est<float>(lv, FactorPtrs<float>{fac->R, fac->Bt, fac->Y}, rhs32, rhs64, x, n, st);
+}
+extern "C" hipError_t uk_launch_solve_est64(const uk_level_t* lv, const uk_factors_t* fac, const double* rhs64,
+                                            double* x, int n, hipStream_t st) {
+  return launch_solve_est<double>(lv, FactorPtrs<double>{fac->R, fac->Bt, fac->Y}, nullptr, rhs64, x, n, st);
 }

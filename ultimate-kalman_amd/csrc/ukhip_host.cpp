@@ -54,6 +54,11 @@ hipError_t uk_launch_fwd(const uk_level_t* lv, const uk_factors32_t* fac, const 
                          int n, hipStream_t st);
 hipError_t uk_launch_solve_est(const uk_level_t* lv, const uk_factors32_t* fac, const float* rhs32,
                                const double* rhs64, double* x, int n, hipStream_t st);
+hipError_t uk_launch_to64(double* dst, const float* src, int64_t count, hipStream_t st);
+hipError_t uk_launch_fwd64(const uk_level_t* lv, const uk_factors_t* fac, const double* g, double* y, double* gnext,
+                           int n, hipStream_t st);
+hipError_t uk_launch_solve_est64(const uk_level_t* lv, const uk_factors_t* fac, const double* rhs64, double* x, int n,
+                                 hipStream_t st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -219,6 +224,12 @@ struct ukhip_batch {
   uk_factors32_t fac32{};
   float* fix32 = nullptr;
   double *gA = nullptr, *gB = nullptr, *gbuf[2]{}, *ys = nullptr, *dx = nullptr;
+  /* FP64 chains of the levels the mixed path eliminates in FP64: top[1] is
+   * chain[1] (level-1 size), top[0] holds at most level 2's steps; chain[0]
+   * keeps the FP64 level-0 chain for the refinement residuals */
+  uk_chain_t top0{};
+  /* captured mixed-precision smoothings, by (refinements, FP32 levels, covariances) */
+  std::map<int64_t, hipGraphExec_t> mixed_graph;
   /* memory source (NULL: hipMalloc) and the host copy of the level tables
    * (kept until an asynchronous upload has completed) */
   DevArena* arena = nullptr;
@@ -418,6 +429,8 @@ static void batch_release(ukhip_batch* b) {
   for (auto st : b->part_streams) (void)hipStreamDestroy(st);
   for (auto g : b->graph)
     if (g) (void)hipGraphExecDestroy(g);
+  for (auto& kv : b->mixed_graph)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
   dev_free(b, b->d_raw);
   dev_free(b, b->d_work);
   dev_free(b, b->d_tab);
@@ -934,17 +947,15 @@ static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
 
 /* The level-0 chain's D = V H is diagonal when every H is the identity and
  * every K is given as 'w' (V diagonal). When C is 'w' too, k_weigh_diag writes
- * the chain and stores D as its diagonal; the phase kernels and k_normal read
- * it so, and phase 2 skips D's exact zeros. */
-static bool level0_d_diagonal(const ukhip_batch* b) {
-  return b->raw.H_ident && !b->raw.K_full && !b->raw.C_full && !b->compress;
-}
-
-/* With every K and C given as 'w', the level-0 chain's B_s = -diag(k_s) F is
- * not stored: its readers form it from the raw K and F. */
-template <class Chain>
-static Chain level0_chain(const ukhip_batch* b, Chain c) {
-  c.d_diag = level0_d_diagonal(b);
+ * the chain and stores D as its diagonal (d_diag); the phase kernels and
+ * k_normal read it so, and phase 2 skips D's exact zeros. With every K and C
+ * given as 'w', the level-0 chain's B_s = -diag(k_s) F is not stored either
+ * (b_synth): its readers form it from the raw K and F. (B: a batch or a
+ * uk_oe_view_t, both carrying raw, nF, compress and mr0.) */
+template <class B, class Chain>
+static Chain level0_chain(const B* b, Chain c) {
+  const bool d_diag = b->raw.H_ident && !b->raw.K_full && !b->raw.C_full && !b->compress;
+  c.d_diag = d_diag;
   c.b_synth = !b->raw.K_full && !b->raw.C_full && !b->compress; /* compressed chains come from k_weigh */
   c.mr = b->mr0;
   c.c_tri = b->compress; /* k_obs_compress leaves every C upper triangular */
@@ -957,58 +968,96 @@ static Chain level0_chain(const ukhip_batch* b, Chain c) {
 /* The whitened level-0 chain (c/kalman_oddeven_smoother.c:1620-1659): k_weigh
  * or the element-parallel k_weigh_diag; with observation blocks taller than n,
  * k_weigh followed by k_obs_compress. */
-static bool weigh_level0(ukhip_batch* b, hipStream_t st) {
+template <class B>
+static bool weigh_level0(B* b, hipStream_t st) {
   if (b->compress) return hip_ok(uk_launch_weigh_obs(&b->raw, &b->chain[0], b->wobs, st), "k_weigh launch");
   const uk_chain_t w0 = level0_chain(b, b->chain[0]);
   return hip_ok(uk_launch_weigh(&b->raw, &w0, st), "k_weigh launch");
 }
 
-static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
-                        hipStream_t st) {
-  const int nl = (int)b->levels.size();
+/* One whole odd-even smoothing (c/kalman_oddeven_smoother.c:1358-1553,
+ * 614-975) over the buffers of v, enqueued on st. */
+static bool run_oddeven_view(uk_oe_view_t* v, int cov, hipEvent_t* ev /* timed_events(nl) events or NULL */,
+                             hipStream_t st) {
+  const int nl = v->nl, n = v->n;
+  const size_t nn = (size_t)n * n;
   if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
-  if (!weigh_level0(b, st)) return false;
+  if (!weigh_level0(v, st)) return false;
   if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
-  for (int l = 0; l < nl; ++l) {
-    uk_chain_t in = l == 0 ? level0_chain(b, b->chain[0]) : b->chain[l & 1];
-    if (l > 0) {
-      in.mr = b->n;
-      in.c_tri = 1;
-    }
-    uk_chain_t out = b->chain[(l + 1) & 1]; /* its flags stay 0: a dense chain */
-    out.mr = b->n;
-    uk_factors_t f = b->fac;
-    const int64_t fb = b->levels[l].fac_base;
-    const size_t nn = (size_t)b->n * b->n;
+  auto fac_at = [&](int l) {
+    uk_factors_t f = v->fac;
+    const int64_t fb = v->fac_base[l];
     f.R += fb * nn;
     f.Bt += fb * nn;
     f.Y += fb * nn;
-    f.cf += fb * b->n;
-    uk_level_t lv = b->levels[l].dev; /* even_start is relative to the level's fac_base */
-    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, b->n, st, ev ? ev + 2 + 4 * l : nullptr),
+    f.cf += fb * n;
+    return f;
+  };
+  for (int l = 0; l < nl; ++l) {
+    uk_chain_t in = l == 0 ? level0_chain(v, v->chain[0]) : v->chain[l & 1];
+    if (l > 0) {
+      in.mr = n;
+      in.c_tri = 1;
+    }
+    uk_chain_t out = v->chain[(l + 1) & 1]; /* its flags stay 0: a dense chain */
+    out.mr = n;
+    const uk_factors_t f = fac_at(l);
+    uk_level_t lv = v->levels[l]; /* even_start is relative to the level's fac_base */
+    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, v->fix, n, st, ev ? ev + 2 + 4 * l : nullptr),
                 "elimination launch"))
       return false;
   }
   for (int l = nl - 1; l >= 0; --l) {
-    uk_factors_t f = b->fac;
-    const int64_t fb = b->levels[l].fac_base;
-    const size_t nn = (size_t)b->n * b->n;
-    f.R += fb * nn;
-    f.Bt += fb * nn;
-    f.Y += fb * nn;
-    f.cf += fb * b->n;
-    uk_level_t lv = b->levels[l].dev;
+    const uk_factors_t f = fac_at(l);
+    uk_level_t lv = v->levels[l];
     hipEvent_t* eb = ev ? ev + 2 + 4 * nl + 2 * l : nullptr;
     if (eb && !hip_ok(hipEventRecord(eb[0], st), "hipEventRecord")) return false;
-    double* Xc = l == 0 ? nullptr : b->X[l & 1];
-    double* XTc = l == 0 ? nullptr : b->XT[l & 1];
-    if (!hip_ok(uk_launch_backsub(&lv, &f, b->xout, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], Xc, XTc, b->n,
-                                  cov, st),
+    double* Xc = l == 0 ? nullptr : v->X[l & 1];
+    double* XTc = l == 0 ? nullptr : v->XT[l & 1];
+    if (!hip_ok(uk_launch_backsub(&lv, &f, v->xout, v->Sout, v->X[(l + 1) & 1], v->XT[(l + 1) & 1], Xc, XTc, n, cov,
+                                  st),
                 "k_backsub launch"))
       return false;
     if (eb && !hip_ok(hipEventRecord(eb[1], st), "hipEventRecord")) return false;
   }
   return true;
+}
+
+/* for the allocation-free entry point (noalloc.hip) */
+extern "C" int uk_run_oddeven_view(uk_oe_view_t* v, int cov, hipStream_t st) {
+  return run_oddeven_view(v, cov, nullptr, st) ? 0 : -1;
+}
+
+static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
+                        hipStream_t st) {
+  const int nl = (int)b->levels.size();
+  std::vector<uk_level_t> lv(nl);
+  std::vector<int64_t> fb(nl);
+  for (int l = 0; l < nl; ++l) {
+    lv[l] = b->levels[l].dev;
+    fb[l] = b->levels[l].fac_base;
+  }
+  uk_oe_view_t v{};
+  v.raw = b->raw;
+  v.chain[0] = b->chain[0];
+  v.chain[1] = b->chain[1];
+  v.fac = b->fac;
+  v.X[0] = b->X[0];
+  v.X[1] = b->X[1];
+  v.XT[0] = b->XT[0];
+  v.XT[1] = b->XT[1];
+  v.xout = b->xout;
+  v.Sout = b->Sout;
+  v.fix = b->fix;
+  v.wobs = b->wobs;
+  v.n = b->n;
+  v.mr0 = b->mr0;
+  v.compress = b->compress ? 1 : 0;
+  v.nl = nl;
+  v.nF = b->nF;
+  v.levels = lv.data();
+  v.fac_base = fb.data();
+  return run_oddeven_view(&v, cov, ev, st);
 }
 
 static ukhip_batch* batch_create_one(int32_t nprob, const ukhip_problem_t* problems) {
@@ -1232,6 +1281,9 @@ static bool mixed_workspace(ukhip_batch* b) {
   const size_t ofx = take((size_t)b->nprob * (nn + n) * 4);
   const size_t ogA = take(S * n * 8), ogB = take(S * n * 8), og0 = take(S * n * 8),
                og1 = take(std::max<size_t>(L1, 1) * n * 8), oys = take(E * n * 8), odx = take(S * n * 8);
+  const size_t L2 = b->levels.size() > 2 ? (size_t)b->levels[2].steps : 1;
+  const size_t tB = take(L2 * nn * 8), tD = take(L2 * nn * 8), tc = take(L2 * n * 8), tC = take(L2 * nn * 8),
+               to = take(L2 * n * 8);
   b->d_mixed = dev_alloc(b, o, "device memory for the mixed-precision workspace");
   if (!b->d_mixed) return false;
   char* w = (char*)b->d_mixed;
@@ -1253,6 +1305,11 @@ static bool mixed_workspace(ukhip_batch* b) {
   b->gbuf[1] = (double*)(w + og1);
   b->ys = (double*)(w + oys);
   b->dx = (double*)(w + odx);
+  b->top0.B = (double*)(w + tB);
+  b->top0.D = (double*)(w + tD);
+  b->top0.c = (double*)(w + tc);
+  b->top0.C = (double*)(w + tC);
+  b->top0.o = (double*)(w + to);
   return true;
 }
 
@@ -1267,10 +1324,28 @@ static uk_factors32_t fac32_at(const ukhip_batch* b, int l) {
   return f;
 }
 
-/* Estimates only, in the original step order of xout. */
-static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
+static uk_factors_t fac64_at(const ukhip_batch* b, int l) {
+  uk_factors_t f = b->fac;
+  const int64_t fb = b->levels[l].fac_base;
+  const size_t nn = (size_t)b->n * b->n;
+  f.R += fb * nn;
+  f.Bt += fb * nn;
+  f.Y += fb * nn;
+  f.cf += fb * b->n;
+  return f;
+}
+
+/* Levels 0 .. fp32_levels - 1 of the elimination run in FP32, the levels
+ * above in FP64 (where the reduced blocks' conditioning compounds); the
+ * estimates, in the original step order of xout, come from the FP32/FP64
+ * transformed right-hand sides and `refinements` FP64 refinement steps over
+ * the factors of both precisions. With cov, the covariances are then formed
+ * from the same factors by the FP64 selected-inversion back substitution
+ * (the FP32 factors widened into the FP64 factor slots), into Sout. */
+static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov, hipStream_t st) {
   if (!mixed_workspace(b)) return false;
   const int n = b->n, nl = (int)b->levels.size();
+  const int lsw = std::max(1, std::min(fp32_levels, nl)); /* first FP64 level (level 0 stays FP32) */
   const size_t nn = (size_t)n * n, S = (size_t)b->steps;
   /* the level-0 chain in FP32 (the FP64 chain stays for the residuals):
    * written beside the FP64 one when the whitening is diagonal, else converted */
@@ -1287,23 +1362,50 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
       !hip_ok(uk_launch_to32(c32.C, c64.C, S * nn, st), "to32") ||
       !hip_ok(uk_launch_to32(c32.o, c64.o, S * n, st), "to32"))
     return false;
+  uk_chain_t top[2] = {b->top0, b->chain[1]};
   for (int l = 0; l < nl; ++l) {
-    uk_chain32_t in = l == 0 ? level0_chain(b, b->chain32[0]) : b->chain32[l & 1];
-    if (l > 0) {
+    uk_level_t lv = b->levels[l].dev;
+    if (l < lsw) {
+      uk_chain32_t in = l == 0 ? level0_chain(b, b->chain32[0]) : b->chain32[l & 1];
+      if (l > 0) {
+        in.mr = n;
+        in.c_tri = 1;
+      }
+      uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
+      out.mr = n;
+      const uk_factors32_t f = fac32_at(b, l);
+      if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;
+      if (l + 1 == lsw && lsw < nl) { /* the next level's chain enters FP64 */
+        const size_t ns = (size_t)b->levels[lsw].steps;
+        const uk_chain_t& t = top[lsw & 1];
+        if (!hip_ok(uk_launch_to64(t.B, out.B, ns * nn, st), "to64") ||
+            !hip_ok(uk_launch_to64(t.D, out.D, ns * nn, st), "to64") ||
+            !hip_ok(uk_launch_to64(t.c, out.c, ns * n, st), "to64") ||
+            !hip_ok(uk_launch_to64(t.C, out.C, ns * nn, st), "to64") ||
+            !hip_ok(uk_launch_to64(t.o, out.o, ns * n, st), "to64"))
+          return false;
+      }
+    } else {
+      uk_chain_t in = top[l & 1];
       in.mr = n;
       in.c_tri = 1;
+      uk_chain_t out = top[(l + 1) & 1];
+      out.mr = n;
+      const uk_factors_t f = fac64_at(b, l);
+      if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, n, st, nullptr), "FP64 elimination launch"))
+        return false;
     }
-    uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
-    out.mr = n;
-    const uk_factors32_t f = fac32_at(b, l);
-    uk_level_t lv = b->levels[l].dev;
-    if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;
   }
-  /* x0 from the FP32 transformed right-hand side */
+  /* x0 from the transformed right-hand sides */
   for (int l = nl - 1; l >= 0; --l) {
-    const uk_factors32_t f = fac32_at(b, l);
     uk_level_t lv = b->levels[l].dev;
-    if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
+    if (l < lsw) {
+      const uk_factors32_t f = fac32_at(b, l);
+      if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
+    } else {
+      const uk_factors_t f = fac64_at(b, l);
+      if (!hip_ok(uk_launch_solve_est64(&lv, &f, f.cf, b->xout, n, st), "solve launch")) return false;
+    }
   }
   const uk_chain_t r64 = level0_chain(b, b->chain[0]);
   for (int it = 0; it < refinements; ++it) {
@@ -1311,31 +1413,121 @@ static bool run_mixed(ukhip_batch* b, int refinements, hipStream_t st) {
                 "residual launch"))
       return false;
     for (int l = 0; l < nl; ++l) {
-      const uk_factors32_t f = fac32_at(b, l);
       uk_level_t lv = b->levels[l].dev;
       double* y = b->ys + b->levels[l].fac_base * n;
-      if (!hip_ok(uk_launch_fwd(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st), "forward sweep launch"))
-        return false;
+      hipError_t e;
+      if (l < lsw) {
+        const uk_factors32_t f = fac32_at(b, l);
+        e = uk_launch_fwd(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
+      } else {
+        const uk_factors_t f = fac64_at(b, l);
+        e = uk_launch_fwd64(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
+      }
+      if (!hip_ok(e, "forward sweep launch")) return false;
     }
     for (int l = nl - 1; l >= 0; --l) {
-      const uk_factors32_t f = fac32_at(b, l);
       uk_level_t lv = b->levels[l].dev;
       const double* y = b->ys + b->levels[l].fac_base * n;
-      if (!hip_ok(uk_launch_solve_est(&lv, &f, nullptr, y, b->dx, n, st), "backward sweep launch")) return false;
+      hipError_t e;
+      if (l < lsw) {
+        const uk_factors32_t f = fac32_at(b, l);
+        e = uk_launch_solve_est(&lv, &f, nullptr, y, b->dx, n, st);
+      } else {
+        const uk_factors_t f = fac64_at(b, l);
+        e = uk_launch_solve_est64(&lv, &f, y, b->dx, n, st);
+      }
+      if (!hip_ok(e, "backward sweep launch")) return false;
     }
     if (!hip_ok(uk_launch_add(b->xout, b->dx, (int64_t)(S * n), st), "update launch")) return false;
+  }
+  if (!cov) return true;
+  /* covariances: the FP32 levels' factors widened into their FP64 slots (the
+   * slots of levels 0 .. lsw-1 are contiguous), then the FP64 back
+   * substitution; its estimates go to the scratch dx, xout keeps the refined ones */
+  const size_t E32 = (size_t)(lsw < nl ? b->levels[lsw].fac_base : b->total_evens);
+  if (!hip_ok(uk_launch_to64(b->fac.R, b->fac32.R, E32 * nn, st), "to64") ||
+      !hip_ok(uk_launch_to64(b->fac.Bt, b->fac32.Bt, E32 * nn, st), "to64") ||
+      !hip_ok(uk_launch_to64(b->fac.Y, b->fac32.Y, E32 * nn, st), "to64") ||
+      !hip_ok(uk_launch_to64(b->fac.cf, b->fac32.cf, E32 * n, st), "to64"))
+    return false;
+  for (int l = nl - 1; l >= 0; --l) {
+    const uk_factors_t f = fac64_at(b, l);
+    uk_level_t lv = b->levels[l].dev;
+    double* Xc = l == 0 ? nullptr : b->X[l & 1];
+    double* XTc = l == 0 ? nullptr : b->XT[l & 1];
+    if (!hip_ok(uk_launch_backsub(&lv, &f, b->dx, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], Xc, XTc, n, 1, st),
+                "k_backsub launch"))
+      return false;
   }
   return true;
 }
 
-extern "C" int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements) {
-  if (!b) { uk_set_error("NULL batch"); return -1; }
+/* Enqueues a whole mixed-precision smoothing on the library stream (the parts
+ * of a composite on forked streams, as enqueue_oddeven). */
+static bool enqueue_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov) {
+  if (b->parts.empty()) return run_mixed(b, refinements, fp32_levels, cov, g_stream);
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins(b->parts.size(), nullptr);
+  bool ok = hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate") &&
+            hip_ok(hipEventRecord(fork, g_stream), "hipEventRecord");
+  for (size_t i = 0; ok && i < b->parts.size(); ++i) {
+    ok = hip_ok(hipStreamWaitEvent(b->part_streams[i], fork, 0), "hipStreamWaitEvent") &&
+         run_mixed(b->parts[i], refinements, fp32_levels, cov, b->part_streams[i]) &&
+         hip_ok(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming), "hipEventCreate") &&
+         hip_ok(hipEventRecord(joins[i], b->part_streams[i]), "hipEventRecord") &&
+         hip_ok(hipStreamWaitEvent(g_stream, joins[i], 0), "hipStreamWaitEvent");
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  for (auto e : joins)
+    if (e) (void)hipEventDestroy(e);
+  return ok;
+}
+
+/* Captures enqueue_mixed once per (refinements, FP32 levels, covariances),
+ * replays it afterwards. The workspaces are allocated before the capture. */
+static bool graph_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov) {
+  if (b->parts.empty()) {
+    if (!mixed_workspace(b)) return false;
+  } else {
+    for (auto* q : b->parts)
+      if (!mixed_workspace(q)) return false;
+  }
+  const int64_t key = ((int64_t)refinements << 20) | ((int64_t)(fp32_levels & 0xFFFF) << 1) | (cov ? 1 : 0);
+  hipGraphExec_t& ge = b->mixed_graph[key];
+  if (!ge) {
+    if (!hip_ok(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return false;
+    const bool ok = enqueue_mixed(b, refinements, fp32_levels, cov);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(g_stream, &g);
+    if (!ok || !hip_ok(e, "hipStreamEndCapture")) {
+      if (g) (void)hipGraphDestroy(g);
+      b->mixed_graph.erase(key);
+      return false;
+    }
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (!hip_ok(ei, "hipGraphInstantiate")) {
+      b->mixed_graph.erase(key);
+      return false;
+    }
+  }
+  return hip_ok(hipGraphLaunch(ge, g_stream), "hipGraphLaunch");
+}
+
+extern "C" int ukhip_batch_smooth_oddeven_mixed_opt(ukhip_batch_t* b, const ukhip_mixed_options_t* opt) {
+  if (!b || !opt) { uk_set_error("NULL batch or options"); return -1; }
   if (!ensure_stream()) return -1;
-  if (refinements < 0 || refinements > 64) { uk_set_error("refinements must be in 0..64"); return -1; }
-  if (b->parts.empty()) return run_mixed(b, refinements, g_stream) ? 0 : -1;
-  for (auto* q : b->parts)
-    if (!run_mixed(q, refinements, g_stream)) return -1;
-  return 0;
+  if (opt->refinements < 0 || opt->refinements > 64) { uk_set_error("refinements must be in 0..64"); return -1; }
+  const int fl = opt->fp32_levels <= 0 ? 64 : std::min(opt->fp32_levels, 64);
+  const bool cov = opt->covariances != 0;
+  if (g_use_graphs) return graph_mixed(b, opt->refinements, fl, cov) ? 0 : -1;
+  return enqueue_mixed(b, opt->refinements, fl, cov) ? 0 : -1;
+}
+
+extern "C" int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements) {
+  const ukhip_mixed_options_t o{refinements, 0, 0};
+  return ukhip_batch_smooth_oddeven_mixed_opt(b, &o);
 }
 
 extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states, double* covs) {
@@ -1535,7 +1727,7 @@ size_t batch_device_bytes(int n, int mr, int64_t steps, bool mixed) {
   const double nn = (double)n * n;
   double per = 8.0 * (14.0 * nn + 10.0 * n + 4.0 * mr + mr * (double)n + (double)mr * mr);
   if (mr > n) per += 8.0 * mr * (n + 1.0) + 4.0; /* whitened observations staged for k_obs_compress, m0 */
-  if (mixed) per += 4.0 * (7.5 * nn + 4 * n) + 8.0 * (6.0 * n);
+  if (mixed) per += 4.0 * (7.5 * nn + 4 * n) + 8.0 * (6.0 * n) + 8.0 * (0.8 * nn + 0.5 * n);
   return (size_t)(per * (double)steps * 1.05) + (size_t)(4u << 20);
 }
 
@@ -1646,7 +1838,7 @@ static bool server_admit(ukhip_server* s, std::unique_lock<std::mutex>& lk) {
     f->b = batch_from_staging(f->st, lengths, st, &s->arena, false);
     ok = f->b != nullptr;
   }
-  if (ok) ok = mixed ? run_mixed(f->b, s->opt.mixed_refinements, st) : run_oddeven(f->b, 1, nullptr, st);
+  if (ok) ok = mixed ? run_mixed(f->b, s->opt.mixed_refinements, 64, false, st) : run_oddeven(f->b, 1, nullptr, st);
   const size_t S = (size_t)f->st.steps, nn = (size_t)n * n;
   if (ok) ok = (f->host_x = (double*)s->pinned.get(S * n * 8)) != nullptr;
   if (ok && !mixed) ok = (f->host_S = (double*)s->pinned.get(S * nn * 8)) != nullptr;

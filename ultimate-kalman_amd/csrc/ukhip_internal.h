@@ -112,6 +112,29 @@ typedef struct {
   const int64_t* orig_start; /* indexed by batch problem */
 } uk_level_t;
 
+/* Everything one odd-even smoothing of a batch reads and writes on the
+ * device, as plain pointers: the raw inputs, the two chains, the factor slots
+ * (level l's at fac_base[l]), the cross-covariance buffers, the outputs, the
+ * per-problem fix-last scratch and, when observation blocks are reduced
+ * (compress), their staging. levels and fac_base are host arrays of nl
+ * entries. ukhip_host.cpp fills one from a batch; the allocation-free entry
+ * point fills one from its static device scratch array. */
+typedef struct {
+  uk_raw_t raw;
+  uk_chain_t chain[2];
+  uk_factors_t fac;
+  double* X[2];
+  double* XT[2];
+  double* xout;
+  double* Sout;
+  double* fix;
+  double* wobs;
+  int32_t n, mr0, compress, nl;
+  int64_t nF;
+  const uk_level_t* levels;
+  const int64_t* fac_base;
+} uk_oe_view_t;
+
 /* Batched sequential filter (UltimateKalman, c/kalman_ultimate.c:140-470):
  * per filter the information factor R (n x n, upper triangular, zero rows
  * while the state is undetermined), its right-hand side y (n), the number of

@@ -60,6 +60,8 @@ def lib():
         L.ukhip_use_mfma.argtypes = [C.c_int]
         L.ukhip_batch_smooth_oddeven_mixed.argtypes = [C.c_void_p, C.c_int32]
         L.ukhip_batch_smooth_oddeven_mixed.restype = C.c_int
+        L.ukhip_batch_smooth_oddeven_mixed_opt.argtypes = [C.c_void_p, C.c_void_p]
+        L.ukhip_batch_smooth_oddeven_mixed_opt.restype = C.c_int
         L.ukhip_server_create.restype = C.c_void_p
         L.ukhip_server_create.argtypes = [C.c_void_p]
         L.ukhip_server_submit.restype = C.c_int64
@@ -263,6 +265,11 @@ class Filter:
             pass
 
 
+class MixedOptions(C.Structure):
+    """ukhip_mixed_options_t (include/ukhip.h)."""
+    _fields_ = [("refinements", C.c_int32), ("fp32_levels", C.c_int32), ("covariances", C.c_int32)]
+
+
 class ServerOptions(C.Structure):
     """ukhip_server_options_t (include/ukhip.h)."""
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
@@ -358,6 +365,16 @@ class Batch:
         steps (covariances untouched)."""
         L = lib()
         _check(L.ukhip_batch_smooth_oddeven_mixed(self.h, refinements), "smooth_mixed")
+        if sync:
+            _check(L.ukhip_synchronize(), "synchronize")
+
+    def smooth_mixed_opt(self, refinements=1, fp32_levels=0, covariances=False, sync=True):
+        """ukhip_batch_smooth_oddeven_mixed_opt: levels 0..fp32_levels-1 in FP32
+        and the rest in FP64 (<= 0: all FP32); with covariances, the covariance
+        output is formed from the block factors too."""
+        L = lib()
+        o = MixedOptions(refinements, fp32_levels, 1 if covariances else 0)
+        _check(L.ukhip_batch_smooth_oddeven_mixed_opt(self.h, C.byref(o)), "smooth_mixed_opt")
         if sync:
             _check(L.ukhip_synchronize(), "synchronize")
 
