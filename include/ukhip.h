@@ -132,6 +132,25 @@ enum { UKHIP_OE_WEIGH = 0, UKHIP_OE_PHASE1, UKHIP_OE_PHASE2, UKHIP_OE_PHASE3, UK
 int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
                                      int32_t launches[UKHIP_OE_SLOTS]);
 
+/* ---- allocation-free odd-even smoothing ------------------------------------
+ * No counterpart in the reference (c/kalman.h): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It is ukhip_batch_create +
+ * ukhip_batch_smooth_oddeven for one problem whose arrays (every pointer of
+ * `problem`) and outputs are already in device memory: states (k*n) and,
+ * unless options has KALMAN_NO_COVARIANCE or covariances is NULL,
+ * covariances (k*n*n), with the batch path's results bit for bit. Every
+ * intermediate buffer comes from a static device array of
+ * UKHIP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a problem
+ * needing more (about 9 n^2 doubles per step) is refused with an error before
+ * anything is launched. Needs 1 <= m <= n; m_i outside 0..m is clamped.
+ * Enqueued on `stream` (NULL: the library stream); returns 0 or -1
+ * (ukhip_last_error). No hipMalloc, hipFree or host allocation is made. All
+ * calls on one GPU share its scratch: calls that overlap on one GPU race. */
+#define UKHIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ukhip_smooth_oddeven_noalloc(const ukhip_problem_t* problem, uint32_t options, double* states,
+                                 double* covariances, void* stream);
+
 /* ---- serving: continuous batching of smoothing requests -------------------
  * A server owns a device memory arena (the HBM budget) and a scheduler
  * thread. Submitted requests (their inputs are copied) wait in arrival
