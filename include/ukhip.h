@@ -93,8 +93,8 @@ int ukhip_synchronize(void);
 int ukhip_batch_smooth_oddeven_mixed(ukhip_batch_t* b, int32_t refinements);
 /* The same with options. fp32_levels: elimination levels 0 .. fp32_levels-1
  * run in FP32 and the levels above in FP64, where the reduced blocks'
- * conditioning compounds and few steps remain (<= 0: every level in FP32;
- * level 0 is always FP32). covariances = 1: the covariances are formed from
+ * conditioning compounds and few steps remain (<= 0: the default, 8; level
+ * 0 is always FP32). covariances = 1: the covariances are formed from
  * the same block factors by the FP64 selected-inversion back substitution
  * (FP32 factors widened), into the batch's covariance output; they carry the
  * FP32 factors' accuracy, not the refined estimates'. The first call per
@@ -121,16 +121,26 @@ int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format, double* o
  * UltimateKalman filter returns them. */
 #define UKHIP_COVARIANCE_W (1u << 24)
 
-/* Timed odd-even run: device milliseconds per kernel, measured with HIP events
- * recorded on the library stream around every launch and summed over the
- * levels. ms[UKHIP_OE_WEIGH .. UKHIP_OE_BACKSUB] are the kernels k_weigh,
- * k_phase1, k_phase2, k_phase3 and k_backsub; ms[UKHIP_OE_TOTAL] is the span
- * from the first launch to the end of the last. launches[] (may be NULL)
- * receives the launch count of each kernel. */
+/* Timed odd-even run: device milliseconds per kernel class, measured with HIP
+ * events recorded on the library stream around every launch (group) and
+ * summed over the levels; the parts of a multi-dimension batch run one after
+ * another. Slots: UKHIP_OE_WEIGH (k_weigh / k_weigh_diag, and k_obs_compress),
+ * PHASE1..3 (k_phase1..3), BACKSUB (k_backsub, k_backsub_mfma, k_estimate),
+ * TOTAL (the span from the first launch to the end of the last); for the
+ * mixed-precision path also SOLVE (k_solve_est), NORMAL (k_normal, k_gather,
+ * k_add), FWD (k_fwd_even, k_fwd_odd) and CONVERT (k_to32 / k_to64).
+ * launches[] (may be NULL) receives the launch count of each slot. */
 enum { UKHIP_OE_WEIGH = 0, UKHIP_OE_PHASE1, UKHIP_OE_PHASE2, UKHIP_OE_PHASE3, UKHIP_OE_BACKSUB, UKHIP_OE_TOTAL,
-       UKHIP_OE_SLOTS };
+       UKHIP_OE_SOLVE, UKHIP_OE_NORMAL, UKHIP_OE_FWD, UKHIP_OE_CONVERT, UKHIP_OE_SLOTS };
 int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
                                      int32_t launches[UKHIP_OE_SLOTS]);
+/* The same per part (one per state dimension), for the FP64 odd-even path
+ * (mixed NULL; options: KALMAN_NO_COVARIANCE) or the mixed-precision one:
+ * ms[p * UKHIP_OE_SLOTS + slot], launches likewise (may be NULL), part_n[p]
+ * the part's state dimension (may be NULL). Returns the number of parts
+ * (<= max_parts) or -1. */
+int ukhip_batch_timed_parts(ukhip_batch_t* b, uint32_t options, const ukhip_mixed_options_t* mixed, double* ms,
+                            int32_t* launches, int32_t* part_n, int32_t max_parts);
 
 /* ---- allocation-free odd-even smoothing ------------------------------------
  * No counterpart in the reference (c/kalman.h): for callers that cannot

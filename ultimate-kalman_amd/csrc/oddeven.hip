@@ -1260,11 +1260,12 @@ extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t
 
 /* The same elimination in single precision (chains, factors and arithmetic). */
 extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
-                                            const uk_factors32_t* fac, float* fix, int n, hipStream_t st) {
+                                            const uk_factors32_t* fac, float* fix, int n, hipStream_t st,
+                                            hipEvent_t* ev) {
   const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri, in->b_synth, in->wk, in->wF, in->wFidx};
   const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri, out->b_synth, out->wk, out->wF, out->wFidx};
   const FactorsT<float> f{fac->R, fac->Bt, fac->Y, fac->cf};
-  return launch_eliminate<float>(lv, ci, co, f, fix, n, st, nullptr);
+  return launch_eliminate<float>(lv, ci, co, f, fix, n, st, ev);
 }
 
 /* Back substitution with covariances for n > 16 on the matrix cores (default)

@@ -44,7 +44,7 @@ hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, doub
 int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options);
 void uk_set_error(const char* fmt, ...);
 hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
-                                 const uk_factors32_t* fac, float* fix, int n, hipStream_t st);
+                                 const uk_factors32_t* fac, float* fix, int n, hipStream_t st, hipEvent_t* ev);
 hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st);
 hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32, hipStream_t st);
 hipError_t uk_launch_add(double* x, const double* d, int64_t count, hipStream_t st);
@@ -940,10 +940,73 @@ static bool pack_problems(int32_t nprob, const ukhip_problem_t* P, Staging& st, 
 /* running                                                                    */
 /* ------------------------------------------------------------------------ */
 
-/* Events of a timed run: [0,1] around k_weigh, then per level l four around
- * the elimination phases at 2 + 4 l and two around the back substitution at
- * 2 + 4 L + 2 l. */
-static size_t timed_events(size_t levels) { return 2 + 6 * levels; }
+/* Device time per kernel class (the UKHIP_OE_* slots) of runs on one stream:
+ * a pair of HIP events around every launch group, summed per slot once the
+ * stream has finished. A phase triple of the elimination records four events
+ * (before phase 1, after each phase). */
+struct SlotTimer {
+  hipStream_t st = nullptr;
+  struct Rec {
+    int slot;
+    hipEvent_t a, b;
+    int launches;
+  };
+  std::vector<Rec> rec;
+  std::vector<hipEvent_t> all;
+  hipEvent_t first = nullptr, last = nullptr;
+  bool ok = true;
+
+  hipEvent_t make() {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    all.push_back(e);
+    return e;
+  }
+  hipEvent_t mark() {
+    hipEvent_t e = make();
+    if (e && hipEventRecord(e, st) != hipSuccess) ok = false;
+    if (!first) first = e;
+    last = e;
+    return e;
+  }
+  /* begin() ... launches ... end(slot, launches) */
+  hipEvent_t pending = nullptr;
+  void begin() { pending = mark(); }
+  void end(int slot, int launches = 1) { rec.push_back({slot, pending, mark(), launches}); }
+  /* four recorded events for uk_launch_eliminate(32) */
+  hipEvent_t quad[4]{};
+  hipEvent_t* phases() {
+    for (auto& e : quad) e = make();
+    return quad;
+  }
+  void phases_done() {
+    if (!first) first = quad[0];
+    last = quad[3];
+    for (int p = 0; p < 3; ++p) rec.push_back({UKHIP_OE_PHASE1 + p, quad[p], quad[p + 1], 1});
+  }
+  /* waits for the stream, adds the slot sums into ms and launches */
+  bool collect(double* ms, int32_t* launches) {
+    if (!ok || !last || hipEventSynchronize(last) != hipSuccess) return false;
+    for (auto& r : rec) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) return false;
+      ms[r.slot] += t;
+      if (launches) launches[r.slot] += r.launches;
+    }
+    float t = 0;
+    if (hipEventElapsedTime(&t, first, last) != hipSuccess) return false;
+    ms[UKHIP_OE_TOTAL] += t;
+    if (launches) launches[UKHIP_OE_TOTAL] += 1;
+    return true;
+  }
+  ~SlotTimer() {
+    for (auto e : all)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
 
 /* The level-0 chain's D = V H is diagonal when every H is the identity and
  * every K is given as 'w' (V diagonal). When C is 'w' too, k_weigh_diag writes
@@ -977,13 +1040,12 @@ static bool weigh_level0(B* b, hipStream_t st) {
 
 /* One whole odd-even smoothing (c/kalman_oddeven_smoother.c:1358-1553,
  * 614-975) over the buffers of v, enqueued on st. */
-static bool run_oddeven_view(uk_oe_view_t* v, int cov, hipEvent_t* ev /* timed_events(nl) events or NULL */,
-                             hipStream_t st) {
+static bool run_oddeven_view(uk_oe_view_t* v, int cov, SlotTimer* tm, hipStream_t st) {
   const int nl = v->nl, n = v->n;
   const size_t nn = (size_t)n * n;
-  if (ev && !hip_ok(hipEventRecord(ev[0], st), "hipEventRecord")) return false;
+  if (tm) tm->begin();
   if (!weigh_level0(v, st)) return false;
-  if (ev && !hip_ok(hipEventRecord(ev[1], st), "hipEventRecord")) return false;
+  if (tm) tm->end(UKHIP_OE_WEIGH, v->compress ? 2 : 1);
   auto fac_at = [&](int l) {
     uk_factors_t f = v->fac;
     const int64_t fb = v->fac_base[l];
@@ -1003,22 +1065,22 @@ static bool run_oddeven_view(uk_oe_view_t* v, int cov, hipEvent_t* ev /* timed_e
     out.mr = n;
     const uk_factors_t f = fac_at(l);
     uk_level_t lv = v->levels[l]; /* even_start is relative to the level's fac_base */
-    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, v->fix, n, st, ev ? ev + 2 + 4 * l : nullptr),
+    if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, v->fix, n, st, tm ? tm->phases() : nullptr),
                 "elimination launch"))
       return false;
+    if (tm) tm->phases_done();
   }
   for (int l = nl - 1; l >= 0; --l) {
     const uk_factors_t f = fac_at(l);
     uk_level_t lv = v->levels[l];
-    hipEvent_t* eb = ev ? ev + 2 + 4 * nl + 2 * l : nullptr;
-    if (eb && !hip_ok(hipEventRecord(eb[0], st), "hipEventRecord")) return false;
+    if (tm) tm->begin();
     double* Xc = l == 0 ? nullptr : v->X[l & 1];
     double* XTc = l == 0 ? nullptr : v->XT[l & 1];
     if (!hip_ok(uk_launch_backsub(&lv, &f, v->xout, v->Sout, v->X[(l + 1) & 1], v->XT[(l + 1) & 1], Xc, XTc, n, cov,
                                   st),
                 "k_backsub launch"))
       return false;
-    if (eb && !hip_ok(hipEventRecord(eb[1], st), "hipEventRecord")) return false;
+    if (tm) tm->end(UKHIP_OE_BACKSUB);
   }
   return true;
 }
@@ -1028,8 +1090,7 @@ extern "C" int uk_run_oddeven_view(uk_oe_view_t* v, int cov, hipStream_t st) {
   return run_oddeven_view(v, cov, nullptr, st) ? 0 : -1;
 }
 
-static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(levels) events or NULL */,
-                        hipStream_t st) {
+static bool run_oddeven(ukhip_batch* b, int cov, SlotTimer* tm, hipStream_t st) {
   const int nl = (int)b->levels.size();
   std::vector<uk_level_t> lv(nl);
   std::vector<int64_t> fb(nl);
@@ -1057,7 +1118,7 @@ static bool run_oddeven(ukhip_batch* b, int cov, hipEvent_t* ev /* timed_events(
   v.nF = b->nF;
   v.levels = lv.data();
   v.fac_base = fb.data();
-  return run_oddeven_view(&v, cov, ev, st);
+  return run_oddeven_view(&v, cov, tm, st);
 }
 
 static ukhip_batch* batch_create_one(int32_t nprob, const ukhip_problem_t* problems) {
@@ -1195,62 +1256,60 @@ extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options
   return 0;
 }
 
+static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov, SlotTimer* tm, hipStream_t st);
+static int mixed_fp32_levels(const ukhip_mixed_options_t* o);
+
+/* One part (a batch without parts) run eagerly with a slot timer; the slot
+ * sums are added into ms and launches. */
+static bool timed_part(ukhip_batch* q, uint32_t options, const ukhip_mixed_options_t* mixed, double* ms,
+                       int32_t* launches) {
+  SlotTimer tm;
+  tm.st = g_stream;
+  const bool ok = mixed ? run_mixed(q, mixed->refinements, mixed_fp32_levels(mixed), mixed->covariances != 0, &tm,
+                                    g_stream)
+                        : run_oddeven(q, (options & KALMAN_NO_COVARIANCE) == 0, &tm, g_stream);
+  if (!ok) return false;
+  if (!tm.collect(ms, launches)) return hip_ok(hipErrorUnknown, "timed smoothing events");
+  return true;
+}
+
+extern "C" int ukhip_batch_timed_parts(ukhip_batch_t* b, uint32_t options, const ukhip_mixed_options_t* mixed,
+                                       double* ms, int32_t* launches, int32_t* part_n, int32_t max_parts) {
+  if (!b || !ms || max_parts < 1) { uk_set_error("NULL batch or result array"); return -1; }
+  if (!ensure_stream()) return -1;
+  if (mixed && (mixed->refinements < 0 || mixed->refinements > 64)) { uk_set_error("refinements must be in 0..64"); return -1; }
+  std::vector<ukhip_batch*> parts = b->parts;
+  if (parts.empty()) parts.push_back(b);
+  if ((int)parts.size() > max_parts) { uk_set_error("the batch has %zu parts, more than max_parts", parts.size()); return -1; }
+  for (size_t i = 0; i < parts.size(); ++i) {
+    double* m = ms + i * UKHIP_OE_SLOTS;
+    int32_t* l = launches ? launches + i * UKHIP_OE_SLOTS : nullptr;
+    for (int t = 0; t < UKHIP_OE_SLOTS; ++t) {
+      m[t] = 0.0;
+      if (l) l[t] = 0;
+    }
+    if (part_n) part_n[i] = parts[i]->n;
+    if (!timed_part(parts[i], options, mixed, m, l)) return -1;
+  }
+  return (int)parts.size();
+}
+
 extern "C" int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_OE_SLOTS],
                                                 int32_t launches[UKHIP_OE_SLOTS]) {
   if (!b || !ms) { uk_set_error("NULL batch or result array"); return -1; }
-  if (!ensure_stream()) return -1;
-  if (!b->parts.empty()) { /* parts one after another, times and launches summed */
-    double acc[UKHIP_OE_SLOTS] = {0};
-    int32_t nacc[UKHIP_OE_SLOTS] = {0};
-    for (auto* q : b->parts) {
-      double m1[UKHIP_OE_SLOTS];
-      int32_t n1[UKHIP_OE_SLOTS];
-      if (ukhip_batch_smooth_oddeven_timed(q, options, m1, n1) != 0) return -1;
-      for (int i = 0; i < UKHIP_OE_SLOTS; ++i) {
-        acc[i] += m1[i];
-        nacc[i] += n1[i];
-      }
-    }
-    for (int i = 0; i < UKHIP_OE_SLOTS; ++i) {
-      ms[i] = acc[i];
-      if (launches) launches[i] = nacc[i];
-    }
-    return 0;
-  }
-  const size_t nl = b->levels.size();
-  std::vector<hipEvent_t> ev(timed_events(nl), nullptr);
-  bool ok = true;
-  for (auto& e : ev)
-    if (ok && !hip_ok(hipEventCreate(&e), "hipEventCreate")) ok = false;
-  ok = ok && run_oddeven(b, (options & KALMAN_NO_COVARIANCE) == 0, ev.data(), g_stream) &&
-       hip_ok(hipEventSynchronize(ev[2 + 4 * nl + 1]), "hipEventSynchronize");
-  if (ok) {
-    auto span = [&](size_t a, size_t c) {
-      float t = 0;
-      (void)hipEventElapsedTime(&t, ev[a], ev[c]);
-      return (double)t;
-    };
-    for (int i = 0; i < UKHIP_OE_SLOTS; ++i) ms[i] = 0.0;
-    ms[UKHIP_OE_WEIGH] = span(0, 1);
-    for (size_t l = 0; l < nl; ++l) {
-      const size_t e = 2 + 4 * l;
-      ms[UKHIP_OE_PHASE1] += span(e, e + 1);
-      ms[UKHIP_OE_PHASE2] += span(e + 1, e + 2);
-      ms[UKHIP_OE_PHASE3] += span(e + 2, e + 3);
-      const size_t eb = 2 + 4 * nl + 2 * l;
-      ms[UKHIP_OE_BACKSUB] += span(eb, eb + 1);
-    }
-    ms[UKHIP_OE_TOTAL] = span(0, 2 + 4 * nl + 1); /* the back substitution of level 0 ends last */
-    if (launches) {
-      launches[UKHIP_OE_WEIGH] = 1;
-      launches[UKHIP_OE_PHASE1] = launches[UKHIP_OE_PHASE2] = launches[UKHIP_OE_PHASE3] = (int32_t)nl;
-      launches[UKHIP_OE_BACKSUB] = (int32_t)nl;
-      launches[UKHIP_OE_TOTAL] = 1 + 4 * (int32_t)nl;
+  const int np = std::max<int>(1, (int)b->parts.size());
+  std::vector<double> m(np * UKHIP_OE_SLOTS);
+  std::vector<int32_t> l(np * UKHIP_OE_SLOTS);
+  if (ukhip_batch_timed_parts(b, options, nullptr, m.data(), l.data(), nullptr, np) < 0) return -1;
+  for (int t = 0; t < UKHIP_OE_SLOTS; ++t) {
+    ms[t] = 0.0;
+    if (launches) launches[t] = 0;
+    for (int p = 0; p < np; ++p) {
+      ms[t] += m[p * UKHIP_OE_SLOTS + t];
+      if (launches) launches[t] += l[p * UKHIP_OE_SLOTS + t];
     }
   }
-  for (auto& e : ev)
-    if (e) (void)hipEventDestroy(e);
-  return ok ? 0 : -1;
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1342,8 +1401,14 @@ static uk_factors_t fac64_at(const ukhip_batch* b, int l) {
  * the factors of both precisions. With cov, the covariances are then formed
  * from the same factors by the FP64 selected-inversion back substitution
  * (the FP32 factors widened into the FP64 factor slots), into Sout. */
-static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov, hipStream_t st) {
+static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov, SlotTimer* tm, hipStream_t st) {
   if (!mixed_workspace(b)) return false;
+  auto tb = [&]() {
+    if (tm) tm->begin();
+  };
+  auto te = [&](int slot, int launches) {
+    if (tm) tm->end(slot, launches);
+  };
   const int n = b->n, nl = (int)b->levels.size();
   const int lsw = std::max(1, std::min(fp32_levels, nl)); /* first FP64 level (level 0 stays FP32) */
   const size_t nn = (size_t)n * n, S = (size_t)b->steps;
@@ -1352,16 +1417,23 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
   const uk_chain_t w0 = level0_chain(b, b->chain[0]); /* its flags imply the fused path below */
+  tb();
   const hipError_t fused = b->compress ? hipErrorNotSupported : uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
   if (fused != hipErrorNotSupported) {
     if (!hip_ok(fused, "k_weigh launch")) return false;
-  } else if (!weigh_level0(b, st) ||
-      !hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
-      !hip_ok(uk_launch_to32(c32.D, c64.D, S * nn, st), "to32") ||
-      !hip_ok(uk_launch_to32(c32.c, c64.c, S * n, st), "to32") ||
-      !hip_ok(uk_launch_to32(c32.C, c64.C, S * nn, st), "to32") ||
-      !hip_ok(uk_launch_to32(c32.o, c64.o, S * n, st), "to32"))
-    return false;
+    te(UKHIP_OE_WEIGH, 1);
+  } else {
+    if (!weigh_level0(b, st)) return false;
+    te(UKHIP_OE_WEIGH, b->compress ? 2 : 1);
+    tb();
+    if (!hip_ok(uk_launch_to32(c32.B, c64.B, S * nn, st), "to32") ||
+        !hip_ok(uk_launch_to32(c32.D, c64.D, S * nn, st), "to32") ||
+        !hip_ok(uk_launch_to32(c32.c, c64.c, S * n, st), "to32") ||
+        !hip_ok(uk_launch_to32(c32.C, c64.C, S * nn, st), "to32") ||
+        !hip_ok(uk_launch_to32(c32.o, c64.o, S * n, st), "to32"))
+      return false;
+    te(UKHIP_OE_CONVERT, 5);
+  }
   uk_chain_t top[2] = {b->top0, b->chain[1]};
   for (int l = 0; l < nl; ++l) {
     uk_level_t lv = b->levels[l].dev;
@@ -1374,16 +1446,21 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
       out.mr = n;
       const uk_factors32_t f = fac32_at(b, l);
-      if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st), "FP32 elimination launch")) return false;
+      if (!hip_ok(uk_launch_eliminate32(&lv, &in, &out, &f, b->fix32, n, st, tm ? tm->phases() : nullptr),
+                  "FP32 elimination launch"))
+        return false;
+      if (tm) tm->phases_done();
       if (l + 1 == lsw && lsw < nl) { /* the next level's chain enters FP64 */
         const size_t ns = (size_t)b->levels[lsw].steps;
         const uk_chain_t& t = top[lsw & 1];
+        tb();
         if (!hip_ok(uk_launch_to64(t.B, out.B, ns * nn, st), "to64") ||
             !hip_ok(uk_launch_to64(t.D, out.D, ns * nn, st), "to64") ||
             !hip_ok(uk_launch_to64(t.c, out.c, ns * n, st), "to64") ||
             !hip_ok(uk_launch_to64(t.C, out.C, ns * nn, st), "to64") ||
             !hip_ok(uk_launch_to64(t.o, out.o, ns * n, st), "to64"))
           return false;
+        te(UKHIP_OE_CONVERT, 5);
       }
     } else {
       uk_chain_t in = top[l & 1];
@@ -1392,13 +1469,16 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       uk_chain_t out = top[(l + 1) & 1];
       out.mr = n;
       const uk_factors_t f = fac64_at(b, l);
-      if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, n, st, nullptr), "FP64 elimination launch"))
+      if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, n, st, tm ? tm->phases() : nullptr),
+                  "FP64 elimination launch"))
         return false;
+      if (tm) tm->phases_done();
     }
   }
   /* x0 from the transformed right-hand sides */
   for (int l = nl - 1; l >= 0; --l) {
     uk_level_t lv = b->levels[l].dev;
+    tb();
     if (l < lsw) {
       const uk_factors32_t f = fac32_at(b, l);
       if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
@@ -1406,16 +1486,20 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       const uk_factors_t f = fac64_at(b, l);
       if (!hip_ok(uk_launch_solve_est64(&lv, &f, f.cf, b->xout, n, st), "solve launch")) return false;
     }
+    te(UKHIP_OE_SOLVE, 1);
   }
   const uk_chain_t r64 = level0_chain(b, b->chain[0]);
   for (int it = 0; it < refinements; ++it) {
+    tb();
     if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->m0, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),
                 "residual launch"))
       return false;
+    te(UKHIP_OE_NORMAL, 2);
     for (int l = 0; l < nl; ++l) {
       uk_level_t lv = b->levels[l].dev;
       double* y = b->ys + b->levels[l].fac_base * n;
       hipError_t e;
+      tb();
       if (l < lsw) {
         const uk_factors32_t f = fac32_at(b, l);
         e = uk_launch_fwd(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
@@ -1424,11 +1508,13 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
         e = uk_launch_fwd64(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
       }
       if (!hip_ok(e, "forward sweep launch")) return false;
+      te(UKHIP_OE_FWD, 2);
     }
     for (int l = nl - 1; l >= 0; --l) {
       uk_level_t lv = b->levels[l].dev;
       const double* y = b->ys + b->levels[l].fac_base * n;
       hipError_t e;
+      tb();
       if (l < lsw) {
         const uk_factors32_t f = fac32_at(b, l);
         e = uk_launch_solve_est(&lv, &f, nullptr, y, b->dx, n, st);
@@ -1437,27 +1523,34 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
         e = uk_launch_solve_est64(&lv, &f, y, b->dx, n, st);
       }
       if (!hip_ok(e, "backward sweep launch")) return false;
+      te(UKHIP_OE_SOLVE, 1);
     }
+    tb();
     if (!hip_ok(uk_launch_add(b->xout, b->dx, (int64_t)(S * n), st), "update launch")) return false;
+    te(UKHIP_OE_NORMAL, 1);
   }
   if (!cov) return true;
   /* covariances: the FP32 levels' factors widened into their FP64 slots (the
    * slots of levels 0 .. lsw-1 are contiguous), then the FP64 back
    * substitution; its estimates go to the scratch dx, xout keeps the refined ones */
   const size_t E32 = (size_t)(lsw < nl ? b->levels[lsw].fac_base : b->total_evens);
+  tb();
   if (!hip_ok(uk_launch_to64(b->fac.R, b->fac32.R, E32 * nn, st), "to64") ||
       !hip_ok(uk_launch_to64(b->fac.Bt, b->fac32.Bt, E32 * nn, st), "to64") ||
       !hip_ok(uk_launch_to64(b->fac.Y, b->fac32.Y, E32 * nn, st), "to64") ||
       !hip_ok(uk_launch_to64(b->fac.cf, b->fac32.cf, E32 * n, st), "to64"))
     return false;
+  te(UKHIP_OE_CONVERT, 4);
   for (int l = nl - 1; l >= 0; --l) {
     const uk_factors_t f = fac64_at(b, l);
     uk_level_t lv = b->levels[l].dev;
     double* Xc = l == 0 ? nullptr : b->X[l & 1];
     double* XTc = l == 0 ? nullptr : b->XT[l & 1];
+    tb();
     if (!hip_ok(uk_launch_backsub(&lv, &f, b->dx, b->Sout, b->X[(l + 1) & 1], b->XT[(l + 1) & 1], Xc, XTc, n, 1, st),
                 "k_backsub launch"))
       return false;
+    te(UKHIP_OE_BACKSUB, 1);
   }
   return true;
 }
@@ -1468,18 +1561,21 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
  * the error after one refinement step (1.74e-9 either way), which the level-0
  * FP32 factors set. */
 constexpr int MIXED_FP32_LEVELS = 8;
+static int mixed_fp32_levels(const ukhip_mixed_options_t* o) {
+  return o->fp32_levels <= 0 ? MIXED_FP32_LEVELS : std::min(o->fp32_levels, 64);
+}
 
 /* Enqueues a whole mixed-precision smoothing on the library stream (the parts
  * of a composite on forked streams, as enqueue_oddeven). */
 static bool enqueue_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov) {
-  if (b->parts.empty()) return run_mixed(b, refinements, fp32_levels, cov, g_stream);
+  if (b->parts.empty()) return run_mixed(b, refinements, fp32_levels, cov, nullptr, g_stream);
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins(b->parts.size(), nullptr);
   bool ok = hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate") &&
             hip_ok(hipEventRecord(fork, g_stream), "hipEventRecord");
   for (size_t i = 0; ok && i < b->parts.size(); ++i) {
     ok = hip_ok(hipStreamWaitEvent(b->part_streams[i], fork, 0), "hipStreamWaitEvent") &&
-         run_mixed(b->parts[i], refinements, fp32_levels, cov, b->part_streams[i]) &&
+         run_mixed(b->parts[i], refinements, fp32_levels, cov, nullptr, b->part_streams[i]) &&
          hip_ok(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming), "hipEventCreate") &&
          hip_ok(hipEventRecord(joins[i], b->part_streams[i]), "hipEventRecord") &&
          hip_ok(hipStreamWaitEvent(g_stream, joins[i], 0), "hipStreamWaitEvent");
@@ -1526,7 +1622,7 @@ extern "C" int ukhip_batch_smooth_oddeven_mixed_opt(ukhip_batch_t* b, const ukhi
   if (!b || !opt) { uk_set_error("NULL batch or options"); return -1; }
   if (!ensure_stream()) return -1;
   if (opt->refinements < 0 || opt->refinements > 64) { uk_set_error("refinements must be in 0..64"); return -1; }
-  const int fl = opt->fp32_levels <= 0 ? MIXED_FP32_LEVELS : std::min(opt->fp32_levels, 64);
+  const int fl = mixed_fp32_levels(opt);
   const bool cov = opt->covariances != 0;
   if (g_use_graphs) return graph_mixed(b, opt->refinements, fl, cov) ? 0 : -1;
   return enqueue_mixed(b, opt->refinements, fl, cov) ? 0 : -1;
@@ -1845,7 +1941,7 @@ static bool server_admit(ukhip_server* s, std::unique_lock<std::mutex>& lk) {
     f->b = batch_from_staging(f->st, lengths, st, &s->arena, false);
     ok = f->b != nullptr;
   }
-  if (ok) ok = mixed ? run_mixed(f->b, s->opt.mixed_refinements, MIXED_FP32_LEVELS, false, st) : run_oddeven(f->b, 1, nullptr, st);
+  if (ok) ok = mixed ? run_mixed(f->b, s->opt.mixed_refinements, MIXED_FP32_LEVELS, false, nullptr, st) : run_oddeven(f->b, 1, nullptr, st);
   const size_t S = (size_t)f->st.steps, nn = (size_t)n * n;
   if (ok) ok = (f->host_x = (double*)s->pinned.get(S * n * 8)) != nullptr;
   if (ok && !mixed) ok = (f->host_S = (double*)s->pinned.get(S * nn * 8)) != nullptr;

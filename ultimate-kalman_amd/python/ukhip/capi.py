@@ -54,6 +54,7 @@ class EquationSet:
         self.problem = problem
         k = problem.k
         self._keep = []
+        self._shared = {}  # id(array) -> matrix pointer: a matrix object used on many steps is converted once
         self.structs = (StepEquations * k)()
         self.ptrs = (C.POINTER(StepEquations) * k)()
         for i in range(k):
@@ -75,21 +76,29 @@ class EquationSet:
             self.ptrs[i] = C.pointer(s)
 
     def _mat(self, a):
+        key = id(a)
+        hit = self._shared.get(key)
+        if hit is not None and hit[0] is a:
+            return hit[1]
+        src = a
         a = _as_fortran(a)
         self._keep.append(a)
         m = KalmanMatrix(a.shape[0], a.shape[1], a.shape[0],
                          a.ctypes.data_as(C.POINTER(C.c_double)))
         self._keep.append(m)
-        return C.pointer(m)
+        ptr = C.pointer(m)
+        self._shared[key] = (src, ptr)
+        return ptr
 
-    def collect(self, free_fn=None, want_cov=True):
-        """Reads (and, given `free_fn`, releases) every step's state/covariance."""
+    def collect(self, free_fn=None, want_cov=True, read=True):
+        """Reads (read=False: skips reading) and, given `free_fn`, releases
+        every step's state/covariance."""
         states, covs = [], []
         for i in range(self.problem.k):
             s = self.structs[i]
-            x = _read(s.state)
+            x = _read(s.state) if read else None
             states.append(None if x is None else x[:, 0])
-            covs.append(_read(s.covariance) if want_cov else None)
+            covs.append(_read(s.covariance) if (want_cov and read) else None)
             if free_fn is not None:
                 if s.state:
                     free_fn(s.state)

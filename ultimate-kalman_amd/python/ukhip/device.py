@@ -27,7 +27,8 @@ class ProblemStruct(C.Structure):
 _lib = None
 
 # slots of ukhip_batch_smooth_oddeven_timed (include/ukhip.h)
-OE_KERNELS = ("k_weigh", "k_phase1", "k_phase2", "k_phase3", "k_backsub", "total")
+OE_KERNELS = ("k_weigh", "k_phase1", "k_phase2", "k_phase3", "k_backsub", "total", "k_solve_est", "k_normal",
+              "k_fwd", "convert")  # the UKHIP_OE_* slots of include/ukhip.h, in order
 
 
 def lib():
@@ -50,6 +51,9 @@ def lib():
         L.ukhip_batch_smooth_oddeven_timed.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
                                                        C.POINTER(C.c_int32)]
         L.ukhip_batch_smooth_oddeven_timed.restype = C.c_int
+        L.ukhip_batch_timed_parts.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
+        L.ukhip_batch_timed_parts.restype = C.c_int
         L.ukhip_batch_download.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ukhip_batch_download.restype = C.c_int
         L.ukhip_batch_download_cov.argtypes = [C.c_void_p, C.c_int32, C.c_char, C.c_void_p]
@@ -386,6 +390,23 @@ class Batch:
         nl = (C.c_int32 * len(OE_KERNELS))()
         _check(lib().ukhip_batch_smooth_oddeven_timed(self.h, options, ms, nl), "smooth_timed")
         return dict(zip(OE_KERNELS, list(ms))), dict(zip(OE_KERNELS, list(nl)))
+
+    def timed_parts(self, options=0, mixed=None):
+        """ukhip_batch_timed_parts: one eager smoothing (FP64 with `options`, or
+        the mixed path with mixed = (refinements, fp32_levels, covariances)),
+        parts one after another, HIP events around every launch group. Returns
+        [(n, {slot: ms}, {slot: launches})] per part."""
+        maxp = 64
+        S = len(OE_KERNELS)
+        ms = (C.c_double * (maxp * S))()
+        nl = (C.c_int32 * (maxp * S))()
+        pn = (C.c_int32 * maxp)()
+        mo = None if mixed is None else C.byref(MixedOptions(mixed[0], mixed[1], 1 if mixed[2] else 0))
+        np_ = lib().ukhip_batch_timed_parts(self.h, options, mo, ms, nl, pn, maxp)
+        if np_ < 0:
+            raise UkhipError("timed_parts: " + last_error())
+        return [(pn[p], dict(zip(OE_KERNELS, ms[p * S:(p + 1) * S])), dict(zip(OE_KERNELS, nl[p * S:(p + 1) * S])))
+                for p in range(np_)]
 
     def download(self, i, want_cov=True):
         p = self.problems[i]

@@ -488,3 +488,4 @@ def packed_from_lists(pr):
     pr.packed = dict(n=n, m=m, F=F, H=None if H_ident else H, c=c, o=o, G=G, observed=np.array(ms) > 0,
                      m_i=np.array(ms, dtype=np.int32), K_steps=K, K_type=kt, C_steps=C, C_type=ct)
     return pr
+

@@ -2,7 +2,7 @@
 
     python -m ukhip.rocprof stats DIR [--per N]
         per-kernel launch counts and durations from DIR/run_kernel_stats.csv
-        (--per N divides totals by N smoothings)
+        or DIR/run_results.db (--per N divides totals by N smoothings)
     python -m ukhip.rocprof traffic --fetch DIR1 --write DIR2 --copy-bytes B
             [--per N] [--workload NAME --json PATH]
         HBM bytes per kernel from a FETCH_SIZE run (DIR1) and a WRITE_SIZE run
@@ -39,11 +39,27 @@ def base_name(name):
 
 
 def kernel_stats(d):
+    """(name, calls, total ns, average ns, percent) per kernel, from
+    run_kernel_stats.csv or, for rocprofv3's SQLite output, run_results.db."""
     rows = []
-    with open(os.path.join(d, "run_kernel_stats.csv")) as f:
-        for r in csv.DictReader(f):
-            rows.append((short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]),
-                         float(r["AverageNs"]), float(r["Percentage"])))
+    csv_path = os.path.join(d, "run_kernel_stats.csv")
+    if os.path.exists(csv_path):
+        with open(csv_path) as f:
+            for r in csv.DictReader(f):
+                rows.append((short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]),
+                             float(r["AverageNs"]), float(r["Percentage"])))
+        return rows
+    import sqlite3
+    db = os.path.join(d, "run_results.db")
+    con = sqlite3.connect(db)
+    acc = collections.defaultdict(lambda: [0, 0.0])
+    for name, dur in con.execute("select name, duration from kernels"):
+        e = acc[short_name(name)]
+        e[0] += 1
+        e[1] += float(dur)
+    total = sum(v[1] for v in acc.values()) or 1.0
+    for name, (calls, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        rows.append((name, calls, tot, tot / calls, 100.0 * tot / total))
     return rows
 
 
