@@ -120,6 +120,13 @@ int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format, double* o
  * returned in the 'W' representation, covariance_type 'W', as the reference's
  * UltimateKalman filter returns them. */
 #define UKHIP_COVARIANCE_W (1u << 24)
+/* Option bit for ukhip_batch_smooth_oddeven(_timed) and ukhip_batch_timed_parts:
+ * only the covariances' diagonals are wanted. The back substitution of the
+ * first elimination level (every second step, whose covariances no other level
+ * reads) then forms diag(S_j) alone and writes n values per step instead of
+ * n*n; read the diagonals with ukhip_batch_download_cov(b, p, 'D', out). The
+ * other entries of the covariance output are left unspecified. */
+#define UKHIP_COVARIANCE_DIAG (1u << 25)
 
 /* Timed odd-even run: device milliseconds per kernel class, measured with HIP
  * events recorded on the library stream around every launch (group) and

@@ -244,14 +244,20 @@ def test_launch_table_set_and_get():
     saved = device.launch_table()
     try:
         assert len(saved) == 18
-        assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4"}
+        assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4", "c3w1"}
         cpl = C.c_int(0)
         assert L.ukhip_qr_launch_set(1, 2, 48, 2, 3) == 0
         assert L.ukhip_qr_launch_get(1, 2, 48, C.byref(cpl)) == 3 and cpl.value == 2
         assert L.ukhip_qr_launch_set(0, 2, 48, 1, 3) == -1   # not compiled
         assert L.ukhip_qr_launch_set(0, 2, 48, 2, 1) == -1
-        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 2) == -1
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 2) == -1   # blocked WY: one workgroup per CU only
+        assert L.ukhip_qr_launch_set(0, 1, 48, 3, 1) == -1   # and for phase 2 only
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 1) == 0
+        assert L.ukhip_qr_launch_get(0, 2, 48, C.byref(cpl)) == 1 and cpl.value == 3
         assert L.ukhip_qr_launch_set(0, 4, 48, 1, 1) == -1
+        assert L.ukhip_qr_launch_set(1, 2, 64, 2, 3) == -1   # left out of the FP32 table at bucket 64
+        assert L.ukhip_qr_launch_set(0, 2, 64, 2, 3) == 0
+        assert L.ukhip_qr_launch_set(1, 1, 64, 2, 3) == 0
         assert L.ukhip_qr_launch_set(0, 1, 8, 1, 1) == -1     # n <= 16 has no variants
         assert L.ukhip_qr_launch_get(0, 1, 8, None) == 0
         device.set_launch_table({"phase3<64, double>": "c2w4", "phase1<32, float>": "c1w1"})

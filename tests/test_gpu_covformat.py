@@ -71,3 +71,35 @@ def test_c_abi_w_representation(dev, oracle):
     worst = max(float(np.max(np.abs(W @ s @ W.T - np.eye(pr.n[i])))) for i, (W, s) in enumerate(zip(Ws, So)))
     say(f"C ABI 'W' covariances: max |W S_oracle W^T - I| = {worst:.2e}")
     assert worst < 1e-9
+
+
+UKHIP_COVARIANCE_DIAG = 1 << 25
+
+
+@pytest.mark.parametrize("n,m", [(6, 6), (13, 5), (40, 20), (64, 32)])
+def test_diagonal_only_path(dev, oracle, n, m):
+    """UKHIP_COVARIANCE_DIAG: the back substitution of the first level forms
+    only diag(S_j) (vector units for n <= 16, matrix cores above). The
+    diagonals equal the full path's to rounding (1e-13) and the oracle's to
+    1e-9; the estimates are bit-identical to the full path's; odd and even
+    lengths and a one-step problem."""
+    F = 0.98 * P.random_orthogonal(P.rng_for(740 + n), n)
+    lengths = (1, 6, 257, 1024) if m == n else (6, 257, 1024)
+    probs = [P.iid_problem(k, n, m, seed=750 + k, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
+                           unobserved=lambda i: i % 10 == 9, name=f"n{n}_k{k}") for k in lengths]
+    b = dev.Batch(probs)
+    try:
+        b.smooth("oddeven")
+        full = [(b.download(i)[0], b.download_cov(i, "D")) for i in range(len(probs))]
+        b.smooth("oddeven", options=UKHIP_COVARIANCE_DIAG)
+        for i, pr in enumerate(probs):
+            x = b.download(i, want_cov=False)[0]
+            D = b.download_cov(i, "D")
+            assert np.array_equal(x, full[i][0]), pr.name
+            d_full = float(np.max(np.abs(D - full[i][1]) / np.abs(full[i][1])))
+            _, oc = oracle.oddeven(pr)
+            d_or = float(np.max(np.abs(D - np.stack([np.diag(c) for c in oc])) / np.abs(full[i][1])))
+            say(f"  {pr.name}: diagonals vs full path {d_full:.1e}, vs oracle {d_or:.1e}")
+            assert d_full < 1e-13 and d_or < 1e-9, pr.name
+    finally:
+        b.free()

@@ -245,9 +245,15 @@ __global__ void __launch_bounds__(((2 * NB + 63) / 64) * 64)
     for (int i = 0; i < NB; ++i) v[i] = (i < n) ? sh.T[q + i * NB] : 0.0;
     back_solve<NB>(v, sh.R, sh.rinv);
     double* S = Sout + oj * nn;
+    if (cov == 2) { /* diagonal only (level 0 with UKHIP_COVARIANCE_DIAG): S_qq in its place */
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < n) S[q + (int64_t)i * n] = v[i]; /* S symmetric: column q written as row q */
+      for (int i = 0; i < NB; ++i)
+        if (i == q) S[q + (int64_t)q * n] = v[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (i < n) S[q + (int64_t)i * n] = v[i]; /* S symmetric: column q written as row q */
+    }
   }
 }
 
@@ -360,7 +366,7 @@ struct BackMShared {
  * Products with R^{-1} skip its zero lower blocks. The estimate is a
  * substitution with R (lanes of wave 1) beside the inverse (lanes of wave 0).
  */
-template <int NB>
+template <int NB, bool DIAG = false>
 __global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
     k_backsub_mfma(uk_level_t lv, uk_factors_t fac, double* __restrict__ xout, double* __restrict__ Sout,
                    const double* __restrict__ Xup, const double* __restrict__ XTup, double* __restrict__ Xcur,
@@ -627,8 +633,18 @@ __global__ void __launch_bounds__(NB * 8, (NB == 48 ? 3 : 1))
   }
   __syncthreads();
 
-  /* ---- S = K R^{-T}: B fragment R^{-T}[k][j] = R^{-1}[j][k], zero for k < j ---- */
   double* S = Sout + oj * nn;
+  if constexpr (DIAG) {
+    /* diagonal only (level 0 with UKHIP_COVARIANCE_DIAG): S_ii = sum_k K_ik R^{-1}_ik
+     * over k >= i (R^{-1} upper triangular), written in its place in the slot */
+    if (tid < n) {
+      double d = 0.0;
+      for (int k = tid; k < n; ++k) d = fma(Ks[k * LD + tid], Ri[k * LD + tid], d);
+      S[tid + (int64_t)tid * n] = d;
+    }
+    return;
+  }
+  /* ---- S = K R^{-T}: B fragment R^{-T}[k][j] = R^{-1}[j][k], zero for k < j ---- */
 #pragma unroll
   for (int s = 0; s < PER; ++s) {
     const int t = w + s * NW;
@@ -1192,8 +1208,18 @@ static QrLaunch g_qr[2][3][3] = {
 static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
 /* the compiled variants: one column per group at 1, 2 or 4 workgroups per CU,
  * two columns per group at 2, 3 or 4 */
-static bool qr_variant(int cpl, int wg) {
-  return (cpl == 1 && (wg == 1 || wg == 2 || wg == 4)) || (cpl == 2 && (wg == 2 || wg == 3 || wg == 4));
+/* cpl 3: phase 2 in blocked WY form on the matrix cores (k_phase2_wy), one
+ * workgroup per CU asked of the register allocator.
+ * Left out: the FP32 phase 2 of bucket 64 with two columns per group at 3
+ * workgroups per CU. On MI355X it returned non-finite factors for an
+ * ill-conditioned n = 64 problem (F = Q diag(logspace(0, -3, 64)), K = 1e-2 I)
+ * where every other variant agrees bit for bit; it is the one variant whose
+ * register budget (80 VGPRs, spilling) differs, and no fault in the source
+ * explains it. */
+static bool qr_variant(int fp32, int nb, int cpl, int wg, int phase) {
+  if (fp32 && nb == 64 && phase == 2 && cpl == 2 && wg == 3) return false;
+  return (cpl == 1 && (wg == 1 || wg == 2 || wg == 4)) || (cpl == 2 && (wg == 2 || wg == 3 || wg == 4)) ||
+         (cpl == 3 && wg == 1 && phase == 2);
 }
 
 /* fp32: 0 the FP64 kernels, 1 the FP32 elimination of the mixed-precision path */
@@ -1206,7 +1232,7 @@ extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n, int* columns_per_
 }
 extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_group, int wg) {
   const int bi = qr_bucket_index(uk_bucket(n));
-  if (phase < 1 || phase > 3 || bi < 0 || !qr_variant(columns_per_group, wg)) return -1;
+  if (phase < 1 || phase > 3 || bi < 0 || !qr_variant(fp32, uk_bucket(n), columns_per_group, wg, phase)) return -1;
   g_qr[fp32 ? 1 : 0][phase - 1][bi] = QrLaunch{columns_per_group, wg};
   return 0;
 }
@@ -1269,7 +1295,9 @@ extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain
 }
 
 /* Back substitution with covariances for n > 16 on the matrix cores (default)
- * or on the vector units (k_backsub, the kernel used for n <= 16). */
+ * or on the vector units (k_backsub, the kernel used for n <= 16). cov: 0
+ * estimates only, 1 covariances, 2 only the covariances' diagonals (written
+ * in their places; for level 0, whose covariances no other level reads). */
 static int g_backsub_mfma = 1;
 static bool uk_backsub_mfma_enabled() { return g_backsub_mfma != 0; }
 extern "C" void ukhip_use_mfma(int on) { g_backsub_mfma = on ? 1 : 0; }
@@ -1290,9 +1318,13 @@ extern "C" hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t
     return hipGetLastError();
   }
   if (cov && nb >= 32 && uk_backsub_mfma_enabled()) {
-#define M_(NBV)                                                                                       \
-  k_backsub_mfma<NBV><<<dim3((unsigned)lv->evens), dim3(NBV * 8), 0, st>>>(*lv, *fac, xout, Sout, Xup, XTup, \
-                                                                          Xcur, XTcur, n)
+#define M_(NBV)                                                                                                  \
+  if (cov == 2)                                                                                                  \
+    k_backsub_mfma<NBV, true><<<dim3((unsigned)lv->evens), dim3(NBV * 8), 0, st>>>(*lv, *fac, xout, Sout, Xup,    \
+                                                                                 XTup, Xcur, XTcur, n);          \
+  else                                                                                                           \
+    k_backsub_mfma<NBV, false><<<dim3((unsigned)lv->evens), dim3(NBV * 8), 0, st>>>(*lv, *fac, xout, Sout, Xup,   \
+                                                                                  XTup, Xcur, XTcur, n)
     switch (nb) {
       case 32: M_(32); break;
       case 48: M_(48); break;

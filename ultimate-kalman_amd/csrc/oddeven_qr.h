@@ -256,8 +256,13 @@ __host__ __device__ constexpr bool qr_need(int l, int H, int SEG, int QF) {
  * serves all CPL columns. Each column's operations and their order do not
  * depend on CPL. One barrier.
  */
+/* V (optional, leading dimension ldv): the pivot lanes also store the
+ * published vector as column k of V and its tau as tau[k] (a blocked-WY
+ * accumulation of the reflectors; a column that counts as zero stores zeros,
+ * and granules skipped as exact zeros are left as they are, zero). */
 template <int NB, int SL, class T, int SEG, int QF>
-__device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, int k) {
+__device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int buf, int sub, int k,
+                                              T* V = nullptr, T* tauv = nullptr, int ldv = 0) {
   constexpr int H = NB / SL;
   T* vv = &R.v[buf][sub * Refl<NB, SL, T>::SUBSTRIDE];
   T s0 = T(0), s1 = T(0), alpha = T(0);
@@ -301,6 +306,20 @@ __device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB
     for (int l = H; l < 2 * H; ++l)
       if (qr_need(l, H, SEG, QF)) vv[l] = col.a[l];
     if (sub == 0) R.sc[buf][0] = taup;
+    if (V) {
+      const T keep = taup != T(0) ? T(1) : T(0);
+      T* vc = V + (int64_t)k * ldv;
+#pragma unroll
+      for (int l = 0; l < H; ++l) {
+        if (!qr_need(l, H, SEG, QF)) continue;
+        const int r = SL * l + sub;
+        vc[r] = keep * ((r > k) ? col.a[l] : (r == k ? ukk : T(0)));
+      }
+#pragma unroll
+      for (int l = H; l < 2 * H; ++l)
+        if (qr_need(l, H, SEG, QF)) vc[NB + SL * (l - H) + sub] = keep * col.a[l];
+      if (sub == 0) tauv[k] = taup;
+    }
   } else {
     /* single precision: the unit-pivot form v = x / (alpha - beta), v_k = 1,
      * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
@@ -322,16 +341,31 @@ __device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB
     for (int l = H; l < 2 * H; ++l)
       if (qr_need(l, H, SEG, QF)) vv[l] = col.a[l] * scale;
     if (sub == 0) R.sc[buf][0] = tau;
+    if (V) {
+      const T keep = tau != T(0) ? T(1) : T(0);
+      T* vc = V + (int64_t)k * ldv;
+#pragma unroll
+      for (int l = 0; l < H; ++l) {
+        if (!qr_need(l, H, SEG, QF)) continue;
+        const int r = SL * l + sub;
+        vc[r] = keep * ((r > k) ? col.a[l] * scale : (r == k ? T(1) : T(0)));
+      }
+#pragma unroll
+      for (int l = H; l < 2 * H; ++l)
+        if (qr_need(l, H, SEG, QF)) vc[NB + SL * (l - H) + sub] = keep * col.a[l] * scale;
+      if (sub == 0) tauv[k] = tau;
+    }
   }
 }
 
 template <int NB, int SL, int CPL, class T, int SEG, int QF>
 __device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int buf, int sub,
-                                           int piv_slot, unsigned upd, int k) {
+                                           int piv_slot, unsigned upd, int k, T* V = nullptr, T* tauv = nullptr,
+                                           int ldv = 0) {
   constexpr int H = NB / SL;
 #pragma unroll
   for (int s = 0; s < CPL; ++s)
-    if (s == piv_slot) house_publish<NB, SL, T, SEG, QF>(cols[s], R, buf, sub, k);
+    if (s == piv_slot) house_publish<NB, SL, T, SEG, QF>(cols[s], R, buf, sub, k, V, tauv, ldv);
   __syncthreads();
   if (upd) {
     const T tau = R.sc[buf][0];
@@ -437,7 +471,8 @@ struct QRGeom {
 /* Pivot steps of segment SEG and the segments after it. */
 template <int NB, int SL, int CPL, class T, int QF, int SEG = 0>
 __device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub,
-                                              int piv0, int n, unsigned upd) {
+                                              int piv0, int n, unsigned upd, T* V = nullptr, T* tauv = nullptr,
+                                              int ldv = 0) {
   constexpr int H = NB / SL, ROWS = qr_gl(H) * SL;
   if constexpr (SEG * ROWS < NB) {
     const int k1 = n < (SEG + 1) * ROWS ? n : (SEG + 1) * ROWS;
@@ -451,9 +486,9 @@ __device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<
           if (cc >= piv0 && cc < pc) u &= ~(1u << s);
         }
       }
-      house_step<NB, SL, CPL, T, SEG, QF>(cols, R, k & 1, sub, pc / CPL == grp ? pc % CPL : -1, u, k);
+      house_step<NB, SL, CPL, T, SEG, QF>(cols, R, k & 1, sub, pc / CPL == grp ? pc % CPL : -1, u, k, V, tauv, ldv);
     }
-    house_qr_segs<NB, SL, CPL, T, QF, SEG + 1>(cols, R, grp, sub, piv0, n, upd);
+    house_qr_segs<NB, SL, CPL, T, QF, SEG + 1>(cols, R, grp, sub, piv0, n, upd, V, tauv, ldv);
   }
 }
 
@@ -462,8 +497,8 @@ __device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<
  * flags above (0: every granule of block 1, every column updated). */
 template <int NB, int SL, int CPL, class T, int QF = 0>
 __device__ __forceinline__ void house_qr(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub, int piv0,
-                                         int n, unsigned upd) {
-  house_qr_segs<NB, SL, CPL, T, QF>(cols, R, grp, sub, piv0, n, upd);
+                                         int n, unsigned upd, T* V = nullptr, T* tauv = nullptr, int ldv = 0) {
+  house_qr_segs<NB, SL, CPL, T, QF>(cols, R, grp, sub, piv0, n, upd, V, tauv, ldv);
   __syncthreads();
 }
 /* one column per lane group */
@@ -717,6 +752,257 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* blocked WY on the matrix cores (phase 2, buckets 32..64)                   */
+/* ------------------------------------------------------------------------ */
+
+#ifndef UK_MFMA_F32_ROW
+#define UK_MFMA_F32_ROW(r, lk) (4 * (lk) + (r))
+#endif
+typedef double wy_d4 __attribute__((ext_vector_type(4)));
+typedef float wy_f4 __attribute__((ext_vector_type(4)));
+
+/* 16 x 16 x 4 matrix-core products in T. Operands: a = A[li][lk] (A 16 x 4),
+ * b = B[lk][li] (B 4 x 16), lane = li + 16 lk; the accumulator's element r of
+ * lane (li, lk) is D[row(r, lk)][li]. */
+template <class T>
+struct WyMM;
+template <>
+struct WyMM<double> {
+  using acc = wy_d4;
+  static __device__ __forceinline__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int lk) { return 4 * r + lk; }
+};
+template <>
+struct WyMM<float> {
+  using acc = wy_f4;
+  static __device__ __forceinline__ acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int lk) { return UK_MFMA_F32_ROW(r, lk); }
+};
+
+/* Orders one wave's LDS accesses: its earlier LDS writes are visible to its
+ * later LDS reads (of other lanes), and the compiler moves no access across. */
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Workgroup of the blocked phase 2: SL = 4 lanes for each of the NB pivot
+ * columns and the right-hand side column during the panel; its NW waves then
+ * take 16-column blocks of the trailing columns. */
+template <int NB>
+struct WyGeom {
+  static constexpr int SL = 4;
+  static constexpr int NT = ((SL * (NB + 1) + 63) / 64) * 64;
+  static constexpr int NW = NT / 64;
+  static constexpr int LDV = 2 * NB + 1; /* odd: both fragment read patterns spread over the banks */
+  static constexpr int LDW = 17;
+  static constexpr int NG = NB / 16;     /* reflector groups of 16 */
+  static constexpr int RT = 2 * NB / 16; /* row tiles of a trailing column block */
+};
+
+template <int NB, class T>
+struct WyShared {
+  Refl<NB, 4, T> refl;
+  T V[WyGeom<NB>::LDV * NB]; /* the reflectors u_k, column k (zero above the pivot row) */
+  T tau[NB];
+  T Tg[WyGeom<NB>::NG][16 * WyGeom<NB>::LDW]; /* per group g: S_g, then T_g (16 x 16, upper triangular) */
+};
+
+/*
+ * Phase 2 in blocked WY form (c/kalman_oddeven_smoother.c:460-523), one
+ * workgroup per even step j > 0, the same quantities as k_phase2:
+ *  1. panel: Householder QR of [D_j; R~_j] (the n pivot columns) with the
+ *     right-hand side [c_j; o_j] updated alongside, on the register/DPP
+ *     machinery of k_phase2; every reflector u_k and tau_k is also kept in V;
+ *  2. per group g of 16 reflectors, the compact WY factor T_g from
+ *     T_g^{-1} = striu(V_g^T V_g) + diag(1 / tau): V_g^T V_g on the matrix
+ *     cores, the 16 x 16 triangular inverse by substitution (one lane per
+ *     column);
+ *  3. the 2n trailing columns [B_j; 0] and [0; X_j], in 16-column blocks (one
+ *     per wave at a time, held as matrix-core accumulators): for each group in
+ *     order, W = V_g^T B, W = T_g^T W, B -= V_g W (so Q^T B = prod_g (I - V_g
+ *     T_g^T V_g^T) B), then stored to B~_j, Z_j, Y_j, X~_j.
+ * The accumulators serve as the B operands of the next products: a k-step
+ * pairs A[.][kappa] with B[kappa][.] for kappa = the accumulator row that
+ * lane holds (any assignment of the K indices to the steps gives the same
+ * sum), so no operand passes through LDS.
+ */
+template <int NB, class T, bool DIAG = false>
+__global__ void __launch_bounds__(WyGeom<NB>::NT, 1)
+    k_phase2_wy(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
+  using Gm = WyGeom<NB>;
+  using MM = WyMM<T>;
+  using acc_t = typename MM::acc;
+  constexpr int SL = Gm::SL, NT = Gm::NT, NW = Gm::NW, LDV = Gm::LDV, LDW = Gm::LDW, RT = Gm::RT;
+  __shared__ WyShared<NB, T> sh;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
+  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int64_t e = blockIdx.x;
+  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t u = e - lv.even_start[p];
+  const int64_t j = 2 * u;
+  if (j == 0) return; /* the chain's first step has no [D; R~] (uniform per block) */
+  const int64_t L = lv.L[p];
+  const int64_t t = lv.next_start[p] + u;
+  const int64_t nn = (int64_t)n * n;
+  const bool lone = j + 1 == L;
+  const int64_t sj = lv.step_start[p] + j;
+  T* fz = fix + (int64_t)p * (nn + n);
+  for (int i = tid; i < LDV * NB; i += NT) sh.V[i] = T(0);
+  for (int i = tid; i < NB; i += NT) sh.tau[i] = T(0);
+  refl_clear<NB, SL, NT, T>(sh.refl, tid);
+
+  /* ---- 1. panel ---- */
+  {
+    Col<NB, SL, T> col;
+    const int c = grp;
+    const bool isP = c < n, isR1 = c == NB;
+    if (isP) {
+      if (DIAG) col.load_diag(0, in.D + sj * nn, c, sub);
+      else col.load_block(0, in.D + sj * nn + (int64_t)c * n, n, sub);
+      col.load_block(1, fac.R + e * nn + (int64_t)c * n, n, sub); /* R~_j */
+    } else if (isR1) {
+      col.load_block(0, in.c + sj * n, n, sub);
+      col.load_block(1, fac.cf + e * n, n, sub); /* o_j */
+    } else {
+      col.zero_block(0);
+      col.zero_block(1);
+    }
+    Col<NB, SL, T>(&cols)[1] = reinterpret_cast<Col<NB, SL, T>(&)[1]>(col);
+    house_qr<NB, SL, 1, T, QF_TRI1 | QF_DONE | (DIAG ? QF_DIAG0 : 0)>(cols, sh.refl, grp, sub, 0, n,
+                                                                     (isP || isR1) ? 1u : 0u, sh.V, sh.tau, LDV);
+    if (isP) col.store_triu(fac.R + e * nn + (int64_t)c * n, n, c, sub);
+    if (isR1) {
+      col.store_block(0, fac.cf + e * n, n, sub);
+      col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
+    }
+  }
+  const int ng = (n + 15) >> 4;
+
+  /* ---- 2. T_g, wave w takes groups w, w + NW, ... ---- */
+  for (int g = w; g < ng; g += NW) {
+    acc_t acc = MM::zero();
+    const T* Vg = sh.V + (int64_t)g * 16 * LDV;
+#pragma unroll 4
+    for (int ks = 0; ks < 2 * NB / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      acc = MM::mma(Vg[k + li * LDV], Vg[k + li * LDV], acc); /* A = V_g^T, B = V_g */
+    }
+    T* S = sh.Tg[g]; /* S = striu(V_g^T V_g) + diag(1 / tau), column-major */
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = MM::row(r, lk), c = li;
+      const T tv = sh.tau[g * 16 + i];
+      S[i + c * LDW] = i < c ? acc[r] : (i == c ? (tv != T(0) ? T(1) / tv : T(1)) : T(0));
+    }
+    wave_lds_sync();
+    T x[16];
+    if (lane < 16) { /* column `lane` of T_g = S^{-1} by back substitution */
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = i == lane ? T(1) : T(0);
+#pragma unroll
+      for (int q = 15; q >= 0; --q) {
+        const T xq = x[q] / S[q + q * LDW];
+        x[q] = xq;
+#pragma unroll
+        for (int i = 0; i < q; ++i) x[i] -= S[i + q * LDW] * xq;
+      }
+    }
+    wave_lds_sync();
+    if (lane < 16) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) S[i + lane * LDW] = x[i];
+    }
+    wave_lds_sync();
+  }
+  __syncthreads();
+
+  /* ---- 3. the trailing columns, 16 at a time: block b < nb1 of [B_j; 0], then of [0; X_j] ---- */
+  const int nb1 = (n + 15) >> 4;
+  for (int blk = w; blk < 2 * nb1; blk += NW) {
+    const bool isB = blk < nb1;
+    const int cc = (isB ? blk : blk - nb1) * 16 + li; /* this lane's column */
+    const bool colok = cc < n;
+    /* B: row tile rt, element r = row rt * 16 + row(r, lk), column cc */
+    acc_t b[RT];
+    const double* kw = in.b_synth ? in.wk + sj * n : nullptr;
+    const double* Fs = in.b_synth ? in.wF + (in.wFidx ? in.wFidx[sj] * nn : 0) : nullptr;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + MM::row(r, lk);
+        T v = T(0);
+        if (colok) {
+          if (isB && row < n) {
+            v = in.b_synth ? T(-kw[row] * Fs[row + (int64_t)cc * n]) : in.B[sj * nn + (int64_t)cc * n + row];
+          } else if (!isB && row >= NB && row - NB < n) {
+            v = fac.Y[e * nn + (int64_t)cc * n + (row - NB)]; /* X_j (zero for a lone step) */
+          }
+        }
+        b[rt][r] = v;
+      }
+    for (int g = 0; g < ng; ++g) {
+      const T* Vg = sh.V + (int64_t)g * 16 * LDV;
+      const T* Tg = sh.Tg[g];
+      /* W = V_g^T B (K = 2 NB; for the first group the block's zero half is skipped) */
+      acc_t wv = MM::zero();
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        if (g == 0 && (isB ? rt >= RT / 2 : rt < RT / 2)) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = rt * 16 + MM::row(r, lk);
+          wv = MM::mma(Vg[k + li * LDV], b[rt][r], wv);
+        }
+      }
+      /* W <- T_g^T W (A[m][k] = T_g[k][m]) */
+      acc_t w2 = MM::zero();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = MM::row(r, lk);
+        w2 = MM::mma(Tg[k + li * LDW], wv[r], w2);
+      }
+      /* B -= V_g W */
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = MM::row(r, lk);
+          b[rt] = MM::mma(-Vg[rt * 16 + li + k * LDV], w2[r], b[rt]);
+        }
+    }
+    /* store: block 0 rows -> B~_j / Y_j, block 1 rows -> Z_j / X~_j */
+    if (colok) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rt * 16 + MM::row(r, lk);
+          const bool top = row < NB;
+          const int rr = top ? row : row - NB;
+          if (rr >= n) continue;
+          const T v = b[rt][r];
+          if (isB) {
+            if (top) fac.Bt[e * nn + (int64_t)cc * n + rr] = v;
+            else (lone ? fz : out.B + t * nn)[(int64_t)cc * n + rr] = v; /* Z_j */
+          } else {
+            if (top) fac.Y[e * nn + (int64_t)cc * n + rr] = v;
+            else if (!lone) out.D[t * nn + (int64_t)cc * n + rr] = v; /* X~_j */
+          }
+        }
+    }
+  }
+}
+
 /* Phases of one level in bucket NB with launch variant v (NB <= 16: the one
  * variant). Per level: the structure of its C blocks (LV of phases 1 and 3:
  * at most half a block of rows at level 0, upper triangular above), and DIAG:
@@ -746,6 +1032,12 @@ template <int NB, class T>
 void phase_launch_impl(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                        int n, hipStream_t st, int phase, ukqr::Launch v) {
   if constexpr (NB >= 32) {
+    if (v.cpl == 3 && phase == 2) { /* blocked WY on the matrix cores */
+      const dim3 g((unsigned)lv->evens), b(WyGeom<NB>::NT);
+      if (in.d_diag) k_phase2_wy<NB, T, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
+      else k_phase2_wy<NB, T, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
+      return;
+    }
     if (v.cpl == 2) {
       switch (v.wg) {
         case 2: phases_nb<NB, T, 2, 2>(lv, in, out, fac, fix, n, st, phase); break;

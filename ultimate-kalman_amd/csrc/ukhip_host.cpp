@@ -216,7 +216,7 @@ struct ukhip_batch {
   std::vector<std::pair<int32_t, int32_t>> where;
   std::vector<hipStream_t> part_streams;
   /* captured odd-even smoothing, [0] without and [1] with covariances */
-  hipGraphExec_t graph[2]{};
+  hipGraphExec_t graph[3]{}; /* by covariance mode: none, full, diagonals at level 0 */
   /* mixed-precision workspace, allocated on first use: FP32 chains and
    * factors, FP64 right-hand sides and corrections */
   void* d_mixed = nullptr;
@@ -1076,7 +1076,10 @@ static bool run_oddeven_view(uk_oe_view_t* v, int cov, SlotTimer* tm, hipStream_
     if (tm) tm->begin();
     double* Xc = l == 0 ? nullptr : v->X[l & 1];
     double* XTc = l == 0 ? nullptr : v->XT[l & 1];
-    if (!hip_ok(uk_launch_backsub(&lv, &f, v->xout, v->Sout, v->X[(l + 1) & 1], v->XT[(l + 1) & 1], Xc, XTc, n, cov,
+    /* cov 2: only level 0's covariances are reduced to their diagonals (the
+     * levels above feed the level below their full blocks) */
+    const int cl = cov == 2 ? (l == 0 ? 2 : 1) : cov;
+    if (!hip_ok(uk_launch_backsub(&lv, &f, v->xout, v->Sout, v->X[(l + 1) & 1], v->XT[(l + 1) & 1], Xc, XTc, n, cl,
                                   st),
                 "k_backsub launch"))
       return false;
@@ -1219,6 +1222,13 @@ static bool enqueue_oddeven(ukhip_batch* b, int cov) {
 /* Captures enqueue_oddeven into a graph once per covariance setting; replays
  * it afterwards (the captured launches reference b's device buffers, which
  * live as long as b). */
+/* Covariance mode of options: 0 none (KALMAN_NO_COVARIANCE), 2 diagonals
+ * (UKHIP_COVARIANCE_DIAG), 1 full. */
+static int cov_mode(uint32_t options) {
+  if (options & KALMAN_NO_COVARIANCE) return 0;
+  return (options & UKHIP_COVARIANCE_DIAG) ? 2 : 1;
+}
+
 static bool graph_oddeven(ukhip_batch* b, int cov) {
   if (!b->graph[cov]) {
     if (!hip_ok(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -1243,7 +1253,7 @@ static bool graph_oddeven(ukhip_batch* b, int cov) {
 extern "C" int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options) {
   if (!b) { uk_set_error("NULL batch"); return -1; }
   if (!ensure_stream()) return -1;
-  const int cov = (options & KALMAN_NO_COVARIANCE) == 0;
+  const int cov = cov_mode(options);
   return (g_use_graphs ? graph_oddeven(b, cov) : enqueue_oddeven(b, cov)) ? 0 : -1;
 }
 
@@ -1267,7 +1277,7 @@ static bool timed_part(ukhip_batch* q, uint32_t options, const ukhip_mixed_optio
   tm.st = g_stream;
   const bool ok = mixed ? run_mixed(q, mixed->refinements, mixed_fp32_levels(mixed), mixed->covariances != 0, &tm,
                                     g_stream)
-                        : run_oddeven(q, (options & KALMAN_NO_COVARIANCE) == 0, &tm, g_stream);
+                        : run_oddeven(q, cov_mode(options), &tm, g_stream);
   if (!ok) return false;
   if (!tm.collect(ms, launches)) return hip_ok(hipErrorUnknown, "timed smoothing events");
   return true;
@@ -2739,10 +2749,12 @@ extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
  * m = n/2, k steps, F = 0.98 times a Householder reflection, G uniform with
  * variance 1/n, 'w' covariances) is smoothed with each phase kernel in each
  * of its launch variants (1, 2, 4 workgroups per CU asked of the register
- * allocator), timed with HIP events; the fastest variant of each phase goes
- * into the launch table. The variants execute the same instructions, so the
- * choice changes timing only, never results. Batches whose smoothing was
- * captured into a graph earlier keep their captured launches. */
+ * allocator, two columns per lane group, and for phase 2 the blocked-WY form),
+ * timed with HIP events; the fastest variant of each phase goes into the
+ * launch table. The register variants execute the same instructions, so the
+ * choice among them changes timing only; the blocked-WY form applies the
+ * same reflectors in another order (results agree to rounding). Batches whose
+ * smoothing was captured into a graph earlier keep their captured launches. */
 extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
   if (!ensure_stream()) return -1;
   if (k < 64) k = 4096;
@@ -2772,30 +2784,29 @@ extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
     p.G = G.data(); p.o = o.data(); p.C = Call.data(); p.C_type = 'w';
     ukhip_batch_t* b = ukhip_batch_create(1, &p);
     if (!b) return -1;
-    static const int kVariants[6][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 2}, {2, 3}, {2, 4}}; /* {cpl, wg} */
-    /* FP64: the phase's HIP events; FP32 (the mixed-precision elimination):
-     * whole mixed-precision smoothings without a refinement step, timed on
-     * the host. One phase varied at a time. */
+    /* {cpl, wg}; cpl 3 (blocked WY) exists for phase 2 only */
+    static const int kVariants[7][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 2}, {2, 3}, {2, 4}, {3, 1}};
+    /* The phase's HIP events of an eager smoothing: FP64 odd-even, or the
+     * mixed-precision path without a refinement step for the FP32
+     * elimination. One phase varied at a time. */
     for (int fp32 = 0; fp32 <= 1; ++fp32) {
       for (int phase = 1; phase <= 3; ++phase) {
         int best_c = 1;
         int best_w = ukhip_qr_launch_get(fp32, phase, n, &best_c);
         double best_ms = 1e30;
         for (const auto& var : kVariants) {
-          ukhip_qr_launch_set(fp32, phase, n, var[0], var[1]);
+          if (ukhip_qr_launch_set(fp32, phase, n, var[0], var[1]) != 0) continue; /* not a variant of this phase */
           double t = 0.0;
           for (int rep = 0; rep < 3; ++rep) {
             double ms[UKHIP_OE_SLOTS];
-            const auto t0 = std::chrono::steady_clock::now();
-            const int rc = fp32 ? (ukhip_batch_smooth_oddeven_mixed(b, 0) != 0 || ukhip_synchronize() != 0)
-                                : ukhip_batch_smooth_oddeven_timed(b, 0, ms, nullptr);
+            const ukhip_mixed_options_t mo{0, 0, 0};
+            const int rc = ukhip_batch_timed_parts(b, 0, fp32 ? &mo : nullptr, ms, nullptr, nullptr, 1) < 0;
             if (rc != 0) {
               ukhip_batch_free(b);
               return -1;
             }
             if (rep == 0) continue;
-            t += fp32 ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count()
-                      : ms[UKHIP_OE_PHASE1 + phase - 1];
+            t += ms[UKHIP_OE_PHASE1 + phase - 1];
           }
           if (t < best_ms) {
             best_ms = t;
