@@ -79,6 +79,11 @@ def bucket(n):
     return next(b for b in (8, 16, 32, 48, 64) if n <= b)
 
 
+def bucket_qr(n):
+    """The bucket of the elimination's and the refinement sweeps' kernels (24 for 17..24)."""
+    return next(b for b in (8, 16, 24, 32, 48, 64) if n <= b)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -102,6 +107,7 @@ def kernel_label(slot, n, m, fp32=False, cov=True, all_w=True, h_ident=True, lev
     state dimension n with m observation rows; level None covers every level
     (the phase kernels' per-level structure flags shown as *)."""
     nb = bucket(n)
+    nq = bucket_qr(n)
     T = "float" if fp32 else "double"
     if slot == "k_weigh":
         return "k_weigh_diag<" + ("true" if fp32 else "false") + ">" if all_w else f"k_weigh<{bucket(max(n, m))}>"
@@ -110,20 +116,23 @@ def kernel_label(slot, n, m, fp32=False, cov=True, all_w=True, h_ident=True, lev
             return f"k_backsub_mfma<{nb}>" if cov else f"k_estimate<{nb}>"
         return f"k_backsub<{nb}>"
     if slot == "k_solve_est":
-        return f"k_solve_est<{nb}, {T}, *>"
+        return f"k_solve_est<{nq}, {T}, *>"
     if slot == "k_normal":
-        return f"k_normal<{bucket(max(n, m))}, *> + k_gather + k_add"
+        nm = bucket_qr(max(n, m))
+        return f"{'k_normal_w' if nm <= 32 else 'k_normal'}<{nm}, *> + k_gather + k_add"
     if slot == "k_fwd":
-        return f"k_fwd_even<{nb}, {T}> + k_fwd_odd<{nb}, {T}>"
+        return f"k_fwd_even<{nq}, {T}> + k_fwd_odd<{nq}, {T}>"
     if slot == "convert":
         return "k_to32 / k_to64"
     phase = int(slot[-1])
-    if nb >= 32:
-        v = device.launch_table()[f"phase{phase}<{nb}, {T}>"]
+    if nq >= 24:
+        v = device.launch_table()[f"phase{phase}<{nq}, {T}>"]
         cpl, wg = int(v[1]), int(v[3])
     else:
         cpl, wg = 1, 0
-    return f"{slot}<{nb}, {T}, {wg}, {cpl}, *>"
+    if cpl == 3:
+        return f"k_phase2_wy<{nq}, {T}, *>"
+    return f"{slot}<{nq}, {T}, {wg}, {cpl}, *>"
 
 
 def traffic_of(kernel, workload):

@@ -72,12 +72,15 @@ void ukhip_use_mfma(int enable);
  * the matrix columns each group of lanes holds (columns_per_group; 2 shares
  * one register copy of each Householder vector between two columns) and the
  * workgroups per CU the register allocation leaves room for: 1, 2 or 4 with
- * one column per group, 2, 3 or 4 with two. ukhip_autotune times every
- * variant on a synthetic problem of k steps (0: 4096) per bucket up to n_max
- * and keeps the fastest; the variants run the same operations on every
- * column, so results do not change. Graphs captured earlier keep their
- * launches. get returns the workgroups per CU (0: no such entry) and stores
- * the columns per group; set and autotune return 0 or -1. */
+ * one column per group, 2, 3 or 4 with two; columns_per_group 3 (phase 2,
+ * buckets 32, 48, 64, one workgroup per CU) is the blocked-WY phase 2 on the
+ * matrix cores. The buckets are n in 17..24, 25..32, 33..48 and 49..64.
+ * ukhip_autotune times every variant on a synthetic problem of k steps (0:
+ * 4096) per bucket up to n_max and keeps the fastest; the register variants
+ * run the same operations on every column, so choosing among them does not
+ * change results (the blocked-WY form agrees to rounding). Graphs captured
+ * earlier keep their launches. get returns the workgroups per CU (0: no such
+ * entry) and stores the columns per group; set and autotune return 0 or -1. */
 int ukhip_autotune(int32_t n_max, int32_t k);
 int ukhip_qr_launch_get(int fp32, int phase, int n, int* columns_per_group);
 int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_group, int workgroups_per_cu);

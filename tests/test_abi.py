@@ -243,7 +243,7 @@ def test_launch_table_set_and_get():
     L = device.lib()
     saved = device.launch_table()
     try:
-        assert len(saved) == 18
+        assert len(saved) == 24
         assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4", "c3w1"}
         cpl = C.c_int(0)
         assert L.ukhip_qr_launch_set(1, 2, 48, 2, 3) == 0

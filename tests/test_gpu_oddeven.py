@@ -270,3 +270,37 @@ def test_diagonal_weights_with_and_without_identity_H(dev, oracle, H_kind):
     xs, cs = dev.smooth_equations(pr, "oddeven")
     ox, oc = oracle.oddeven(pr)
     assert max_rel_err(xs, ox) < TOL and max_rel_err(cs, oc) < TOL
+
+
+@pytest.mark.parametrize("n", [17, 20, 24])
+def test_bucket24(dev, oracle, n):
+    """State dimensions 17..24 run the elimination and the refinement sweeps
+    in bucket 24 (uk_bucket_qr), whose half-block structure keeps 16 rows:
+    m = 16 takes the half-block kernels, m = 17 and m = n the dense ones (one
+    batch per m: the structure is chosen per batch part). FP64 estimates and
+    covariances against the oracle; the mixed path after two refinement steps
+    against the FP64 estimates; 'w' and 'C' weights."""
+    F = 0.97 * P.random_orthogonal(P.rng_for(960 + n), n)
+    _oracle_threads(oracle)
+    worst, worst_mixed = 0.0, 0.0
+    try:
+        for m, kind in ((5, "w"), (16, "C"), (min(17, n), "w"), (n, "C")):
+            probs = [P.iid_problem(k, n, m, seed=970 + k + m, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1, K_kind=kind,
+                                   C_kind=kind, unobserved=lambda i: i % 10 == 9, name=f"n{n}_m{m}_{kind}_k{k}")
+                     for k in ((3, 130, 517) if 3 * m >= n else (7, 130, 517))]  # k m >= n: a unique solution
+            b = dev.Batch(probs)
+            try:
+                b.smooth("oddeven")
+                fp64 = [b.download(i) for i in range(len(probs))]
+                b.smooth_mixed_opt(2, 0, False)
+                for i, pr in enumerate(probs):
+                    ox, oc = oracle.oddeven(pr)
+                    e = max(max_rel_err(list(fp64[i][0]), ox), max_rel_err(list(fp64[i][1]), oc))
+                    em = max_rel_err(list(b.download(i, want_cov=False)[0]), list(fp64[i][0]))
+                    worst, worst_mixed = max(worst, e), max(worst_mixed, em)
+                    assert e < TOL and em < TOL, pr.name
+            finally:
+                b.free()
+        say(f"n={n}: FP64 vs oracle {worst:.1e}, mixed (2 refinements) vs FP64 {worst_mixed:.1e}")
+    finally:
+        oracle.lib.oracle_set_threads(1)

@@ -5,9 +5,9 @@ replaces and the oracle.
 Bars: FP64 estimates and covariances within 1e-9 of the oracle and 1e-12 of
 the register kernel (the same reflectors applied in another order); the FP32
 elimination in WY form, after two FP64 refinement steps, within 1e-9 of the
-FP64 estimates. Buckets 32, 48 and 64; diagonal whitening (level-0 D stored
-as its diagonal, B formed on load) and dense whitening; odd lengths (lone last
-steps) and short problems.
+FP64 estimates. Buckets 32, 48 and 64 (bucket 24 has no blocked form);
+diagonal whitening (level-0 D stored as its diagonal, B formed on load) and
+dense whitening; odd lengths (lone last steps) and short problems.
 """
 import ctypes as C
 import os
@@ -45,7 +45,7 @@ def _set_phase2(dev, fp32, n, variant):
     return (cpl.value, wg)
 
 
-@pytest.mark.parametrize("n,m,kind", [(20, 10, "w"), (32, 16, "C"), (40, 20, "w"), (48, 24, "C"), (64, 32, "w")])
+@pytest.mark.parametrize("n,m,kind", [(28, 14, "w"), (32, 16, "C"), (40, 20, "w"), (48, 24, "C"), (64, 32, "w")])
 def test_wy_phase2_matches(dev, oracle, n, m, kind):
     F = 0.98 * P.random_orthogonal(P.rng_for(900 + n), n)
     probs = [P.iid_problem(k, n, m, seed=910 + k + n, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1, K_kind=kind,

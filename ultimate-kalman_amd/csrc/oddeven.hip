@@ -1122,6 +1122,14 @@ extern "C" int uk_bucket(int n) {
   if (n <= 64) return 64;
   return -1;
 }
+/* The elimination's and the refinement sweeps' buckets: uk_bucket's, and 24
+ * for 16 < n <= 24 (three quarters of bucket 32's rows and lanes). The other
+ * kernels keep uk_bucket's; every bucket serves any n up to its size on the
+ * same layouts (blocks stored n x n). */
+extern "C" int uk_bucket_qr(int n) {
+  if (n > 16 && n <= 24) return 24;
+  return uk_bucket(n);
+}
 
 extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st) {
   if (!raw->K_full && !raw->C_full) { /* every covariance given as 'w' */
@@ -1187,25 +1195,27 @@ extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t*
   return hipGetLastError();
 }
 
-/* Run-time launch table of the phase kernels at NB >= 32 (FP64, and the FP32
+/* Run-time launch table of the phase kernels at NB >= 24 (FP64, and the FP32
  * elimination of the mixed-precision path), per precision, phase and bucket
- * (32, 48, 64): the columns each lane group holds (cpl) and the workgroups per
+ * (32, 48, 64, 24): the columns each lane group holds (cpl) and the workgroups per
  * CU asked of the register allocator (wg). Starts from variants measured
  * fastest on MI355X (n = 24, 32, 48 and 64, k = 200000), rewritten by
  * ukhip_autotune. */
 struct QrLaunch {
   int cpl, wg;
 };
-static QrLaunch g_qr[2][3][3] = {
-    /* FP64:  bucket 32   48        64 */
-    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 1}},
-     /* phase 2 */ {{2, 2}, {1, 1}, {1, 1}},
-     /* phase 3 */ {{1, 2}, {2, 2}, {2, 3}}},
+static QrLaunch g_qr[2][3][4] = {
+    /* FP64:  bucket 32   48        64        24 */
+    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 1}, {2, 3}},
+     /* phase 2 */ {{2, 2}, {1, 1}, {1, 1}, {2, 2}},
+     /* phase 3 */ {{1, 2}, {2, 2}, {2, 3}, {1, 2}}},
     /* FP32 */
-    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 4}},
-     /* phase 2 */ {{2, 3}, {1, 4}, {2, 2}},
-     /* phase 3 */ {{1, 2}, {1, 1}, {2, 2}}}};
-static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : -1)); }
+    {/* phase 1 */ {{2, 3}, {2, 2}, {1, 4}, {2, 3}},
+     /* phase 2 */ {{2, 3}, {1, 4}, {2, 2}, {2, 3}},
+     /* phase 3 */ {{1, 2}, {1, 1}, {2, 2}, {1, 2}}}};
+static int qr_bucket_index(int nb) {
+  return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : (nb == 24 ? 3 : -1)));
+}
 /* the compiled variants: one column per group at 1, 2 or 4 workgroups per CU,
  * two columns per group at 2, 3 or 4 */
 /* cpl 3: phase 2 in blocked WY form on the matrix cores (k_phase2_wy), one
@@ -1219,20 +1229,20 @@ static int qr_bucket_index(int nb) { return nb == 32 ? 0 : (nb == 48 ? 1 : (nb =
 static bool qr_variant(int fp32, int nb, int cpl, int wg, int phase) {
   if (fp32 && nb == 64 && phase == 2 && cpl == 2 && wg == 3) return false;
   return (cpl == 1 && (wg == 1 || wg == 2 || wg == 4)) || (cpl == 2 && (wg == 2 || wg == 3 || wg == 4)) ||
-         (cpl == 3 && wg == 1 && phase == 2);
+         (cpl == 3 && wg == 1 && phase == 2 && nb % 16 == 0);
 }
 
 /* fp32: 0 the FP64 kernels, 1 the FP32 elimination of the mixed-precision path */
 extern "C" int ukhip_qr_launch_get(int fp32, int phase, int n, int* columns_per_group) {
-  const int bi = qr_bucket_index(uk_bucket(n));
+  const int bi = qr_bucket_index(uk_bucket_qr(n));
   if (phase < 1 || phase > 3 || bi < 0) return 0;
   const QrLaunch v = g_qr[fp32 ? 1 : 0][phase - 1][bi];
   if (columns_per_group) *columns_per_group = v.cpl;
   return v.wg;
 }
 extern "C" int ukhip_qr_launch_set(int fp32, int phase, int n, int columns_per_group, int wg) {
-  const int bi = qr_bucket_index(uk_bucket(n));
-  if (phase < 1 || phase > 3 || bi < 0 || !qr_variant(fp32, uk_bucket(n), columns_per_group, wg, phase)) return -1;
+  const int bi = qr_bucket_index(uk_bucket_qr(n));
+  if (phase < 1 || phase > 3 || bi < 0 || !qr_variant(fp32, uk_bucket_qr(n), columns_per_group, wg, phase)) return -1;
   g_qr[fp32 ? 1 : 0][phase - 1][bi] = QrLaunch{columns_per_group, wg};
   return 0;
 }
@@ -1241,10 +1251,11 @@ namespace {
 template <int NB, class T>
 void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                   int n, hipStream_t st, int phase) {
-  if constexpr (NB >= 32) {
+  if constexpr (NB >= 24) {
     const QrLaunch q = g_qr[sizeof(T) == 8 ? 0 : 1][phase - 1][qr_bucket_index(NB)];
     const ukqr::Launch v{q.cpl, q.wg};
-    if constexpr (NB == 32) ukqr::phase_launch_b32(lv, in, out, fac, fix, n, st, phase, v);
+    if constexpr (NB == 24) ukqr::phase_launch_b24(lv, in, out, fac, fix, n, st, phase, v);
+    else if constexpr (NB == 32) ukqr::phase_launch_b32(lv, in, out, fac, fix, n, st, phase, v);
     else if constexpr (NB == 48) ukqr::phase_launch_b48(lv, in, out, fac, fix, n, st, phase, v);
     else ukqr::phase_launch_b64(lv, in, out, fac, fix, n, st, phase, v);
   } else {
@@ -1255,7 +1266,7 @@ void phase_launch(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& ou
 template <class T>
 hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac,
                             T* fix, int n, hipStream_t st, hipEvent_t* ev) {
-  const int nb = uk_bucket(n > in.mr ? n : in.mr);
+  const int nb = uk_bucket_qr(n > in.mr ? n : in.mr);
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
@@ -1266,7 +1277,8 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
     mark(ph);                                                     \
   }
   switch (nb) {
-    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(24, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_)
+    UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }
 #undef L_

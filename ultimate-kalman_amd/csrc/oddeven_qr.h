@@ -1,8 +1,8 @@
 /*
  * oddeven_qr.h -- the QR kernels of the odd-even elimination (phases 1..3),
  * shared by oddeven.hip (n <= 16, and the device helpers its other kernels
- * use) and oddeven_qr{32,48,64}.hip (one translation unit per bucket above 16,
- * so that their many launch variants compile in parallel).
+ * use) and oddeven_qr{24,32,48,64}.hip (one translation unit per bucket above
+ * 16, so that their many launch variants compile in parallel).
  *
  * Reference: c/kalman_oddeven_smoother.c (phases :379-560, fix-last
  * :1452-1476, renaming :563-597).
@@ -44,6 +44,7 @@ struct Launch {
                            const FactorsT<double>& fac, double* fix, int n, hipStream_t st, int phase, Launch v);    \
   void phase_launch_b##NBV(const uk_level_t* lv, const ChainT<float>& in, const ChainT<float>& out,                   \
                            const FactorsT<float>& fac, float* fix, int n, hipStream_t st, int phase, Launch v);
+UKQR_DECLARE(24)
 UKQR_DECLARE(32)
 UKQR_DECLARE(48)
 UKQR_DECLARE(64)
@@ -219,8 +220,8 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  *     QF_TRI1: block 1 starts upper triangular in the pivot columns (R~ in
  *     phase 2; C = G~ of the levels above 0 in phases 1 and 3), and every
  *     reflector keeps it so: rows <= k, granules <= SEG;
- *     QF_HALF1: rows >= NB / 2 of block 1 are zero in every column and stay
- *     so: granules < NG / 2.
+ *     QF_HALF1: rows >= qr_half_rows of block 1 are zero in every column and
+ *     stay so: granules < ceil(NG / 2).
  *     QF_DIAG0: block 0 of the pivot columns starts diagonal (D_j in phase 2
  *     at level 0 when every H is the identity and every K is given as 'w'):
  *     reflector i reaches block 0 only at row i, so u_k's block-0 part is its
@@ -234,14 +235,22 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  */
 enum { QF_TRI1 = 1, QF_DONE = 2, QF_HALF0 = 4, QF_HALF1 = 8, QF_DIAG0 = 16 };
 __host__ __device__ constexpr int qr_gl(int H) { return H <= 16 ? 2 : 4; }
+/* the granules of a block that QF_HALF0 / QF_HALF1 keep: the first half,
+ * rounded up (bucket 24: 2 of 3) */
+__host__ __device__ constexpr int qr_half_granules(int H) { return (H / qr_gl(H) + 1) / 2; }
+/* rows of a block that QF_HALF0 / QF_HALF1 keep for bucket NB with SL lanes
+ * per column (NB / 2, and 16 in bucket 24) */
+__host__ __device__ constexpr int qr_half_rows(int NB, int SL) {
+  return qr_half_granules(NB / SL) * qr_gl(NB / SL) * SL;
+}
 __host__ __device__ constexpr bool qr_need(int l, int H, int SEG, int QF) {
-  const int ng = H / qr_gl(H);
+  const int hg = qr_half_granules(H);
   if (l < H) {
     const int g = l / qr_gl(H);
-    return g >= SEG && (!(QF & QF_HALF0) || g < ng / 2 || g == SEG) && (!(QF & QF_DIAG0) || g == SEG);
+    return g >= SEG && (!(QF & QF_HALF0) || g < hg || g == SEG) && (!(QF & QF_DIAG0) || g == SEG);
   }
   const int g = (l - H) / qr_gl(H);
-  return (!(QF & QF_TRI1) || g <= SEG) && (!(QF & QF_HALF1) || g < ng / 2);
+  return (!(QF & QF_TRI1) || g <= SEG) && (!(QF & QF_HALF1) || g < hg);
 }
 
 /*
@@ -325,11 +334,22 @@ __device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB
      * tau = (beta - alpha) / beta in [1, 2]; squaring the column's scale, as
      * the form above does, would leave FP32's range on nearly dependent
      * columns (m << n) */
+    /* The square root and the two reciprocals are the hardware's 1-ulp
+     * instructions (v_sqrt_f32, v_rcp_f32) instead of the correctly rounded
+     * sequences: they sit on the step's critical path, and an ulp in tau or
+     * in v's scale is the size of the FP32 rounding the factorization makes
+     * anyway (the FP64 refinement removes both). Sums of squares below
+     * 1e-28 are scaled by 2^64 first, keeping the square root's argument
+     * normal. */
     T tau = T(0), scale = T(0);
     if (s != T(0)) {
-      const T beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-      tau = (beta - alpha) / beta;
-      scale = T(1) / (alpha - beta);
+      const T t2 = fma(alpha, alpha, s);
+      const bool tiny = t2 < T(1e-28);
+      T nrm = __builtin_amdgcn_sqrtf(tiny ? t2 * T(0x1p64) : t2);
+      nrm = tiny ? nrm * T(0x1p-32) : nrm;
+      const T beta = -copysign(nrm, alpha);
+      tau = fma(-alpha, __builtin_amdgcn_rcpf(beta), T(1)); /* (beta - alpha) / beta */
+      scale = __builtin_amdgcn_rcpf(alpha - beta);
     }
 #pragma unroll
     for (int l = 0; l < H; ++l) {
@@ -533,8 +553,9 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * Columns (groups of SL lanes): P = [0,n) u_j, Q = [n,2n) u_{j+1},
  * R1 = 2n right-hand side, Bl = [2n+1,3n+1) u_{j-1}.
  */
-/* LV: the level's stacked rows (C) as structure: 0 dense, 1 at most NB / 2
- * rows (level 0 with m <= NB / 2), 2 upper triangular (the levels above 0) */
+/* LV: the level's stacked rows (C) as structure: 0 dense, 1 at most
+ * qr_half_rows rows (level 0 with m <= NB / 2; 16 in bucket 24), 2 upper
+ * triangular (the levels above 0) */
 template <int NB, class T, int WG = 0, int CPL = 1, int LV = 0>
 __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
@@ -583,7 +604,7 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
     if (isP || isQ || isR1) upd |= 1u << q;
     else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
-  /* LV 1: the stacked rows C_j, o_j fill at most NB / 2 rows; LV 2: C_j is
+  /* LV 1: the stacked rows C_j, o_j fill at most qr_half_rows rows; LV 2: C_j is
    * upper triangular (the pivot columns' block 1) */
   house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp, sub, 0, n,
                                                                                         upd);
@@ -728,7 +749,7 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
     else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
   }
   /* LV 1: both blocks (D~_{j+1}, C_{j+1} and the right-hand sides) fill at
-   * most NB / 2 rows; LV 2: C_{j+1} (block 1 of the pivot columns) is upper
+   * most qr_half_rows rows; LV 2: C_{j+1} (block 1 of the pivot columns) is upper
    * triangular */
   house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF0 | QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp,
                                                                                                   sub, 0, n, upd);
@@ -1010,7 +1031,9 @@ __global__ void __launch_bounds__(WyGeom<NB>::NT, 1)
 template <int NB, class T, int WG, int CPL>
 void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
                hipStream_t st, int phase) {
-  const int lvk = in.c_tri ? 2 : (2 * in.mr <= NB ? 1 : 0);
+  const int lvk = in.c_tri ? 2 : (in.mr <= qr_half_rows(NB, QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::SL) ? 1 : 0);
+  static_assert(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::SL == QRGeom<NB, NB + 1, 3, WG, CPL>::SL,
+                "phases 1 and 3 share the half-block row count");
   if (phase == 1) {
     const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT);
     if (lvk == 2) k_phase1<NB, T, WG, CPL, 2><<<g, b, 0, st>>>(*lv, in, out, fac, n);
@@ -1031,12 +1054,14 @@ void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, 
 template <int NB, class T>
 void phase_launch_impl(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix,
                        int n, hipStream_t st, int phase, ukqr::Launch v) {
-  if constexpr (NB >= 32) {
-    if (v.cpl == 3 && phase == 2) { /* blocked WY on the matrix cores */
-      const dim3 g((unsigned)lv->evens), b(WyGeom<NB>::NT);
-      if (in.d_diag) k_phase2_wy<NB, T, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
-      else k_phase2_wy<NB, T, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
-      return;
+  if constexpr (NB >= 24) {
+    if constexpr (NB % 16 == 0) {
+      if (v.cpl == 3 && phase == 2) { /* blocked WY on the matrix cores */
+        const dim3 g((unsigned)lv->evens), b(WyGeom<NB>::NT);
+        if (in.d_diag) k_phase2_wy<NB, T, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
+        else k_phase2_wy<NB, T, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
+        return;
+      }
     }
     if (v.cpl == 2) {
       switch (v.wg) {

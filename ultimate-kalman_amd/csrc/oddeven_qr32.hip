@@ -1,6 +1,6 @@
 /*
  * oddeven_qr32.hip -- the odd-even QR phase kernels of bucket 32 (state
- * dimensions 17..32) in every launch variant, FP64 and FP32.
+ * dimensions 25..32) in every launch variant, FP64 and FP32.
  */
 #include "oddeven_qr.h"
 

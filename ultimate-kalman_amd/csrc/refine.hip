@@ -52,6 +52,25 @@ __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob,
   return lo;
 }
 
+/* Row and column of element idx = lane + 64 t (t = 0, 1, ...) of a
+ * column-major block with `rows` rows: one division per lane, then 64
+ * elements further per step without one (a division by a run-time value is
+ * a long instruction sequence). */
+struct Walk64 {
+  int r, c, dr, dc, rows;
+  __device__ __forceinline__ Walk64(int lane, int rows_) : rows(rows_) {
+    dc = 64 / rows;
+    dr = 64 - dc * rows;
+    c = lane / rows;
+    r = lane - c * rows;
+  }
+  __device__ __forceinline__ void next() {
+    r += dr;
+    c += dc;
+    if (r >= rows) r -= rows, ++c;
+  }
+};
+
 /* dst[i] = (float) src[i] */
 __global__ void __launch_bounds__(256) k_to32(float* __restrict__ dst, const double* __restrict__ src, int64_t count) {
   for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (float)src[i];
@@ -191,6 +210,165 @@ __global__ void __launch_bounds__(256) k_normal(uk_chain_t ch, const int8_t* fir
   }
 }
 
+/* Orders one wave's LDS accesses (its writes before its later reads of other
+ * lanes' values); the compiler moves no access across it. */
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/*
+ * The same quantities as k_normal for NB <= 32, one wave per step and four
+ * steps per workgroup (no workgroup barrier: each wave orders its own LDS
+ * staging). Every product runs on all 64 lanes: lane i + 32 h (h = 0, 1)
+ * takes entry i of the result over half h of the summation range, and the
+ * halves are added across the wave.
+ */
+template <int NB, bool DD>
+__global__ void __launch_bounds__(256) k_normal_w(uk_chain_t ch, const int8_t* first, const int32_t* m_obs,
+                                                  const double* __restrict__ x, double* __restrict__ gA,
+                                                  double* __restrict__ gB, int n, int64_t steps) {
+  static_assert(NB <= 32, "one half-wave per row or column");
+  constexpr int L = NB + 1, WPB = 4, PS = (NB * NB + 63) / 64;
+  __shared__ double Ms[WPB][NB * L];            /* B, then C */
+  __shared__ double Ds[WPB][DD ? 1 : NB * L];   /* D when it is stored in full */
+  __shared__ double vec[WPB][5][NB];            /* x_s, x_{s-1}, re, ro, diag(D) */
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, i = lane & 31;
+  const int64_t s = (int64_t)blockIdx.x * WPB + w;
+  if (s >= steps) return; /* uniform per wave */
+  double* M = Ms[w];
+  double* Dm = Ds[w];
+  double* xs = vec[w][0];
+  double* xm = vec[w][1];
+  double* re = vec[w][2];
+  double* ro = vec[w][3];
+  double* dd = vec[w][4];
+  const int64_t nn = (int64_t)n * n;
+  const int mr = ch.mr;
+  const bool evo = first[s] == 0;
+  const int m = m_obs[s];
+  const int nb2 = n * n, cn = mr * n;
+  /* every element of the step's blocks in flight at once (indices clamped
+   * into the step's slots; values outside a block are never stored) */
+  double tcv[PS], tbv[PS], tdv[DD ? 1 : PS];
+#pragma unroll
+  for (int t = 0; t < PS; ++t) {
+    const int idx = lane + 64 * t;
+    tcv[t] = ch.C[s * nn + (idx < cn ? idx : 0)];
+  }
+  if (evo) {
+    if (ch.b_synth) {
+      const double* wFs = ch.wF + (ch.wFidx ? ch.wFidx[s] * nn : 0);
+      Walk64 wb(lane, n);
+#pragma unroll
+      for (int t = 0; t < PS; ++t) {
+        const int idx = lane + 64 * t, ic = idx < nb2 ? idx : 0;
+        tbv[t] = -ch.wk[s * n + wb.r] * wFs[ic]; /* row wb.r of B (any row for idx >= n^2) */
+        wb.next();
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < PS; ++t) {
+        const int idx = lane + 64 * t;
+        tbv[t] = ch.B[s * nn + (idx < nb2 ? idx : 0)];
+      }
+    }
+    if (DD) {
+      if (lane < n) dd[lane] = ch.D[s * nn + lane];
+    } else {
+#pragma unroll
+      for (int t = 0; t < (DD ? 1 : PS); ++t) {
+        const int idx = lane + 64 * t;
+        tdv[t] = ch.D[s * nn + (idx < nb2 ? idx : 0)];
+      }
+    }
+  }
+  if (lane < n) {
+    xs[lane] = x[s * n + lane];
+    xm[lane] = evo ? x[(s - 1) * n + lane] : 0.0;
+  }
+  /* summation range of this half: [j0, j0 + cn_h) of n, [r0, r0 + cm_h) of mr.
+   * The loops run a compile-time number of iterations (all LDS reads issue
+   * back to back) at indices clamped into the staged arrays; iterations past
+   * the range leave the sums as they are (a select, so stale LDS values are
+   * never used). */
+  constexpr int HB = (NB + 1) / 2;
+  const int hn = (n + 1) >> 1, hm = (mr + 1) >> 1;
+  const int j0 = h * hn, cn_h = h ? n - hn : hn;
+  const int r0 = h * hm, cm_h = h ? mr - hm : hm;
+  const int ir = i < n ? i : 0;   /* row / column index of this lane, clamped */
+  const int irc = i < mr ? i : 0; /* row of C, clamped */
+  auto cl = [](int v) { return v < NB ? v : NB - 1; };
+  double ga = 0.0, gb = 0.0;
+  if (evo) {
+    Walk64 wb(lane, n);
+#pragma unroll
+    for (int t = 0; t < PS; ++t) {
+      if (lane + 64 * t < nb2) {
+        if (!DD) Dm[wb.r + wb.c * L] = tdv[DD ? 0 : t];
+        M[wb.r + wb.c * L] = tbv[t];
+      }
+      wb.next();
+    }
+    wave_sync();
+    /* re = c - D x_s - B x_{s-1}, row i (D diagonal: its term once, in half 0) */
+    double p = (DD && h == 0) ? dd[ir] * xs[ir] : 0.0, q = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < HB; ++jj) {
+      const int j = cl(j0 + jj);
+      const bool on = jj < cn_h;
+      q = on ? fma(M[ir + j * L], xm[j], q) : q;
+      if (!DD) p = on ? fma(Dm[ir + j * L], xs[j], p) : p;
+    }
+    p += q;
+    p += __shfl_xor(p, 32);
+    if (h == 0 && i < n) re[i] = ch.c[s * n + i] - p;
+    wave_sync();
+    /* gA = D^T re, gB = B^T re, column i */
+    if (DD && h == 0) ga = dd[ir] * re[ir];
+#pragma unroll
+    for (int rr = 0; rr < HB; ++rr) {
+      const int r = cl(j0 + rr);
+      const bool on = rr < cn_h;
+      gb = on ? fma(M[r + ir * L], re[r], gb) : gb;
+      if (!DD) ga = on ? fma(Dm[r + ir * L], re[r], ga) : ga;
+    }
+    wave_sync(); /* B consumed: its block takes C */
+  }
+  if (m > 0) {
+    Walk64 wc(lane, mr);
+#pragma unroll
+    for (int t = 0; t < PS; ++t) {
+      if (lane + 64 * t < cn) M[wc.r + wc.c * L] = tcv[t]; /* C is mr x n, ld mr; rows >= m are zero */
+      wc.next();
+    }
+    wave_sync();
+    /* ro = o - C x_s, row i (rows >= m count as zero) */
+    double p = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < HB; ++jj) {
+      const int j = cl(j0 + jj);
+      p = jj < cn_h ? fma(M[irc + j * L], xs[j], p) : p;
+    }
+    p += __shfl_xor(p, 32);
+    if (h == 0 && i < mr) ro[i] = i < m ? ch.o[s * n + i] - p : 0.0;
+    wave_sync();
+    /* gA += C^T ro, column i */
+#pragma unroll
+    for (int rr = 0; rr < HB; ++rr) {
+      const int r = cl(r0 + rr);
+      ga = rr < cm_h ? fma(M[r + ir * L], ro[r], ga) : ga;
+    }
+  }
+  ga += __shfl_xor(ga, 32);
+  gb += __shfl_xor(gb, 32);
+  if (h == 0 && i < n) {
+    gA[s * n + i] = ga;
+    gB[s * n + i] = gb;
+  }
+}
+
 /* g_s = gA_s + gB_{s+1} (when step s+1 belongs to the same problem) */
 __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ gA, const double* __restrict__ gB,
                                                 const int8_t* first, double* __restrict__ g, int64_t steps, int n) {
@@ -227,10 +405,11 @@ __global__ void __launch_bounds__(64) k_fwd_even(uk_level_t lv, const T* __restr
       const int idx = lane + 64 * t;
       rr[t] = idx < n * n ? R[e * nn + idx] : T(0);
     }
+    Walk64 wr(lane, n);
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      const int idx = lane + 64 * t;
-      if (idx < n * n) Rs[idx % n + (idx / n) * L] = rr[t];
+      if (lane + 64 * t < n * n) Rs[wr.r + wr.c * L] = rr[t];
+      wr.next();
     }
   }
   double v = lane < n ? g[sj * n + lane] : 0.0;
@@ -284,14 +463,14 @@ __global__ void __launch_bounds__(64) k_fwd_odd(uk_level_t lv, const T* __restri
     yj[lane] = y[e * n + lane];
     yn[lane] = has_n ? y[(e + 1) * n + lane] : 0.0;
   }
+  Walk64 wy(lane, n);
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int idx = lane + 64 * t;
-    if (idx < n * n) {
-      const int r = idx % n, c = idx / n;
-      Ys[r + c * L] = ry[t];
-      Bs[r + c * L] = rb[t];
+    if (lane + 64 * t < n * n) {
+      Ys[wy.r + wy.c * L] = ry[t];
+      Bs[wy.r + wy.c * L] = rb[t];
     }
+    wy.next();
   }
   __syncthreads();
   if (lane < n) {
@@ -370,7 +549,7 @@ __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> f
 /* launchers                                                                  */
 /* ------------------------------------------------------------------------ */
 
-extern "C" int uk_bucket(int n);
+extern "C" int uk_bucket_qr(int n);
 
 #define UK_DISPATCH(NBV, CALL) \
   case NBV: { CALL(NBV); } break;
@@ -392,14 +571,22 @@ extern "C" hipError_t uk_launch_add(double* x, const double* d, int64_t count, h
 extern "C" hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first, const int32_t* m_obs,
                                        const double* x, double* gA, double* gB, double* g, int64_t steps, int n,
                                        hipStream_t st) {
-  const int nb = uk_bucket(n > ch->mr ? n : ch->mr);
+  const int nb = uk_bucket_qr(n > ch->mr ? n : ch->mr);
+  if (steps <= 0) return hipSuccess;
+  /* NB <= 32: one wave per step, four steps per workgroup */
+  const dim3 gw((unsigned)((steps + 3) / 4));
+#define W_(NBV)                                                                                              \
+  if (ch->d_diag) k_normal_w<NBV, true><<<gw, dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n, steps); \
+  else k_normal_w<NBV, false><<<gw, dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n, steps)
 #define L_(NBV)                                                                                       \
   if (ch->d_diag) k_normal<NBV, true><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n); \
   else k_normal<NBV, false><<<dim3((unsigned)steps), dim3(256), 0, st>>>(*ch, first, m_obs, x, gA, gB, n)
   switch (nb) {
-    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    UK_DISPATCH(8, W_) UK_DISPATCH(16, W_) UK_DISPATCH(24, W_) UK_DISPATCH(32, W_) UK_DISPATCH(48, L_)
+    UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }
+#undef W_
 #undef L_
   const int64_t blocks = std::min<int64_t>((steps * n + 255) / 256, 65536);
   k_gather<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(gA, gB, first, g, steps, n);
@@ -418,12 +605,13 @@ extern "C" hipError_t uk_launch_to64(double* dst, const float* src, int64_t coun
 template <class T>
 static hipError_t launch_fwd(const uk_level_t* lv, const T* R, const T* Bt, const T* Y, const double* g, double* y,
                              double* gnext, int n, hipStream_t st) {
-  const int nb = uk_bucket(n);
+  const int nb = uk_bucket_qr(n);
 #define L_(NBV)                                                                                   \
   k_fwd_even<NBV, T><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, R, g, y, n);            \
   k_fwd_odd<NBV, T><<<dim3((unsigned)lv->units), dim3(64), 0, st>>>(*lv, Bt, Y, g, y, gnext, n)
   switch (nb) {
-    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(24, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_)
+    UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }
 #undef L_
@@ -444,12 +632,13 @@ extern "C" hipError_t uk_launch_fwd64(const uk_level_t* lv, const uk_factors_t* 
 template <class T>
 static hipError_t launch_solve_est(const uk_level_t* lv, const FactorPtrs<T>& f, const float* rhs32,
                                    const double* rhs64, double* x, int n, hipStream_t st) {
-  const int nb = uk_bucket(n);
+  const int nb = uk_bucket_qr(n);
 #define L_(NBV)                                                                                                \
   if (rhs32) k_solve_est<NBV, T, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
   else k_solve_est<NBV, T, double><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
   switch (nb) {
-    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_) UK_DISPATCH(64, L_)
+    UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(24, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_)
+    UK_DISPATCH(64, L_)
     default: return hipErrorInvalidValue;
   }
 #undef L_

@@ -2745,7 +2745,7 @@ extern "C" void uk_ingest_rollback(uk_ingest* g, int64_t si) {
 
 
 
-/* For every bucket (32, 48, 64) up to n_max: a synthetic problem (n = bucket,
+/* For every bucket (24, 32, 48, 64) up to n_max: a synthetic problem (n = bucket,
  * m = n/2, k steps, F = 0.98 times a Householder reflection, G uniform with
  * variance 1/n, 'w' covariances) is smoothed with each phase kernel in each
  * of its launch variants (1, 2, 4 workgroups per CU asked of the register
@@ -2763,7 +2763,7 @@ extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
     st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
     return (double)((st * 0x2545F4914F6CDD1Dull) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
   };
-  for (int nb : {32, 48, 64}) {
+  for (int nb : {24, 32, 48, 64}) {
     if (nb > std::max(32, (int)n_max)) break;
     const int n = nb, m = nb / 2;
     const size_t nn = (size_t)n * n;

@@ -102,7 +102,7 @@ def launch_table():
     L = lib()
     out = {}
     for f, t in ((0, "double"), (1, "float")):
-        for nb in (32, 48, 64):
+        for nb in (24, 32, 48, 64):
             for p in (1, 2, 3):
                 cpl = C.c_int(0)
                 wg = L.ukhip_qr_launch_get(f, p, nb, C.byref(cpl))
