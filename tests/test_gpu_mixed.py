@@ -159,3 +159,41 @@ def test_mixed_graph_replays_match_eager(dev):
     finally:
         L.ukhip_use_graphs(1)
         b.free()
+
+
+def test_storage_on_demand(dev):
+    """A batch smoothed only by the mixed path for its estimates never holds
+    the FP64 elimination's storage: asking it for covariances is an error
+    (not stale memory). An FP64 smoothing afterwards allocates that storage
+    and gives the same result as on a fresh batch; mixed smoothings with other
+    precision switches (their FP64 levels' storage made per switch) and with
+    covariances then still agree with a fresh batch bit for bit."""
+    specs = P.batch_specs(count=6, seed=12, lmin=40, lmax=700)
+    probs = [P.batch_problem(s) for s in specs]
+    L = dev.lib()
+    b = dev.Batch(probs)
+    fresh = dev.Batch(probs)
+    try:
+        b.smooth_mixed_opt(1, 2, False)
+        with pytest.raises(dev.UkhipError):
+            b.download(0)
+        b.smooth("oddeven")
+        fresh.smooth("oddeven")
+        for i in range(len(probs)):
+            x, c = b.download(i)
+            xf, cf = fresh.download(i)
+            assert np.array_equal(x, xf) and np.array_equal(c, cf), i
+        for fl, cov in ((3, False), (1, True), (8, True)):
+            b.smooth_mixed_opt(2, fl, cov)
+            fresh.smooth_mixed_opt(2, fl, cov)
+            for i in range(len(probs)):
+                x, c = b.download(i, want_cov=cov)
+                xf, cf = fresh.download(i, want_cov=cov)
+                assert np.array_equal(x, xf), (fl, cov, i)
+                if cov:
+                    assert np.array_equal(c, cf), (fl, cov, i)
+        say("mixed-only batch: no covariance storage; FP64 and mixed results equal a fresh batch's")
+    finally:
+        L.ukhip_use_graphs(1)
+        b.free()
+        fresh.free()

@@ -1015,6 +1015,7 @@ int run_assoc(ukhip_batch* b) {
   void* ws;
   size_t ws_bytes;
   uk_batch_view(b, &raw, &xout, &Sout, &S, &nprob, &kk, &ws, &ws_bytes);
+  if (!xout || !Sout) return -1;
   const int n = raw->n;
   uk_chain_t wch;
   const int32_t* m_obs = nullptr;
@@ -1177,6 +1178,7 @@ extern "C" int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options) {
   void* ws;
   size_t wsb;
   uk_batch_view(b, &raw, &xout, &Sout, &S, &nprob, &k, &ws, &wsb);
+  if (!xout || !Sout) return -1; /* the output storage could not be allocated (error recorded) */
   uk_chain_t ch0;
   const int32_t* m0;
   uk_batch_chain0(b, &ch0, &m0);

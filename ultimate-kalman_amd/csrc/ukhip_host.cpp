@@ -195,6 +195,12 @@ struct ukhip_batch {
   /* device memory */
   void* d_raw = nullptr;
   void* d_work = nullptr;
+  /* the FP64 elimination's own storage (chain[1], fac, X, XT, Sout), allocated
+   * by ensure_elim on the first FP64 smoothing, associative smoothing or
+   * covariance request: a batch smoothed only by the mixed-precision path
+   * for its estimates never holds it */
+  void* d_elim = nullptr;
+  size_t elim_bytes = 0;
   void* d_tab = nullptr;
   size_t raw_bytes = 0, work_bytes = 0;
   uk_raw_t raw{};
@@ -224,10 +230,18 @@ struct ukhip_batch {
   uk_factors32_t fac32{};
   float* fix32 = nullptr;
   double *gA = nullptr, *gB = nullptr, *gbuf[2]{}, *ys = nullptr, *dx = nullptr;
-  /* FP64 chains of the levels the mixed path eliminates in FP64: top[1] is
-   * chain[1] (level-1 size), top[0] holds at most level 2's steps; chain[0]
-   * keeps the FP64 level-0 chain for the refinement residuals */
-  uk_chain_t top0{};
+  /* The FP64 storage of the levels the mixed path eliminates in FP64, per
+   * first FP64 level lsw (allocated on first use, kept until the batch is
+   * freed, since captured graphs refer to it): the two chains (levels of
+   * lsw's parity in ch[lsw & 1]) and the factor slots of the levels from lsw
+   * up (level l's at levels[l].fac_base - levels[lsw].fac_base). chain[0]
+   * keeps the FP64 level-0 chain for the refinement residuals. */
+  struct TopStore {
+    void* mem = nullptr;
+    uk_chain_t ch[2]{};
+    uk_factors_t fac{};
+  };
+  std::map<int, TopStore> tops;
   /* captured mixed-precision smoothings, by (refinements, FP32 levels, covariances) */
   std::map<int64_t, hipGraphExec_t> mixed_graph;
   /* memory source (NULL: hipMalloc) and the host copy of the level tables
@@ -433,10 +447,69 @@ static void batch_release(ukhip_batch* b) {
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
   dev_free(b, b->d_raw);
   dev_free(b, b->d_work);
+  dev_free(b, b->d_elim);
   dev_free(b, b->d_tab);
   if (b->d_assoc) (void)hipFree(b->d_assoc);
   dev_free(b, b->d_mixed);
+  for (auto& kv : b->tops) dev_free(b, kv.second.mem);
   delete b;
+}
+
+/* The FP64 elimination's storage beyond the level-0 chain: chain[1] (level-1
+ * size), the factor slots of every level, the cross covariances of two
+ * levels (level 0's are not needed: no level below reads them, so X[0]
+ * holds level 2's at most) and the covariance output. Idempotent; call it
+ * before a stream capture, never inside one. */
+static bool ensure_elim(ukhip_batch* b) {
+  if (b->d_elim) return true;
+  const size_t n = b->n, nn = n * n, S = (size_t)b->steps;
+  const int64_t L1 = b->levels.size() > 1 ? b->levels[1].steps : 1;
+  const int64_t L2 = b->levels.size() > 2 ? b->levels[2].steps : 1;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t rr = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return rr;
+  };
+  const size_t c1 = (size_t)std::max<int64_t>(L1, 1);
+  const size_t cB = take(c1 * nn * 8), cD = take(c1 * nn * 8), cc = take(c1 * n * 8), cC = take(c1 * nn * 8),
+               co = take(c1 * n * 8);
+  const size_t E = (size_t)b->total_evens;
+  const size_t fR = take(E * nn * 8), fB = take(E * nn * 8), fY = take(E * nn * 8), fc = take(E * n * 8);
+  size_t xo[2], xto[2];
+  for (int cb = 0; cb < 2; ++cb) {
+    const size_t cs = (size_t)std::max<int64_t>(cb == 0 ? L2 : L1, 1);
+    xo[cb] = take(cs * nn * 8);
+    xto[cb] = take(cs * nn * 8);
+  }
+  const size_t Soo = take(S * nn * 8);
+  void* mem = dev_alloc(b, o, "device memory for the FP64 elimination");
+  if (!mem) return false;
+  char* w = (char*)mem;
+  b->chain[1].B = (double*)(w + cB);
+  b->chain[1].D = (double*)(w + cD);
+  b->chain[1].c = (double*)(w + cc);
+  b->chain[1].C = (double*)(w + cC);
+  b->chain[1].o = (double*)(w + co);
+  b->fac.R = (double*)(w + fR);
+  b->fac.Bt = (double*)(w + fB);
+  b->fac.Y = (double*)(w + fY);
+  b->fac.cf = (double*)(w + fc);
+  for (int cb = 0; cb < 2; ++cb) {
+    b->X[cb] = (double*)(w + xo[cb]);
+    b->XT[cb] = (double*)(w + xto[cb]);
+  }
+  b->Sout = (double*)(w + Soo);
+  b->d_elim = mem;
+  b->elim_bytes = o;
+  return true;
+}
+/* ensure_elim for a batch or every part of a composite */
+static bool ensure_elim_all(ukhip_batch* b) {
+  if (b->parts.empty()) return ensure_elim(b);
+  for (auto* q : b->parts)
+    if (!ensure_elim(q)) return false;
+  return true;
 }
 
 /* Allocates device memory (from `arena`, or hipMalloc), uploads staging and
@@ -496,58 +569,38 @@ static ukhip_batch* batch_from_staging(Staging& st, const std::vector<int64_t>& 
   b->mr0 = b->compress ? n : st.mr;
   b->m0 = b->compress ? (const int32_t*)(r + st.offme) : b->raw.m;
 
-  /* workspace: chains (2), factors, cross covariances (2 x 2), outputs */
-  int64_t L1 = b->levels.size() > 1 ? b->levels[1].steps : 1;
+  /* workspace: the level-0 chain, the estimates and the per-problem fix-last
+   * scratch; the rest of the FP64 elimination's storage comes with
+   * ensure_elim */
   size_t o = 0;
   auto take = [&](size_t bytes) {
     size_t rr = o;
     o += (bytes + 255) & ~(size_t)255;
     return rr;
   };
-  size_t chain_off[2][5];
-  for (int cb = 0; cb < 2; ++cb) {
-    const size_t cs = cb == 0 ? S : (size_t)std::max<int64_t>(L1, 1);
-    chain_off[cb][0] = take(cs * nn * 8);
-    chain_off[cb][1] = take(cs * nn * 8);
-    chain_off[cb][2] = take(cs * n * 8);
-    chain_off[cb][3] = take(cs * nn * 8);
-    chain_off[cb][4] = take(cs * n * 8);
-  }
-  const size_t E = (size_t)b->total_evens;
-  const size_t fR = take(E * nn * 8), fB = take(E * nn * 8), fY = take(E * nn * 8), fc = take(E * n * 8);
-  /* cross covariances of level l go to X[l & 1]; level 0's are not needed
-   * (no level below reads them), so X[0] holds level 2's at most */
-  const int64_t L2 = b->levels.size() > 2 ? b->levels[2].steps : 1;
-  size_t xo[2], xto[2];
-  for (int cb = 0; cb < 2; ++cb) {
-    const size_t cs = (size_t)std::max<int64_t>(cb == 0 ? L2 : L1, 1);
-    xo[cb] = take(cs * nn * 8);
-    xto[cb] = take(cs * nn * 8);
-  }
-  const size_t xoo = take(S * n * 8), Soo = take(S * nn * 8);
+  size_t chain_off[5];
+  chain_off[0] = take(S * nn * 8);
+  chain_off[1] = take(S * nn * 8);
+  chain_off[2] = take(S * n * 8);
+  chain_off[3] = take(S * nn * 8);
+  chain_off[4] = take(S * n * 8);
+  const size_t xoo = take(S * n * 8);
   const size_t fxo = take((size_t)b->nprob * (nn + n) * 8);
   const size_t wbo = b->compress ? take(S * (size_t)st.mr * (n + 1) * 8) : 0;
   b->work_bytes = o;
   b->d_work = dev_alloc(b, o, "device memory for the workspace");
   if (!b->d_work) { batch_release(b); return nullptr; }
   char* w = (char*)b->d_work;
-  for (int cb = 0; cb < 2; ++cb) {
-    b->chain[cb].B = (double*)(w + chain_off[cb][0]);
-    b->chain[cb].D = (double*)(w + chain_off[cb][1]);
-    b->chain[cb].c = (double*)(w + chain_off[cb][2]);
-    b->chain[cb].C = (double*)(w + chain_off[cb][3]);
-    b->chain[cb].o = (double*)(w + chain_off[cb][4]);
-    b->X[cb] = (double*)(w + xo[cb]);
-    b->XT[cb] = (double*)(w + xto[cb]);
-  }
-  b->fac.R = (double*)(w + fR);
-  b->fac.Bt = (double*)(w + fB);
-  b->fac.Y = (double*)(w + fY);
-  b->fac.cf = (double*)(w + fc);
+  b->chain[0].B = (double*)(w + chain_off[0]);
+  b->chain[0].D = (double*)(w + chain_off[1]);
+  b->chain[0].c = (double*)(w + chain_off[2]);
+  b->chain[0].C = (double*)(w + chain_off[3]);
+  b->chain[0].o = (double*)(w + chain_off[4]);
   b->xout = (double*)(w + xoo);
-  b->Sout = (double*)(w + Soo);
   b->fix = (double*)(w + fxo);
   b->wobs = b->compress ? (double*)(w + wbo) : nullptr;
+  /* server batches (arena-backed) take all of their storage up front */
+  if (arena && !ensure_elim(b)) { batch_release(b); return nullptr; }
 
   /* level tables: one int64 array; prob indices stored widened */
   std::vector<int64_t> tab;
@@ -1094,6 +1147,7 @@ extern "C" int uk_run_oddeven_view(uk_oe_view_t* v, int cov, hipStream_t st) {
 }
 
 static bool run_oddeven(ukhip_batch* b, int cov, SlotTimer* tm, hipStream_t st) {
+  if (!ensure_elim(b)) return false;
   const int nl = (int)b->levels.size();
   std::vector<uk_level_t> lv(nl);
   std::vector<int64_t> fb(nl);
@@ -1231,6 +1285,7 @@ static int cov_mode(uint32_t options) {
 
 static bool graph_oddeven(ukhip_batch* b, int cov) {
   if (!b->graph[cov]) {
+    if (!ensure_elim_all(b)) return false; /* no allocation inside the capture */
     if (!hip_ok(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
       return false;
     const bool ok = enqueue_oddeven(b, cov);
@@ -1260,6 +1315,7 @@ extern "C" int ukhip_batch_smooth_oddeven(ukhip_batch_t* b, uint32_t options) {
 extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options) {
   if (!b) { uk_set_error("NULL batch"); return -1; }
   if (!ensure_stream()) return -1;
+  if (!ensure_elim_all(b)) return -1; /* the covariance output */
   if (b->parts.empty()) return uk_assoc_smooth(b, options);
   for (auto* q : b->parts)
     if (uk_assoc_smooth(q, options) != 0) return -1;
@@ -1350,9 +1406,6 @@ static bool mixed_workspace(ukhip_batch* b) {
   const size_t ofx = take((size_t)b->nprob * (nn + n) * 4);
   const size_t ogA = take(S * n * 8), ogB = take(S * n * 8), og0 = take(S * n * 8),
                og1 = take(std::max<size_t>(L1, 1) * n * 8), oys = take(E * n * 8), odx = take(S * n * 8);
-  const size_t L2 = b->levels.size() > 2 ? (size_t)b->levels[2].steps : 1;
-  const size_t tB = take(L2 * nn * 8), tD = take(L2 * nn * 8), tc = take(L2 * n * 8), tC = take(L2 * nn * 8),
-               to = take(L2 * n * 8);
   b->d_mixed = dev_alloc(b, o, "device memory for the mixed-precision workspace");
   if (!b->d_mixed) return false;
   char* w = (char*)b->d_mixed;
@@ -1374,12 +1427,71 @@ static bool mixed_workspace(ukhip_batch* b) {
   b->gbuf[1] = (double*)(w + og1);
   b->ys = (double*)(w + oys);
   b->dx = (double*)(w + odx);
-  b->top0.B = (double*)(w + tB);
-  b->top0.D = (double*)(w + tD);
-  b->top0.c = (double*)(w + tc);
-  b->top0.C = (double*)(w + tC);
-  b->top0.o = (double*)(w + to);
   return true;
+}
+
+/* The FP64 storage of levels lsw .. top for the mixed path (see TopStore);
+ * idempotent per lsw, never inside a stream capture. */
+static const ukhip_batch::TopStore* ensure_top(ukhip_batch* b, int lsw) {
+  auto it = b->tops.find(lsw);
+  if (it != b->tops.end()) return &it->second;
+  const int nl = (int)b->levels.size();
+  const size_t n = b->n, nn = n * n;
+  const size_t E = (size_t)(b->total_evens - b->levels[lsw].fac_base);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t ch[2][5];
+  for (int par = 0; par < 2; ++par) { /* ch[lsw & 1] holds level lsw, the other level lsw + 1 */
+    const int l = (par == (lsw & 1)) ? lsw : lsw + 1;
+    const size_t cs = l < nl ? (size_t)std::max<int64_t>(b->levels[l].steps, 1) : 1;
+    ch[par][0] = take(cs * nn * 8);
+    ch[par][1] = take(cs * nn * 8);
+    ch[par][2] = take(cs * n * 8);
+    ch[par][3] = take(cs * nn * 8);
+    ch[par][4] = take(cs * n * 8);
+  }
+  const size_t fR = take(E * nn * 8), fB = take(E * nn * 8), fY = take(E * nn * 8), fc = take(E * n * 8);
+  void* mem = dev_alloc(b, o, "device memory for the mixed path's FP64 levels");
+  if (!mem) return nullptr;
+  char* w = (char*)mem;
+  ukhip_batch::TopStore ts;
+  ts.mem = mem;
+  for (int par = 0; par < 2; ++par) {
+    ts.ch[par].B = (double*)(w + ch[par][0]);
+    ts.ch[par].D = (double*)(w + ch[par][1]);
+    ts.ch[par].c = (double*)(w + ch[par][2]);
+    ts.ch[par].C = (double*)(w + ch[par][3]);
+    ts.ch[par].o = (double*)(w + ch[par][4]);
+  }
+  ts.fac.R = (double*)(w + fR);
+  ts.fac.Bt = (double*)(w + fB);
+  ts.fac.Y = (double*)(w + fY);
+  ts.fac.cf = (double*)(w + fc);
+  return &(b->tops[lsw] = ts);
+}
+
+/* first FP64 level of a mixed smoothing with fp32_levels FP32 levels (level 0
+ * stays FP32); nl when every level is FP32 */
+static int mixed_lsw(const ukhip_batch* b, int fp32_levels) {
+  const int nl = (int)b->levels.size();
+  return std::max(1, std::min(fp32_levels, nl));
+}
+
+/* Everything a mixed smoothing allocates, before any capture. */
+static bool mixed_prepare(ukhip_batch* b, int fp32_levels, bool cov) {
+  if (!b->parts.empty()) {
+    for (auto* q : b->parts)
+      if (!mixed_prepare(q, fp32_levels, cov)) return false;
+    return true;
+  }
+  if (!mixed_workspace(b)) return false;
+  const int lsw = mixed_lsw(b, fp32_levels);
+  if (lsw < (int)b->levels.size() && !ensure_top(b, lsw)) return false;
+  return !cov || ensure_elim(b);
 }
 
 static uk_factors32_t fac32_at(const ukhip_batch* b, int l) {
@@ -1393,9 +1505,11 @@ static uk_factors32_t fac32_at(const ukhip_batch* b, int l) {
   return f;
 }
 
-static uk_factors_t fac64_at(const ukhip_batch* b, int l) {
-  uk_factors_t f = b->fac;
-  const int64_t fb = b->levels[l].fac_base;
+/* level l's FP64 factor slots: in the full factor storage (base 0), or in a
+ * TopStore whose first level is `from` */
+static uk_factors_t fac64_at(const ukhip_batch* b, int l, const uk_factors_t& all, int from = 0) {
+  uk_factors_t f = all;
+  const int64_t fb = b->levels[l].fac_base - b->levels[from].fac_base;
   const size_t nn = (size_t)b->n * b->n;
   f.R += fb * nn;
   f.Bt += fb * nn;
@@ -1412,7 +1526,7 @@ static uk_factors_t fac64_at(const ukhip_batch* b, int l) {
  * from the same factors by the FP64 selected-inversion back substitution
  * (the FP32 factors widened into the FP64 factor slots), into Sout. */
 static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov, SlotTimer* tm, hipStream_t st) {
-  if (!mixed_workspace(b)) return false;
+  if (!mixed_prepare(b, fp32_levels, cov)) return false;
   auto tb = [&]() {
     if (tm) tm->begin();
   };
@@ -1420,7 +1534,9 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
     if (tm) tm->end(slot, launches);
   };
   const int n = b->n, nl = (int)b->levels.size();
-  const int lsw = std::max(1, std::min(fp32_levels, nl)); /* first FP64 level (level 0 stays FP32) */
+  const int lsw = mixed_lsw(b, fp32_levels); /* first FP64 level (level 0 stays FP32) */
+  const ukhip_batch::TopStore* ts = lsw < nl ? &b->tops.at(lsw) : nullptr;
+  auto fac64 = [&](int l) { return fac64_at(b, l, ts->fac, lsw); };
   const size_t nn = (size_t)n * n, S = (size_t)b->steps;
   /* the level-0 chain in FP32 (the FP64 chain stays for the residuals):
    * written beside the FP64 one when the whitening is diagonal, else converted */
@@ -1444,7 +1560,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       return false;
     te(UKHIP_OE_CONVERT, 5);
   }
-  uk_chain_t top[2] = {b->top0, b->chain[1]};
+  uk_chain_t top[2] = {ts ? ts->ch[0] : uk_chain_t{}, ts ? ts->ch[1] : uk_chain_t{}};
   for (int l = 0; l < nl; ++l) {
     uk_level_t lv = b->levels[l].dev;
     if (l < lsw) {
@@ -1478,7 +1594,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       in.c_tri = 1;
       uk_chain_t out = top[(l + 1) & 1];
       out.mr = n;
-      const uk_factors_t f = fac64_at(b, l);
+      const uk_factors_t f = fac64(l);
       if (!hip_ok(uk_launch_eliminate(&lv, &in, &out, &f, b->fix, n, st, tm ? tm->phases() : nullptr),
                   "FP64 elimination launch"))
         return false;
@@ -1493,7 +1609,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
       const uk_factors32_t f = fac32_at(b, l);
       if (!hip_ok(uk_launch_solve_est(&lv, &f, f.cf, nullptr, b->xout, n, st), "solve launch")) return false;
     } else {
-      const uk_factors_t f = fac64_at(b, l);
+      const uk_factors_t f = fac64(l);
       if (!hip_ok(uk_launch_solve_est64(&lv, &f, f.cf, b->xout, n, st), "solve launch")) return false;
     }
     te(UKHIP_OE_SOLVE, 1);
@@ -1514,7 +1630,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
         const uk_factors32_t f = fac32_at(b, l);
         e = uk_launch_fwd(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
       } else {
-        const uk_factors_t f = fac64_at(b, l);
+        const uk_factors_t f = fac64(l);
         e = uk_launch_fwd64(&lv, &f, b->gbuf[l & 1], y, b->gbuf[(l + 1) & 1], n, st);
       }
       if (!hip_ok(e, "forward sweep launch")) return false;
@@ -1529,7 +1645,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
         const uk_factors32_t f = fac32_at(b, l);
         e = uk_launch_solve_est(&lv, &f, nullptr, y, b->dx, n, st);
       } else {
-        const uk_factors_t f = fac64_at(b, l);
+        const uk_factors_t f = fac64(l);
         e = uk_launch_solve_est64(&lv, &f, y, b->dx, n, st);
       }
       if (!hip_ok(e, "backward sweep launch")) return false;
@@ -1542,7 +1658,8 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
   if (!cov) return true;
   /* covariances: the FP32 levels' factors widened into their FP64 slots (the
    * slots of levels 0 .. lsw-1 are contiguous), then the FP64 back
-   * substitution; its estimates go to the scratch dx, xout keeps the refined ones */
+   * substitution over those and the FP64 levels' own factors; its estimates
+   * go to the scratch dx, xout keeps the refined ones */
   const size_t E32 = (size_t)(lsw < nl ? b->levels[lsw].fac_base : b->total_evens);
   tb();
   if (!hip_ok(uk_launch_to64(b->fac.R, b->fac32.R, E32 * nn, st), "to64") ||
@@ -1552,7 +1669,7 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
     return false;
   te(UKHIP_OE_CONVERT, 4);
   for (int l = nl - 1; l >= 0; --l) {
-    const uk_factors_t f = fac64_at(b, l);
+    const uk_factors_t f = l < lsw ? fac64_at(b, l, b->fac) : fac64(l);
     uk_level_t lv = b->levels[l].dev;
     double* Xc = l == 0 ? nullptr : b->X[l & 1];
     double* XTc = l == 0 ? nullptr : b->XT[l & 1];
@@ -1599,12 +1716,7 @@ static bool enqueue_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool
 /* Captures enqueue_mixed once per (refinements, FP32 levels, covariances),
  * replays it afterwards. The workspaces are allocated before the capture. */
 static bool graph_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov) {
-  if (b->parts.empty()) {
-    if (!mixed_workspace(b)) return false;
-  } else {
-    for (auto* q : b->parts)
-      if (!mixed_workspace(q)) return false;
-  }
+  if (!mixed_prepare(b, fp32_levels, cov)) return false; /* no allocation inside the capture */
   const int64_t key = ((int64_t)refinements << 20) | ((int64_t)(fp32_levels & 0xFFFF) << 1) | (cov ? 1 : 0);
   hipGraphExec_t& ge = b->mixed_graph[key];
   if (!ge) {
@@ -1650,6 +1762,10 @@ extern "C" int ukhip_batch_download(ukhip_batch_t* b, int32_t p, double* states,
   const int64_t o = b->orig_start[p], k = b->k[p];
   if (states && !hip_ok(hipMemcpyAsync(states, b->xout + o * n, k * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
     return -1;
+  if (covs && !b->Sout) {
+    uk_set_error("no covariances were computed for this batch (only mixed-precision estimates)");
+    return -1;
+  }
   if (covs && !hip_ok(hipMemcpyAsync(covs, b->Sout + o * nn, k * nn * 8, hipMemcpyDeviceToHost, g_stream), "download"))
     return -1;
   return hip_ok(hipStreamSynchronize(g_stream), "download") ? 0 : -1;
@@ -1660,6 +1776,10 @@ extern "C" int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format
   if (!b->parts.empty()) return ukhip_batch_download_cov(b->parts[b->where[p].first], b->where[p].second, format, out);
   if (format == 'C') return ukhip_batch_download(b, p, nullptr, out);
   if (format != 'D' && format != 'W') { uk_set_error("covariance format '%c': use 'C', 'D' or 'W'", format); return -1; }
+  if (!b->Sout) {
+    uk_set_error("no covariances were computed for this batch (only mixed-precision estimates)");
+    return -1;
+  }
   const size_t n = b->n, nn = n * n;
   const int64_t o = b->orig_start[p], k = b->k[p];
   const size_t bytes = (size_t)k * (format == 'D' ? n : nn) * 8;
@@ -1674,11 +1794,13 @@ extern "C" int ukhip_batch_download_cov(ukhip_batch_t* b, int32_t p, char format
 }
 
 /* Device pointers for sibling translation units (associative path). */
+/* The covariance output is allocated here when the batch has none yet (*Sout
+ * is NULL, with the error recorded, when that fails). */
 extern "C" void uk_batch_view(ukhip_batch* b, const uk_raw_t** raw, double** xout, double** Sout, int64_t* steps,
                               int* nprob, const int64_t** k, void** assoc_ws, size_t* assoc_bytes) {
   *raw = &b->raw;
   *xout = b->xout;
-  *Sout = b->Sout;
+  *Sout = ensure_elim(b) ? b->Sout : nullptr;
   *steps = b->steps;
   *nprob = b->nprob;
   *k = b->k.data();
