@@ -788,7 +788,23 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the long workload's timed region, copy MIB MiB device-to-device once (a plain copy "
                          "kernel of known size for calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
+    ap.add_argument("--mixed-chunk-only", type=int, default=0, metavar="N",
+                    help="profiling aid: the first N problems of the batching workload as one device batch, two eager "
+                         "mixed smoothings (graphs off; --copy-calibration runs between them), no bench line")
     args = ap.parse_args()
+    if args.mixed_chunk_only:
+        probs = [P.batch_problem(sp, lists=False) for sp in P.batch_specs(count=args.mixed_chunk_only)]
+        b = device.Batch(probs)
+        L = device.lib()
+        L.ukhip_use_graphs(0)
+        b.smooth_mixed_opt(args.refine, 0, False)
+        if args.copy_calibration:
+            copy_calibration(args.copy_calibration)
+        b.smooth_mixed_opt(args.refine, 0, False)
+        L.ukhip_synchronize()
+        b.free()
+        say(f"mixed chunk of {len(probs)} problems: two eager smoothings done")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); --gpus {args.gpus} is not supported",
               file=sys.stderr)

@@ -1131,10 +1131,19 @@ extern "C" int uk_bucket_qr(int n) {
   return uk_bucket(n);
 }
 
+/* Steps per workgroup of k_weigh_diag: about 2048 elements of work per
+ * 256-thread workgroup (the n x n pass is skipped when B is formed on load and
+ * D is stored as its diagonal). */
+static int weigh_diag_spw(const uk_raw_t* raw, const uk_chain_t* out) {
+  const int n = raw->n, mr = raw->mr;
+  const int per = (out->b_synth && out->d_diag ? 0 : n * n) + mr * n + n + mr;
+  const int spw = 2048 / (per > 0 ? per : 1);
+  return spw < 1 ? 1 : (spw > 64 ? 64 : spw);
+}
+
 extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st) {
   if (!raw->K_full && !raw->C_full) { /* every covariance given as 'w' */
-    const int nn = raw->n * raw->n;
-    const int spw = nn >= 1024 ? 1 : 1024 / nn;
+    const int spw = weigh_diag_spw(raw, out);
     const int64_t blocks = (raw->steps + spw - 1) / spw;
     k_weigh_diag<false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, uk_chain32_t{}, spw);
     return hipGetLastError();
@@ -1188,8 +1197,7 @@ extern "C" hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t*
 extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32,
                                           hipStream_t st) {
   if (raw->K_full || raw->C_full) return hipErrorNotSupported;
-  const int nn = raw->n * raw->n;
-  const int spw = nn >= 1024 ? 1 : 1024 / nn;
+  const int spw = weigh_diag_spw(raw, out);
   const int64_t blocks = (raw->steps + spw - 1) / spw;
   k_weigh_diag<true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, *out32, spw);
   return hipGetLastError();

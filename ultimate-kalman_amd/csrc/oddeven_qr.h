@@ -177,10 +177,14 @@ struct Col {
 };
 
 /* LDS reflector buffers: [buffer][sub][2H + 4] (the +4 skews the sub-lane
- * arrays onto different banks), then [buffer][2] scalars. */
+ * arrays onto different banks), then [buffer][2] scalars. A stride that
+ * lands on bank 0 or 16 of the 32 four-byte banks (2H + 4 = 16 in bucket 24)
+ * would put two sub-lanes' reads on the same banks: it is padded by 4 more. */
 template <int NB, int SL, class T = double>
 struct Refl {
-  static constexpr int SUBSTRIDE = 2 * (NB / SL) + 4;
+  static constexpr int SUBSTRIDE0 = 2 * (NB / SL) + 4;
+  static constexpr int SUBSTRIDE =
+      SUBSTRIDE0 + ((SUBSTRIDE0 * (int)sizeof(T) / 4) % 16 == 0 ? 4 : 0);
   T v[2][SL * SUBSTRIDE];
   T sc[2][2];
 };

@@ -9,9 +9,14 @@
         (DIR2), each calibrated by what the counter reads for the plain copy
         kernel of B bytes that the profiled program ran (bench.py
         --copy-calibration); --json merges the result into PATH under NAME.
+    python -m ukhip.rocprof counters DIR [DIR ...] [--cus 256] [--xcds 8]
+        per-kernel utilisation from SQ/GRBM counter runs (each DIR one
+        rocprofv3 --pmc run; the kernels' dispatches summed over the runs'
+        counters).
 
 Counter values are in KiB per dispatch. The calibration factor of a counter is
-B / (1024 * the counter's value on the copy kernel).
+B / (1024 * the counter's value on the copy kernel). The HIP runtime's own fill
+kernels (the calibration's buffer initialisation) are left out of the tables.
 """
 import argparse
 import collections
@@ -79,12 +84,77 @@ def traffic(fetch_dir, write_dir, copy_bytes, per=1):
     cw = copy_bytes / (1024.0 * ws[COPY_KERNEL][1])
     out = {}
     for k in fs:
-        if k == COPY_KERNEL or k.startswith("at::"):
+        if k == COPY_KERNEL or k.startswith("at::") or k.startswith("__amd_rocclr_"):
             continue
         fb = fs[k][1] * 1024.0 * cf / per
         wb = ws.get(k, [0, 0.0])[1] * 1024.0 * cw / per
         out[k] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_step": fb + wb, "launches_per_step": fs[k][0] / per}
     return out, cf, cw
+
+
+def counter_table(dirs, cus=256, xcds=8):
+    """Per kernel (short name): dispatches, device ms (dispatch timestamps) and
+    utilisation figures from the SQ/GRBM counters found in `dirs`:
+      valu_busy   100 * SQ_ACTIVE_INST_VALU / CUs / GRBM cycles (rocprof's
+                  VALUBusy: VALU issue against one wave64 instruction per
+                  4 cycles per SIMD);
+      waves_cu    resident waves per CU, 4 * SQ_WAVE_CYCLES / (GRBM cycles * CUs);
+      lds_util    100 * SQ_LDS_IDX_ACTIVE / (GRBM cycles * CUs);
+      bank_conf   SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - conflicts);
+      mfma_busy   100 * SQ_VALU_MFMA_BUSY_CYCLES / (GRBM cycles * CUs);
+      valu_per_wave, lds_per_wave, fma_share (FMA F32 + F64 of all VALU).
+    GRBM_GUI_ACTIVE is reported summed over the XCDs, so GRBM cycles are its
+    value / xcds."""
+    acc = collections.defaultdict(lambda: collections.defaultdict(float))
+    for d in dirs:
+        seen = set()
+        with open(os.path.join(d, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                k = short_name(r["Kernel_Name"])
+                e = acc[k]
+                e[r["Counter_Name"] + "@" + d] += float(r["Counter_Value"])
+                key = (d, r["Dispatch_Id"])
+                if key not in seen:
+                    seen.add(key)
+                    e["dispatches@" + d] += 1
+                    e["ns@" + d] += float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+    out = {}
+    for k, e in acc.items():
+        def tot(c):
+            v = [x for kk, x in e.items() if kk.split("@")[0] == c]
+            return sum(v) if v else None
+
+        def per_dir(c, d):
+            return e.get(c + "@" + d)
+
+        row = {"dispatches": int(max(e.get("dispatches@" + d, 0) for d in dirs)),
+               "ms": max(e.get("ns@" + d, 0.0) for d in dirs) / 1e6}
+        for d in dirs:  # ratios within one run (its own GRBM cycles)
+            g = per_dir("GRBM_GUI_ACTIVE", d)
+            if not g:
+                continue
+            cyc = g / xcds
+            for name, c, scale in (("valu_busy", "SQ_ACTIVE_INST_VALU", 100.0 / cus),
+                                   ("waves_cu", "SQ_WAVE_CYCLES", 4.0 / cus),
+                                   ("lds_util", "SQ_LDS_IDX_ACTIVE", 100.0 / cus),
+                                   ("mfma_busy", "SQ_VALU_MFMA_BUSY_CYCLES", 100.0 / cus)):
+                v = per_dir(c, d)
+                if v is not None:
+                    row[name] = v * scale / cyc
+            w, vi, li = per_dir("SQ_WAVES", d), per_dir("SQ_INSTS_VALU", d), per_dir("SQ_INSTS_LDS", d)
+            if w:
+                if vi is not None:
+                    row["valu_per_wave"] = vi / w
+                if li is not None:
+                    row["lds_per_wave"] = li / w
+            bc, ia = per_dir("SQ_LDS_BANK_CONFLICT", d), per_dir("SQ_LDS_IDX_ACTIVE", d)
+            if bc is not None and ia:
+                row["bank_conf"] = bc / max(ia - bc, 1.0)
+        f32, f64, vall = tot("SQ_INSTS_VALU_FMA_F32"), tot("SQ_INSTS_VALU_FMA_F64"), tot("SQ_INSTS_VALU")
+        if f32 is not None and f64 is not None and vall:
+            row["fma_share"] = (f32 + f64) / vall
+        out[k] = row
+    return out
 
 
 def main(argv=None):
@@ -101,7 +171,23 @@ def main(argv=None):
     t.add_argument("--workload")
     t.add_argument("--json")
     t.add_argument("--algorithmic", action="append", default=[], help="KERNEL=BYTES per step")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--xcds", type=int, default=8)
+    c.add_argument("--top", type=int, default=24)
     args = ap.parse_args(argv)
+    if args.cmd == "counters":
+        tab = counter_table(args.dirs, args.cus, args.xcds)
+        cols = ("valu_busy", "waves_cu", "lds_util", "bank_conf", "mfma_busy", "valu_per_wave", "lds_per_wave",
+                "fma_share")
+        print(f"{'kernel':44s} {'disp':>5s} {'ms':>8s} " + " ".join(f"{c:>13s}" for c in cols))
+        rows = [(k, v) for k, v in tab.items() if not k.startswith("__amd_rocclr_")]
+        for k, v in sorted(rows, key=lambda kv: -kv[1]["ms"])[:args.top]:
+            cells = " ".join(f"{'' if v.get(c) is None else format(v[c], '.3f' if c in ('bank_conf', 'fma_share') else '.1f'):>13s}"
+                             for c in cols)
+            print(f"{k[:44]:44s} {v['dispatches']:5d} {v['ms']:8.2f} {cells}")
+        return 0
     if args.cmd == "stats":
         print(f"{'kernel':40s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'ms/step':>9s} {'%':>6s}")
         rows = kernel_stats(args.dir)
