@@ -591,11 +591,30 @@ def run_long(args):
         device.lib().ukhip_synchronize()
         ams = (time.perf_counter() - ta) * 1e3 / args.assoc_steps
         xa, Sa = b.download(0)
+        # per kernel class (HIP events after each class), and the roofline of the dominant one
+        ams_k, anl = b.smooth_associative_timed()
+        afl = flops.associative_flops(pr.n[0], m_list_of(pr))
+        cls = [c for c in device.AS_KERNELS if c != "total"]
+        adom = max(cls, key=lambda c: ams_k[c])
+        a_ach = afl[adom] / (ams_k[adom] * 1e-3) / 1e12
+        names = {"fscan": "ka_scan_reduce/round/down<48, true>", "sscan": "ka_scan_reduce/round/down<48, false>",
+                 "felem": "ka_felem<48>", "selem": "ka_selem<48>", "model": "ka_model<48>", "prior": "ka_prior<48>",
+                 "filtered": "ka_filtered", "results": "ka_results", "weigh": "k_weigh<48>"}
         out["associative"] = {
             "algorithm": "associative scan (Saerkkae-Garcia-Fernandez), FP64, with covariances",
             "ms_per_step": ams, "value": pr.k / (ams * 1e-3), "unit": "timesteps/s", "steps": args.assoc_steps,
             "max_rel_err_states_vs_oddeven": step_rel_err(xa, x64),
-            "max_rel_err_covdiag_vs_oddeven": step_rel_err(np.einsum("kii->ki", Sa), d64)}
+            "max_rel_err_covdiag_vs_oddeven": step_rel_err(np.einsum("kii->ki", Sa), d64),
+            "kernels_ms_per_step": {c: ams_k[c] for c in device.AS_KERNELS},
+            "kernels_tflops": {c: afl[c] / (ams_k[c] * 1e-3) / 1e12 for c in cls if afl[c] > 0 and ams_k[c] > 0},
+            "roofline": {"bound": "valu", "kernel": names[adom], "achieved": a_ach, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": a_ach / FP64_PEAK_TFLOPS, "traffic": None,
+                         "algorithmic_flops_per_step": afl[adom], "launches_per_step": anl[adom],
+                         "kernel_ms_per_step": ams_k[adom], "avg_launch_ms": ams_k[adom] / max(anl[adom], 1),
+                         "measured": "HIP events after each kernel class of one associative smoothing",
+                         "flops": "ukhip.flops.associative_flops: the reference's matrix operations, two "
+                                  "combines per element in each scan"},
+            "algorithmic_tflops_whole_path": afl["total"] / (ams * 1e-3) / 1e12}
         del Sa
         say(f"associative {ams:.2f} ms/step, vs odd-even: states {out['associative']['max_rel_err_states_vs_oddeven']:.2e}"
             f", covariance diagonals {out['associative']['max_rel_err_covdiag_vs_oddeven']:.2e}")
@@ -620,7 +639,18 @@ def run_long(args):
                 "max_rel_err_covdiag": step_rel_err(np.einsum("kii->ki", gS), rd)}
             out["cpu_port_allcores"] = cpu_port_allcores(name, args.cpu_sample_k)
             if not args.no_associative:
-                out["associative"]["cpu_baseline"] = cpu_baseline(name, args.cpu_sample_k_assoc, "associative")
+                abase, aprs, arx, ard = cpu_baseline(name, args.cpu_sample_k_assoc, "associative", want_outputs=True)
+                out["associative"]["cpu_baseline"] = abase
+                ba = device.Batch([aprs])
+                ba.smooth("associative")
+                ax, aS = ba.download(0)
+                ba.free()
+                out["associative"]["vs_cpu_output"] = {
+                    "sample": f"the same workload at k={args.cpu_sample_k_assoc}, GPU associative against the "
+                              f"{'reference' if abase['kind'] == 'reference' else 'oracle'}'s associative output "
+                              "on the same inputs",
+                    "max_rel_err_states": step_rel_err(ax, arx),
+                    "max_rel_err_covdiag": step_rel_err(np.einsum("kii->ki", aS), ard)}
         except Exception as e:  # the GPU figures stand without it
             out["cpu_baseline"] = {"error": str(e)}
     return out

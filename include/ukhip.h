@@ -152,6 +152,19 @@ int ukhip_batch_smooth_oddeven_timed(ukhip_batch_t* b, uint32_t options, double 
 int ukhip_batch_timed_parts(ukhip_batch_t* b, uint32_t options, const ukhip_mixed_options_t* mixed, double* ms,
                             int32_t* launches, int32_t* part_n, int32_t max_parts);
 
+/* Timed associative run (c/kalman_associative_smoother.c:1199-1240): device
+ * milliseconds per kernel class, from HIP events recorded on the library
+ * stream after each class, parts one after another; added into ms and
+ * launches (may be NULL), which the caller zeroes. Slots: WEIGH (the dense
+ * whitening), MODEL (ka_model), PRIOR (ka_prior), FELEM (ka_felem), FSCAN
+ * (the filtering scan: ka_scan_reduce, ka_scan_round, ka_scan_down), FILTERED
+ * (ka_filtered), SELEM (ka_selem), SSCAN (the smoothing scan), RESULTS
+ * (ka_results), TOTAL (first launch to last). */
+enum { UKHIP_AS_WEIGH = 0, UKHIP_AS_MODEL, UKHIP_AS_PRIOR, UKHIP_AS_FELEM, UKHIP_AS_FSCAN, UKHIP_AS_FILTERED,
+       UKHIP_AS_SELEM, UKHIP_AS_SSCAN, UKHIP_AS_RESULTS, UKHIP_AS_TOTAL, UKHIP_AS_SLOTS };
+int ukhip_batch_smooth_associative_timed(ukhip_batch_t* b, uint32_t options, double ms[UKHIP_AS_SLOTS],
+                                         int32_t launches[UKHIP_AS_SLOTS]);
+
 /* ---- allocation-free odd-even smoothing ------------------------------------
  * No counterpart in the reference (c/kalman.h): for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,

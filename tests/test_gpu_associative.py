@@ -217,3 +217,25 @@ def test_kalman_t_associative(dev, oracle):
         L.matrix_free(cvm)
         assert rel_err(x, ox[i]) < TOL and rel_err(cv, oc[i]) < TOL, i
     L.kalman_free(kal)
+
+
+def test_timed_classes(dev):
+    """ukhip_batch_smooth_associative_timed: the same results as the untimed
+    call, bit for bit, on a batch of two state dimensions; every kernel class
+    launched, the class times adding up to the span."""
+    probs = [P.iid_problem(k, n, n // 2 + 1, seed=700 + k, F=0.97 * P.random_orthogonal(P.rng_for(701 + n), n),
+                           G_scale=1 / np.sqrt(n), name=f"n{n}_k{k}") for n, k in ((6, 300), (20, 257))]
+    b = dev.Batch(probs)
+    try:
+        b.smooth("associative")
+        ref = [b.download(i) for i in range(len(probs))]
+        ms, nl = b.smooth_associative_timed()
+        for i in range(len(probs)):
+            x, S = b.download(i)
+            assert np.array_equal(x, ref[i][0]) and np.array_equal(S, ref[i][1]), i
+        classes = [c for c in dev.AS_KERNELS if c != "total"]
+        assert all(nl[c] >= 2 for c in classes), nl  # two parts, every class launched in each
+        assert abs(sum(ms[c] for c in classes) - ms["total"]) < 1e-3 * ms["total"] + 1e-3, ms
+        say("associative classes: " + ", ".join(f"{c} {ms[c]:.3f} ms" for c in dev.AS_KERNELS))
+    finally:
+        b.free()

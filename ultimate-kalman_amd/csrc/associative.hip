@@ -38,6 +38,8 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/kalman.h"
+#include "../../include/ukhip.h"
 #include "ukhip_internal.h"
 
 extern "C" void uk_set_error(const char* fmt, ...);
@@ -982,6 +984,16 @@ bool ok(hipError_t e, const char* what) {
   return false;
 }
 
+/* Event timing of one associative smoothing (ukhip_batch_smooth_associative_timed):
+ * an event after each kernel class, in launch order; slot i's time is
+ * ev[i] -> ev[i + 1]. Set for the duration of one timed call on this thread. */
+struct AsTimer {
+  hipEvent_t ev[UKHIP_AS_SLOTS + 1]{};
+  int32_t launches[UKHIP_AS_SLOTS]{};
+  double ms[UKHIP_AS_SLOTS]{};
+};
+thread_local AsTimer* t_timer = nullptr;
+
 /* chunks of at most `chunk` elements over `len` elements of one problem;
  * dir = +1 starts at `base`, dir = -1 at base + len - 1. */
 int64_t make_chunks(std::vector<Chunk>& out, int64_t base, int64_t len, int dir, int64_t chunk, int64_t& tot0) {
@@ -1131,36 +1143,69 @@ int run_assoc(ukhip_batch* b) {
   if (!ok(hipMemcpyAsync(W + o_tab, tab.data(), to, hipMemcpyHostToDevice, st), "upload scan tables")) return -1;
 
   const dim3 blk(AT);
+  AsTimer* tm = t_timer;
+  auto mark = [&](int slot, int launches) { /* slot's launches are done: its closing event */
+    if (!tm) return;
+    tm->launches[slot] += launches;
+    (void)hipEventRecord(tm->ev[slot + 1], st);
+  };
+  if (tm) (void)hipEventRecord(tm->ev[0], st);
   if (!ok(uk_batch_weigh_dense0(b, st), "k_weigh")) return -1;
+  mark(UKHIP_AS_WEIGH, 1);
   ka_model<NB><<<dim3((unsigned)S), blk, 0, st>>>(wch, raw->first, md, n);
+  mark(UKHIP_AS_MODEL, 1);
   ka_prior<NB><<<dim3((unsigned)nprob), blk, 0, st>>>(d_probs, wch, m_obs, md, n, mr);
+  mark(UKHIP_AS_PRIOR, 1);
   if (ne > 0) {
     const unsigned nc = (unsigned)fch.size();
     ka_felem<NB><<<dim3((unsigned)ne), blk, 0, st>>>(d_probs, d_elem_prob, wch, m_obs, md, fe, n, mr);
+    mark(UKHIP_AS_FELEM, 1);
     ka_scan_reduce<NB, true><<<dim3(nc), blk, 0, st>>>(d_fch, fe, ft[0], n, gw);
-    int cur = 0;
+    int cur = 0, rounds = 0;
     for (int64_t d = 1; d < maxr_f; d <<= 1) {
       ka_scan_round<NB, true><<<dim3(nc), blk, 0, st>>>(d_fch, ft[cur], ft[cur ^ 1], (int)d, n, gw);
       cur ^= 1;
+      ++rounds;
     }
     ka_scan_down<NB, true><<<dim3(nc), blk, 0, st>>>(d_fch, fe, ft[cur], n, gw);
+    mark(UKHIP_AS_FSCAN, 2 + rounds);
     ka_filtered<<<dim3((unsigned)ne), blk, 0, st>>>(d_probs, d_elem_prob, fe, md, n);
+    mark(UKHIP_AS_FILTERED, 1);
+  } else {
+    mark(UKHIP_AS_FELEM, 0);
+    mark(UKHIP_AS_FSCAN, 0);
+    mark(UKHIP_AS_FILTERED, 0);
   }
   {
     const unsigned nc = (unsigned)sch.size();
     ka_selem<NB><<<dim3((unsigned)S), blk, 0, st>>>(d_probs, d_step_prob, md, se, n);
+    mark(UKHIP_AS_SELEM, 1);
     ka_scan_reduce<NB, false><<<dim3(nc), blk, 0, st>>>(d_sch, se, stt[0], n, gw);
-    int cur = 0;
+    int cur = 0, rounds = 0;
     for (int64_t d = 1; d < maxr_s; d <<= 1) {
       ka_scan_round<NB, false><<<dim3(nc), blk, 0, st>>>(d_sch, stt[cur], stt[cur ^ 1], (int)d, n, gw);
       cur ^= 1;
+      ++rounds;
     }
     ka_scan_down<NB, false><<<dim3(nc), blk, 0, st>>>(d_sch, se, stt[cur], n, gw);
+    mark(UKHIP_AS_SSCAN, 2 + rounds);
     ka_results<<<dim3((unsigned)S), blk, 0, st>>>(se, xout, Sout, n);
+    mark(UKHIP_AS_RESULTS, 1);
   }
   if (!ok(hipGetLastError(), "associative launches")) return -1;
   /* the host tables above must outlive the asynchronous upload */
-  return ok(hipStreamSynchronize(st), "associative smoothing") ? 0 : -1;
+  if (!ok(hipStreamSynchronize(st), "associative smoothing")) return -1;
+  if (tm) {
+    for (int i = 0; i < UKHIP_AS_TOTAL; ++i) {
+      float f = 0.f;
+      if (!ok(hipEventElapsedTime(&f, tm->ev[i], tm->ev[i + 1]), "associative timing")) return -1;
+      tm->ms[i] += f;
+    }
+    float f = 0.f;
+    if (!ok(hipEventElapsedTime(&f, tm->ev[0], tm->ev[UKHIP_AS_TOTAL]), "associative timing")) return -1;
+    tm->ms[UKHIP_AS_TOTAL] += f;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -1190,4 +1235,26 @@ extern "C" int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options) {
     case 64: return run_assoc<64>(b);
     default: uk_set_error("associative: state dimension %d is above 64", raw->n); return -1;
   }
+}
+
+/* ukhip_batch_smooth_associative_timed (include/ukhip.h): parts one after
+ * another, each with its events; the slot sums are returned. */
+extern "C" int uk_assoc_smooth_timed(struct ukhip_batch* b, uint32_t options, double* ms, int32_t* launches) {
+  AsTimer tm;
+  for (auto& e : tm.ev)
+    if (!ok(hipEventCreate(&e), "hipEventCreate")) {
+      for (auto& x : tm.ev)
+        if (x) (void)hipEventDestroy(x);
+      return -1;
+    }
+  t_timer = &tm;
+  const int rc = uk_assoc_smooth(b, options);
+  t_timer = nullptr;
+  for (auto& e : tm.ev) (void)hipEventDestroy(e);
+  if (rc != 0) return rc;
+  for (int i = 0; i < UKHIP_AS_SLOTS; ++i) {
+    if (ms) ms[i] += tm.ms[i];
+    if (launches) launches[i] += tm.launches[i];
+  }
+  return 0;
 }

@@ -42,6 +42,7 @@ hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, doub
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
                              hipStream_t st);
 int uk_assoc_smooth(struct ukhip_batch* b, uint32_t options);
+int uk_assoc_smooth_timed(struct ukhip_batch* b, uint32_t options, double* ms, int32_t* launches);
 void uk_set_error(const char* fmt, ...);
 hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                  const uk_factors32_t* fac, float* fix, int n, hipStream_t st, hipEvent_t* ev);
@@ -1319,6 +1320,17 @@ extern "C" int ukhip_batch_smooth_associative(ukhip_batch_t* b, uint32_t options
   if (b->parts.empty()) return uk_assoc_smooth(b, options);
   for (auto* q : b->parts)
     if (uk_assoc_smooth(q, options) != 0) return -1;
+  return 0;
+}
+
+extern "C" int ukhip_batch_smooth_associative_timed(ukhip_batch_t* b, uint32_t options, double* ms,
+                                                    int32_t* launches) {
+  if (!b) { uk_set_error("NULL batch"); return -1; }
+  if (!ensure_stream()) return -1;
+  if (!ensure_elim_all(b)) return -1;
+  if (b->parts.empty()) return uk_assoc_smooth_timed(b, options, ms, launches);
+  for (auto* q : b->parts)
+    if (uk_assoc_smooth_timed(q, options, ms, launches) != 0) return -1;
   return 0;
 }
 

@@ -29,6 +29,8 @@ _lib = None
 # slots of ukhip_batch_smooth_oddeven_timed (include/ukhip.h)
 OE_KERNELS = ("k_weigh", "k_phase1", "k_phase2", "k_phase3", "k_backsub", "total", "k_solve_est", "k_normal",
               "k_fwd", "convert")  # the UKHIP_OE_* slots of include/ukhip.h, in order
+# slots of ukhip_batch_smooth_associative_timed (UKHIP_AS_*), in order
+AS_KERNELS = ("weigh", "model", "prior", "felem", "fscan", "filtered", "selem", "sscan", "results", "total")
 
 
 def lib():
@@ -54,6 +56,9 @@ def lib():
         L.ukhip_batch_timed_parts.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double),
                                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
         L.ukhip_batch_timed_parts.restype = C.c_int
+        L.ukhip_batch_smooth_associative_timed.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
+                                                           C.POINTER(C.c_int32)]
+        L.ukhip_batch_smooth_associative_timed.restype = C.c_int
         L.ukhip_batch_download.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ukhip_batch_download.restype = C.c_int
         L.ukhip_batch_download_cov.argtypes = [C.c_void_p, C.c_int32, C.c_char, C.c_void_p]
@@ -390,6 +395,15 @@ class Batch:
         nl = (C.c_int32 * len(OE_KERNELS))()
         _check(lib().ukhip_batch_smooth_oddeven_timed(self.h, options, ms, nl), "smooth_timed")
         return dict(zip(OE_KERNELS, list(ms))), dict(zip(OE_KERNELS, list(nl)))
+
+    def smooth_associative_timed(self, options=0):
+        """Associative smoothing timed with HIP events after each kernel class
+        (ukhip_batch_smooth_associative_timed). Returns ({class: ms},
+        {class: launches}) over AS_KERNELS."""
+        ms = (C.c_double * len(AS_KERNELS))()
+        nl = (C.c_int32 * len(AS_KERNELS))()
+        _check(lib().ukhip_batch_smooth_associative_timed(self.h, options, ms, nl), "smooth_associative_timed")
+        return dict(zip(AS_KERNELS, list(ms))), dict(zip(AS_KERNELS, list(nl)))
 
     def timed_parts(self, options=0, mixed=None):
         """ukhip_batch_timed_parts: one eager smoothing (FP64 with `options`, or

@@ -204,3 +204,49 @@ def oddeven_backsub_bytes(n, k, cov=True):
             break
         L //= 2
     return 8.0 * total
+
+
+def _mldivide(N, P):
+    """matrix_create_mldivide(A (N x N), B (N x P)) of the reference
+    (c/matrix_ops.c:610-630): dgeqrf, dormqr, dtrtrs."""
+    return _qr(N, N) + _ormqr(N, N, P) + _trtrs(N, P)
+
+
+def associative_flops(n, m_list):
+    """FLOPs of kalman_smooth_associative on one problem (constant n, square
+    H, 'w' covariances so that the explicit covariances cost nothing), per
+    kernel class of the GPU path, from the reference's matrix operations:
+      felem  build_filtering_element_new, observed steps
+             (c/kalman_associative_smoother.c:370-505), per step with m rows;
+      fscan  filteringAssociativeOperation (:889-980) per combine, two
+             combines per element (the work-efficient parallel scan the
+             reference runs through prefix_sums_pointers, c/parallel_tbb.cpp:78-121;
+             a sequential scan would do one);
+      selem  build_smoothing_element_new (:697-753), every step but the last;
+      sscan  smoothingAssociativeOperation (:982-1016), two per element;
+      prior  the first step's information filter (:335-368), once.
+    weigh, model, filtered and results are the GPU path's own data movement
+    (no reference arithmetic): zero. Vector terms are included."""
+    k = len(m_list)
+    f = dict(weigh=0.0, model=0.0, prior=0.0, felem=0.0, fscan=0.0, filtered=0.0, selem=0.0, sscan=0.0,
+             results=0.0)
+    g = _gemm
+    for i, m in enumerate(m_list[1:], start=1):
+        if m == 0:
+            continue
+        fe = (g(n, m, n) + g(m, m, n) + _mldivide(m, n) + g(n, m, n)   # K_i G^T, G K G^T, S^-T G, K
+              + g(m, n, n) + g(n, n, m) + g(m, 1, n) + g(n, 1, m)      # G F, K G F, G c, K (o - G c)
+              + g(n, n, m) + g(n, n, n) + g(m, 1, n) + g(n, m, n)      # K G, K G K_i, G c, F^T G^T S^-1
+              + g(n, 1, m) + g(m, n, n) + g(n, n, m))                  # e, G F, J
+        f["felem"] += fe
+    combine_f = (g(n, n, n) + _mldivide(n, n) + g(n, n, n) + _mldivide(n, n) + g(n, n, n) + g(n, 1, n)
+                 + g(n, 1, n) + g(n, n, n) + g(n, n, n) + g(n, 1, n) + g(n, 1, n) + g(n, n, n) + g(n, n, n))
+    combine_s = g(n, n, n) + g(n, 1, n) + g(n, n, n) + g(n, n, n)
+    f["fscan"] = 2.0 * max(k - 1, 0) * combine_f
+    f["sscan"] = 2.0 * k * combine_s
+    selem = g(n, n, n) + g(n, n, n) + _mldivide(n, n) + g(n, 1, n) + g(n, 1, n) + g(n, n, n) + g(n, n, n)
+    f["selem"] = max(k - 1, 0) * selem
+    m0 = m_list[0] if m_list else 0
+    f["prior"] = g(n, m0, n) + g(n, n, m0) + _mldivide(n, n) if m0 else 0.0
+    f["total"] = sum(v for key, v in f.items())
+    return f
