@@ -230,10 +230,29 @@ typedef struct {
   const double* o;    /* nfilters * m */
   const int32_t* m_i; /* nfilters observation counts (0 = unobserved), or NULL for m */
   const double* C;    /* nfilters * m: 'w' representation */
+  /* representations of K and C for this step, as kalman_evolve / kalman_observe
+   * take them (c/kalman.h:126-127): 0 or 'w' the vectors above; 'W', 'C', 'U'
+   * or 'F' a matrix per filter (K: nfilters * n*n; C: nfilters * m*m, ld m),
+   * whitened on the device first (c/kalman_base.c:61-175) */
+  char K_type;
+  char C_type;
 } ukhip_filter_inputs_t;
 ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int32_t m);
 /* One step of every filter (the first call has no evolution equation). */
 int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in);
+/* Keeps every step's final factors (Rdiag, Rsupdiag, y of the reference's
+ * UltimateKalman step) in device memory for ukhip_filter_smooth: at most
+ * max_steps steps (nfilters * (max_steps - 1) * (2 n*n + n) doubles). Call it
+ * before the first step. Returns 0 or -1. */
+int ukhip_filter_keep_history(ukhip_filter_t* f, int64_t max_steps);
+/* The smoother pass of the reference's UltimateKalman (c/kalman_ultimate.c:
+ * 559-620) over the steps so far, on the device: states (steps * nfilters *
+ * n: step after step, filter after filter) and, if W is not NULL, every
+ * step's covariance in the 'W' representation (steps * nfilters * n*n,
+ * column-major; covariance = (W^T W)^{-1}, as kalman_covariance returns with
+ * type 'W' after kalman_smooth). Needs ukhip_filter_keep_history. The filter
+ * state is unchanged (further steps may follow). Returns 0 or -1. */
+int ukhip_filter_smooth(ukhip_filter_t* f, double* states, double* W);
 /* The latest filtered estimates (nfilters * n; NaN while a filter's state is
  * not yet determined, as the reference returns) and, if R is not NULL, the
  * covariances in the reference's 'W' representation: R (nfilters * n*n, upper

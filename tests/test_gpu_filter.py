@@ -146,3 +146,121 @@ def test_last_step_is_the_smoothed_step(dev):
             assert ex < 1e-9 and eS < 1e-9
     finally:
         b.free()
+
+
+def _full_inputs(probs, i, n, m, kt, ct):
+    """Step i of every problem (random_problem: per-step F, H, c, dense K and C
+    in representations kt / ct) as ukhip_filter_step arrays."""
+    nf = len(probs)
+    F = np.stack([pr.F[i] if pr.F[i] is not None else np.eye(n) for pr in probs])
+    H = np.stack([pr.H[i] if pr.H[i] is not None else np.eye(n) for pr in probs])
+    c = np.stack([(pr.c[i] if pr.c[i] is not None else np.zeros((n, 1))).ravel() for pr in probs])
+    Kd = np.ones((nf, n)) if kt == "w" else np.stack([np.eye(n)] * nf)
+    G, o, mi = np.zeros((nf, m, n)), np.zeros((nf, m)), np.zeros(nf, dtype=np.int32)
+    Cd = np.ones((nf, m)) if ct == "w" else np.stack([np.eye(m)] * nf)
+    for p, pr in enumerate(probs):
+        if pr.K[i] is not None:
+            Kd[p] = pr.K[i].ravel() if kt == "w" else pr.K[i]
+        if pr.o[i] is not None:
+            G[p], o[p], mi[p] = pr.G[i], pr.o[i].ravel(), m
+            Cd[p] = pr.C[i].ravel() if ct == "w" else pr.C[i]
+    return dict(F=F, H=H, c=c, K=Kd, G=G, o=o, C=Cd, m_i=mi, K_type=kt, C_type=ct)
+
+
+@pytest.mark.parametrize("kt,ct", [("C", "W"), ("W", "C"), ("U", "w"), ("w", "F")])
+def test_full_covariance_representations(dev, kt, ct):
+    """K and C given as matrices (whitened on the device by k_weigh before the
+    filter step), with random H, F, c per step: estimates and information
+    matrices against the reference's ULTIMATE filter step by step."""
+    nf, n, m, k = 4, 6, 3, 30
+    probs = [P.random_problem(950 + p, k, n, m, K_kind=kt, C_kind=ct, H_kind="R",
+                              unobserved=tuple(range(p % 3 + 2, k, 7))) for p in range(nf)]
+    eqs = [capi.EquationSet(pr) for pr in probs]
+    L, kals = _reference_filters(probs)
+    flt = dev.Filter(nf, n, m)
+    worst_x, worst_info, determined = 0.0, 0.0, 0
+    try:
+        for i in range(k):
+            flt.step(**_full_inputs(probs, i, n, m, kt, ct))
+            x, R = flt.estimates()
+            for p, (kal, es) in enumerate(zip(kals, eqs)):
+                s = es.structs[i]
+                L.kalman_evolve(kal, n, s.H, s.F, s.c, s.K, kt.encode())
+                L.kalman_observe(kal, s.G, s.o, s.C, ct.encode())
+                em, cm = L.kalman_estimate(kal, i), L.kalman_covariance(kal, i)
+                rx, rc = capi._read(em)[:, 0], capi._read(cm)
+                L.matrix_free(em)
+                L.matrix_free(cm)
+                if np.all(np.isnan(rx)):
+                    assert np.all(np.isnan(x[p])), (p, i)
+                    continue
+                determined += 1
+                worst_x = max(worst_x, rel_err(x[p], rx))
+                worst_info = max(worst_info, rel_err(R[p].T @ R[p], rc.T @ rc))
+        say(f"K '{kt}', C '{ct}': {determined} determined steps, estimates {worst_x:.2e}, "
+            f"information matrices {worst_info:.2e}")
+        assert determined > nf * (k - 6) and worst_x < 1e-9 and worst_info < 1e-9
+    finally:
+        flt.close()
+        for kal in kals:
+            L.kalman_free(kal)
+
+
+@pytest.mark.parametrize("kind", ["w", "full"])
+def test_smoother_pass_matches_reference(dev, kind):
+    """ukhip_filter_smooth over the kept history against the reference's
+    ULTIMATE kalman_smooth (c/kalman_ultimate.c:559-620) on every step:
+    smoothed estimates, and covariances as information matrices W^T W; and
+    against the odd-even smoother on the GPU (covariance = (W^T W)^{-1})."""
+    nf, n, m, k = 4, 6, 3, 40
+    if kind == "w":
+        probs = _problems(nf, n, m, k, unob=True)
+        inputs = [_inputs(probs, i, n, m) for i in range(k)]
+        kt = ct = "w"
+    else:
+        kt, ct = "C", "W"
+        probs = [P.random_problem(960 + p, k, n, m, K_kind=kt, C_kind=ct, H_kind="R",
+                                  unobserved=tuple(range(p % 3 + 2, k, 7))) for p in range(nf)]
+        inputs = [_full_inputs(probs, i, n, m, kt, ct) for i in range(k)]
+    eqs = [capi.EquationSet(pr) for pr in probs]
+    L, kals = _reference_filters(probs)
+    L.kalman_smooth.argtypes = [C.c_void_p]
+    flt = dev.Filter(nf, n, m)
+    try:
+        flt.keep_history(k)
+        for i in range(k):
+            flt.step(**inputs[i])
+            for kal, es in zip(kals, eqs):
+                s = es.structs[i]
+                L.kalman_evolve(kal, n, s.H, s.F, s.c, s.K, kt.encode())
+                L.kalman_observe(kal, s.G, s.o, s.C, ct.encode())
+        xs, Ws = flt.smooth()
+        worst_x, worst_info = 0.0, 0.0
+        for p, kal in enumerate(kals):
+            L.kalman_smooth(kal)
+            for i in range(k):
+                em, cm = L.kalman_estimate(kal, i), L.kalman_covariance(kal, i)
+                rx, rc = capi._read(em)[:, 0], capi._read(cm)
+                assert L.kalman_covariance_type(kal, i) == b"W"
+                L.matrix_free(em)
+                L.matrix_free(cm)
+                worst_x = max(worst_x, rel_err(xs[i, p], rx))
+                worst_info = max(worst_info, rel_err(Ws[i, p].T @ Ws[i, p], rc.T @ rc))
+        b = dev.Batch(probs if kind == "w" else [P.packed_from_lists(pr) for pr in probs])
+        try:
+            b.smooth("oddeven")
+            worst_oe = 0.0
+            for p in range(nf):
+                xo, So = b.download(p)
+                for i in range(k):
+                    worst_oe = max(worst_oe, rel_err(xs[i, p], xo[i]),
+                                   rel_err(np.linalg.inv(Ws[i, p].T @ Ws[i, p]), So[i]))
+        finally:
+            b.free()
+        say(f"filter smoother ({kind}): vs reference estimates {worst_x:.2e}, information matrices "
+            f"{worst_info:.2e}; vs odd-even {worst_oe:.2e}")
+        assert worst_x < 1e-9 and worst_info < 1e-9 and worst_oe < 1e-9
+    finally:
+        flt.close()
+        for kal in kals:
+            L.kalman_free(kal)

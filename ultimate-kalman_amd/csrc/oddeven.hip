@@ -1031,11 +1031,17 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT))
   if (!first) {
     if (isP) {
       col.load_block(0, st.R + f * nn + (int64_t)c * n, n, sub);
-      col.load_block(1, in.F + (in.F_shared ? 0 : f * nn) + (int64_t)c * n, n, sub);
-      col.scale_block(1, kw, n, sub, -1.0);
+      if (in.pre) {
+        col.load_block(1, in.wB + f * nn + (int64_t)c * n, n, sub); /* -V F */
+      } else {
+        col.load_block(1, in.F + (in.F_shared ? 0 : f * nn) + (int64_t)c * n, n, sub);
+        col.scale_block(1, kw, n, sub, -1.0);
+      }
     }
     if (isB) {
-      if (in.H) {
+      if (in.pre) {
+        col.load_block(1, in.wD + f * nn + (int64_t)bc * n, n, sub); /* V H */
+      } else if (in.H) {
         col.load_block(1, in.H + f * nn + (int64_t)bc * n, n, sub);
         col.scale_block(1, kw, n, sub, 1.0);
       } else {
@@ -1048,24 +1054,37 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT))
     }
     if (isY) {
       col.load_block(0, st.y + f * n, n, sub);
-      if (in.c) {
+      if (in.pre) {
+        col.load_block(1, in.wc + f * n, n, sub); /* V c */
+      } else if (in.c) {
         col.load_block(1, in.c + f * n, n, sub);
         col.scale_block(1, kw, n, sub, 1.0);
       }
     }
     house_qr<NB, SL, double>(col, refl, c, sub, 0, n, isP || isB || isY);
+    if (st.hist) { /* the previous step's final rows: Rdiag, Rsupdiag, y (c/kalman_ultimate.c:140-260) */
+      const int64_t hs = st.hslot * st.nf + f;
+      if (isP) col.store_triu(st.hRd + hs * nn + (int64_t)c * n, n, c, sub);
+      if (isB) col.store_block(0, st.hRs + hs * nn + (int64_t)bc * n, n, sub);
+      if (isY) col.store_block(0, st.hy + hs * n, n, sub);
+    }
     if (isB || isY) col.move_1_to_0();
   }
   const int m = in.m_i ? in.m_i[f] : mr;
   if (m > 0) {
-    const double* cw = in.C + f * mr;
-    if (isB) {
-      col.load_block(1, in.G + f * mr * n + (int64_t)bc * mr, m, sub);
-      col.scale_block(1, cw, m, sub, 1.0);
-    }
-    if (isY) {
-      col.load_block(1, in.o + f * mr, m, sub);
-      col.scale_block(1, cw, m, sub, 1.0);
+    if (in.pre) {
+      if (isB) col.load_block(1, in.wC + f * nn + (int64_t)bc * mr, m, sub); /* W G, ld mr in an n*n slot */
+      if (isY) col.load_block(1, in.wo + f * n, m, sub);                      /* W o */
+    } else {
+      const double* cw = in.C + f * mr;
+      if (isB) {
+        col.load_block(1, in.G + f * mr * n + (int64_t)bc * mr, m, sub);
+        col.scale_block(1, cw, m, sub, 1.0);
+      }
+      if (isY) {
+        col.load_block(1, in.o + f * mr, m, sub);
+        col.scale_block(1, cw, m, sub, 1.0);
+      }
     }
   }
   house_qr<NB, SL, double>(col, refl, c, sub, n, n, isB || isY);
@@ -1103,6 +1122,104 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT))
     if (tid < n) st.x[f * n + tid] = r;
   }
   if (tid == 0) st.z[f] = z;
+}
+
+/*
+ * The smoother pass of the batched UltimateKalman filters over the kept
+ * history, one workgroup per filter, steps last .. 0 (c/kalman_ultimate.c:
+ * 559-620): x_last = the filtered estimate, x_s = Rdiag_s^{-1} (y_s -
+ * Rsupdiag_s x_{s+1}); the covariance factor ('W': covariance = (W^T W)^{-1})
+ * W_last = R_last, and W_s = the last n rows of Q^T [Rdiag_s; 0] for the QR
+ * [Rsupdiag_s; W_{s+1}] = Q R. Columns (SL lanes each): [0, n) the pivots
+ * [Rsupdiag_s; W_{s+1}], [n, 2n) [Rdiag_s; 0]. xs and Ws: slot s of filter f
+ * at (s * nf + f) * (n | n*n).
+ */
+template <int NB>
+struct SmoothShared {
+  double Wn[NB * NB]; /* W_{s+1}, column-major, ld NB */
+  double Rd[NB * NB];
+  double rs[NB];      /* y_s - Rsupdiag_s x_{s+1} */
+  double xn[NB];      /* x_{s+1} */
+  double rinv[NB];
+};
+
+template <int NB>
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB, 1>::NT))
+    k_filter_smooth(uk_filter_t st, int n, int64_t steps, double* __restrict__ xs, double* __restrict__ Ws) {
+  constexpr int SL = QRGeom<NB, 2 * NB, 1>::SL;
+  constexpr int NT = QRGeom<NB, 2 * NB, 1>::NT;
+  __shared__ Refl<NB, SL, double> refl;
+  __shared__ SmoothShared<NB> sh;
+  const int tid = threadIdx.x, sub = tid & (SL - 1), c = tid / SL;
+  const int64_t f = blockIdx.x, nf = st.nf;
+  const int64_t nn = (int64_t)n * n;
+  const bool isP = c < n, isS = c >= n && c < 2 * n;
+  const int sc = isS ? c - n : 0;
+  /* the last step: the filter's own factor and estimate */
+  {
+    const int64_t o = (steps - 1) * nf + f;
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, q = i / NB;
+      sh.Wn[i] = (r < n && q < n) ? st.R[f * nn + (int64_t)q * n + r] : 0.0;
+    }
+    for (int i = tid; i < n; i += NT) sh.xn[i] = xs[o * n + i] = st.x[f * n + i];
+    for (int i = tid; i < nn; i += NT) Ws[o * nn + i] = st.R[f * nn + i];
+  }
+  __syncthreads();
+  for (int64_t s = steps - 2; s >= 0; --s) {
+    const int64_t hs = s * nf + f;
+    const double* Rd = st.hRd + hs * nn;
+    const double* Rs = st.hRs + hs * nn;
+    /* estimate: rs = y - Rsupdiag x_{s+1} (row per thread), then the triangular solve on wave 0 */
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, q = i / NB;
+      sh.Rd[i] = (r < n && q < n) ? Rd[(int64_t)q * n + r] : (r == q ? 1.0 : 0.0);
+    }
+    if (tid < n) {
+      double v = st.hy[hs * n + tid];
+      for (int j = 0; j < n; ++j) v = fma(-Rs[(int64_t)j * n + tid], sh.xn[j], v);
+      sh.rs[tid] = v;
+    }
+    __syncthreads();
+    if (tid < n) sh.rinv[tid] = 1.0 / sh.Rd[tid + tid * NB];
+    __syncthreads();
+    if (tid < 64) {
+      double r = tid < n ? sh.rs[tid] : 0.0;
+      for (int q = n - 1; q >= 0; --q) {
+        const double xq = __shfl(r, q) * sh.rinv[q];
+        if (tid == q) r = xq;
+        else if (tid < q) r -= sh.Rd[tid + q * NB] * xq;
+      }
+      if (tid < n) xs[hs * n + tid] = r;
+      if (tid < n) sh.rs[tid] = r; /* x_s, the next iteration's x_{s+1} */
+    }
+    /* covariance factor: QR of [Rsupdiag_s; W_{s+1}] applied to [Rdiag_s; 0] */
+    Col<NB, SL, double> col;
+    col.zero_block(0);
+    col.zero_block(1);
+    if (isP) {
+      col.load_block(0, Rs + (int64_t)c * n, n, sub);
+#pragma unroll
+      for (int l = 0; l < NB / SL; ++l) {
+        const int r = SL * l + sub;
+        col.a[NB / SL + l] = r < n ? sh.Wn[r + c * NB] : 0.0;
+      }
+    }
+    if (isS) col.load_block(0, Rd + (int64_t)sc * n, n, sub);
+    house_qr<NB, SL, 1, double, QF_DONE>(reinterpret_cast<Col<NB, SL, double>(&)[1]>(col), refl, c, sub, 0, n,
+                                        (isP || isS) ? 1u : 0u);
+    /* house_qr ends with a barrier: every read of Wn and xn above is done */
+    if (isS) {
+      col.store_block(1, Ws + hs * nn + (int64_t)sc * n, n, sub);
+#pragma unroll
+      for (int l = 0; l < NB / SL; ++l) {
+        const int r = SL * l + sub;
+        if (r < n) sh.Wn[r + sc * NB] = col.a[NB / SL + l];
+      }
+    }
+    if (tid < n) sh.xn[tid] = sh.rs[tid];
+    __syncthreads();
+  }
 }
 
 } /* namespace */
@@ -1377,6 +1494,19 @@ extern "C" hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n
     default: return hipErrorInvalidValue;
   }
 #undef C_
+  return hipGetLastError();
+}
+
+/* The smoother pass over every filter's history (k_filter_smooth). */
+extern "C" hipError_t uk_launch_filter_smooth(const uk_filter_t* st, int n, int64_t steps, double* xs, double* Ws,
+                                              hipStream_t s) {
+  const int nb = uk_bucket(n);
+#define S_(NBV) k_filter_smooth<NBV><<<dim3((unsigned)st->nf), dim3(QRGeom<NBV, 2 * NBV, 1>::NT), 0, s>>>(*st, n, steps, xs, Ws)
+  switch (nb) {
+    UK_DISPATCH(8, S_) UK_DISPATCH(16, S_) UK_DISPATCH(32, S_) UK_DISPATCH(48, S_) UK_DISPATCH(64, S_)
+    default: return hipErrorInvalidValue;
+  }
+#undef S_
   return hipGetLastError();
 }
 

@@ -37,6 +37,8 @@ hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const
                                const uk_factors_t* fac, double* fix, int n, hipStream_t st, hipEvent_t* ev);
 hipError_t uk_launch_filter_step(const uk_filter_t* st, const uk_filter_in_t* in, int nf, int n, int first,
                                  hipStream_t s);
+hipError_t uk_launch_filter_smooth(const uk_filter_t* st, int n, int64_t steps, double* xs, double* Ws,
+                                   hipStream_t s);
 hipError_t uk_launch_cov_format(const double* S, int64_t steps, int n, int fmt, double* dst, hipStream_t st);
 hipError_t uk_launch_backsub(const uk_level_t* lv, const uk_factors_t* fac, double* xout, double* Sout,
                              const double* Xup, const double* XTup, double* Xcur, double* XTcur, int n, int cov,
@@ -2351,6 +2353,21 @@ struct ukhip_filter {
   char* d_in = nullptr;
   char* h_in = nullptr; /* pinned staging of one step's inputs */
   size_t in_bytes = 0;
+  /* full-representation K or C: the step whitened by k_weigh into wch, with
+   * per-filter tables for its uk_raw_t view (F block indices own / shared,
+   * first-step flags 1 / 0, m_i = mr, a zero vector) */
+  void* d_aux = nullptr;
+  const int64_t *fidx_own = nullptr, *fidx_zero = nullptr;
+  const int8_t *first1 = nullptr, *first0 = nullptr;
+  const int32_t* mfull = nullptr;
+  const double* zeros = nullptr;
+  void* d_wch = nullptr;
+  uk_chain_t wch{};
+  /* history (ukhip_filter_keep_history) and the smoother's outputs */
+  void* d_hist = nullptr;
+  int64_t hist_cap = 0; /* slots: steps 0 .. hist_cap - 1 */
+  void* d_smooth = nullptr;
+  int64_t smooth_cap = 0;
 };
 
 extern "C" ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int32_t m) {
@@ -2365,8 +2382,9 @@ extern "C" ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int3
   f->mr = std::max(m, 1);
   const size_t P = (size_t)nfilters, nn = (size_t)n * n, mr = (size_t)f->mr;
   const size_t state_bytes = P * (nn + 2 * (size_t)n) * 8 + P * 4;
-  /* one step's inputs: F, H, c, K, G, o, C, m_i */
-  f->in_bytes = P * (2 * nn + 2 * (size_t)n + mr * n + 2 * mr) * 8 + P * 4;
+  /* one step's inputs: F, H, c, K (n*n at most), G, o, C (m*m at most), m_i,
+   * the K and C type codes; each array 256-byte aligned */
+  f->in_bytes = P * (3 * nn + (size_t)n + mr * n + mr + mr * mr) * 8 + P * 6 + 12 * 256;
   bool ok = hip_ok(hipMalloc(&f->d_state, state_bytes), "hipMalloc(filter state)") &&
             hip_ok(hipMemsetAsync(f->d_state, 0, state_bytes, g_stream), "hipMemset(filter state)") &&
             hip_ok(hipMalloc((void**)&f->d_in, f->in_bytes), "hipMalloc(filter inputs)") &&
@@ -2380,13 +2398,70 @@ extern "C" ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int3
   f->st.y = (double*)(d + P * nn * 8);
   f->st.x = (double*)(d + P * (nn + n) * 8);
   f->st.z = (int32_t*)(d + P * (nn + 2 * (size_t)n) * 8);
+  f->st.nf = nfilters;
   return f;
 }
+
+/* The tables and whitened-step buffers of the full-representation path,
+ * allocated on its first use. */
+static bool filter_aux(ukhip_filter* f) {
+  if (f->d_aux) return true;
+  const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n;
+  std::vector<char> h;
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t at = h.size();
+    h.resize(at + ((bytes + 255) & ~(size_t)255), 0);
+    std::memcpy(h.data() + at, src, bytes);
+    return at;
+  };
+  std::vector<int64_t> own(P), zero(P, 0);
+  for (size_t p = 0; p < P; ++p) own[p] = (int64_t)p;
+  std::vector<int8_t> one8(P, 1), zero8(P, 0);
+  std::vector<int32_t> mf(P, f->mr);
+  std::vector<double> z(P * n, 0.0);
+  const size_t o_own = put(own.data(), P * 8), o_zero = put(zero.data(), P * 8), o_f1 = put(one8.data(), P),
+               o_f0 = put(zero8.data(), P), o_m = put(mf.data(), P * 4), o_z = put(z.data(), P * n * 8);
+  if (!hip_ok(hipMalloc(&f->d_aux, h.size()), "hipMalloc(filter tables)")) return false;
+  if (!hip_ok(hipMemcpy(f->d_aux, h.data(), h.size(), hipMemcpyHostToDevice), "upload filter tables")) {
+    (void)hipFree(f->d_aux);
+    f->d_aux = nullptr;
+    return false;
+  }
+  char* a = (char*)f->d_aux;
+  f->fidx_own = (const int64_t*)(a + o_own);
+  f->fidx_zero = (const int64_t*)(a + o_zero);
+  f->first1 = (const int8_t*)(a + o_f1);
+  f->first0 = (const int8_t*)(a + o_f0);
+  f->mfull = (const int32_t*)(a + o_m);
+  f->zeros = (const double*)(a + o_z);
+  const size_t wbytes = P * (3 * nn + 2 * n) * 8;
+  if (!hip_ok(hipMalloc(&f->d_wch, wbytes), "hipMalloc(filter whitened step)")) return false;
+  double* w = (double*)f->d_wch;
+  f->wch.B = w;
+  f->wch.D = w + P * nn;
+  f->wch.C = w + 2 * P * nn;
+  f->wch.c = w + 3 * P * nn;
+  f->wch.o = w + 3 * P * nn + P * n;
+  f->wch.mr = f->mr;
+  return true;
+}
+
+static bool full_rep(char t) { return t == 'W' || t == 'C' || t == 'U' || t == 'F'; }
 
 extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in) {
   if (!f || !in) { uk_set_error("NULL filter or inputs"); return -1; }
   const bool first = f->steps == 0;
   const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n, mr = (size_t)f->mr;
+  const char kt = in->K_type ? in->K_type : 'w', ct = in->C_type ? in->C_type : 'w';
+  if ((kt != 'w' && !full_rep(kt)) || (ct != 'w' && !full_rep(ct))) {
+    uk_set_error("ukhip_filter_step: K_type and C_type must be 'w', 'W', 'C', 'U' or 'F'");
+    return -1;
+  }
+  const bool full = full_rep(kt) || full_rep(ct);
+  if (f->d_hist && !first && f->steps - 1 >= f->hist_cap) {
+    uk_set_error("ukhip_filter_step: the history holds %lld steps", (long long)(f->hist_cap + 1));
+    return -1;
+  }
   if (!in->K && !first) { uk_set_error("ukhip_filter_step: K is required after the first step"); return -1; }
   if (!first && !in->F) { uk_set_error("ukhip_filter_step: F is required after the first step"); return -1; }
   bool observed = in->m_i == nullptr;
@@ -2411,19 +2486,98 @@ extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t*
   din.F_shared = in->F_shared;
   din.H = (const double*)put(first ? nullptr : in->H, P * nn * 8);
   din.c = (const double*)put(first ? nullptr : in->c, P * n * 8);
-  din.K = (const double*)put(in->K, P * n * 8);
+  din.K = (const double*)put(in->K, P * (full_rep(kt) ? nn : n) * 8);
   din.G = (const double*)put(in->G, P * mcols * n * 8);
   din.o = (const double*)put(in->o, P * mcols * 8);
-  din.C = (const double*)put(in->C, P * mcols * 8);
+  din.C = (const double*)put(in->C, P * mcols * (full_rep(ct) ? mcols : 1) * 8);
   din.m_i = (const int32_t*)put(in->m_i, P * 4);
   din.mr = (int32_t)mr;
+  std::vector<int8_t> codes;
+  const int8_t* dKt = nullptr;
+  const int8_t* dCt = nullptr;
+  if (full) {
+    codes.assign(2 * P, 0);
+    std::fill(codes.begin(), codes.begin() + P, (int8_t)cov_code(kt));
+    std::fill(codes.begin() + P, codes.end(), (int8_t)cov_code(ct));
+    dKt = (const int8_t*)put(codes.data(), P);
+    dCt = (const int8_t*)put(codes.data() + P, P);
+  }
   if (!din.K) din.K = f->st.y; /* first step: K is never read */
-  if (off > f->in_bytes + 8 * 256) { uk_set_error("ukhip_filter_step: staging overflow"); return -1; }
-  if (!hip_ok(hipMemcpyAsync(f->d_in, f->h_in, off, hipMemcpyHostToDevice, g_stream), "filter upload") ||
-      !hip_ok(uk_launch_filter_step(&f->st, &din, f->nf, f->n, first ? 1 : 0, g_stream), "filter step launch"))
+  if (off > f->in_bytes) { uk_set_error("ukhip_filter_step: staging overflow"); return -1; }
+  if (!hip_ok(hipMemcpyAsync(f->d_in, f->h_in, off, hipMemcpyHostToDevice, g_stream), "filter upload")) return -1;
+  if (full) { /* whiten the step on the device (k_weigh), then the filter step reads it */
+    if (!filter_aux(f)) return -1;
+    uk_raw_t raw{};
+    raw.n = f->n;
+    raw.mr = f->mr;
+    raw.steps = f->nf;
+    raw.H_ident = din.H == nullptr;
+    raw.K_full = full_rep(kt);
+    raw.C_full = full_rep(ct);
+    raw.F = din.F ? din.F : f->zeros;
+    raw.Fidx = in->F_shared ? f->fidx_zero : f->fidx_own;
+    raw.H = din.H;
+    raw.c = din.c ? din.c : f->zeros;
+    raw.K = din.K;
+    raw.G = din.G ? din.G : f->zeros;
+    raw.o = din.o ? din.o : f->zeros;
+    raw.C = din.C ? din.C : f->zeros;
+    raw.Kt = dKt;
+    raw.Ct = dCt;
+    raw.m = din.m_i ? din.m_i : f->mfull;
+    raw.first = first ? f->first1 : f->first0;
+    if (!hip_ok(uk_launch_weigh(&raw, &f->wch, g_stream), "filter whitening launch")) return -1;
+    din.pre = 1;
+    din.wB = f->wch.B;
+    din.wD = f->wch.D;
+    din.wc = f->wch.c;
+    din.wC = f->wch.C;
+    din.wo = f->wch.o;
+  }
+  f->st.hist = (f->d_hist && !first) ? 1 : 0;
+  f->st.hslot = first ? 0 : f->steps - 1;
+  if (!hip_ok(uk_launch_filter_step(&f->st, &din, f->nf, f->n, first ? 1 : 0, g_stream), "filter step launch"))
     return -1;
   ++f->steps;
   return 0;
+}
+
+extern "C" int ukhip_filter_keep_history(ukhip_filter_t* f, int64_t max_steps) {
+  if (!f) { uk_set_error("NULL filter"); return -1; }
+  if (f->steps > 0) { uk_set_error("ukhip_filter_keep_history: call it before the first step"); return -1; }
+  if (max_steps < 1) { uk_set_error("ukhip_filter_keep_history: max_steps must be >= 1"); return -1; }
+  if (f->d_hist) (void)hipFree(f->d_hist);
+  f->d_hist = nullptr;
+  f->hist_cap = 0;
+  const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n, slots = (size_t)std::max<int64_t>(max_steps - 1, 1);
+  if (!hip_ok(hipMalloc(&f->d_hist, slots * P * (2 * nn + n) * 8), "hipMalloc(filter history)")) return -1;
+  double* h = (double*)f->d_hist;
+  f->st.hRd = h;
+  f->st.hRs = h + slots * P * nn;
+  f->st.hy = h + 2 * slots * P * nn;
+  f->hist_cap = max_steps - 1;
+  return 0;
+}
+
+extern "C" int ukhip_filter_smooth(ukhip_filter_t* f, double* states, double* W) {
+  if (!f) { uk_set_error("NULL filter"); return -1; }
+  if (!f->d_hist) { uk_set_error("ukhip_filter_smooth: needs ukhip_filter_keep_history before the first step"); return -1; }
+  if (f->steps < 1) { uk_set_error("ukhip_filter_smooth: no steps yet"); return -1; }
+  const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n, K = (size_t)f->steps;
+  if (f->smooth_cap < f->steps) {
+    if (f->d_smooth) (void)hipFree(f->d_smooth);
+    f->d_smooth = nullptr;
+    f->smooth_cap = 0;
+    if (!hip_ok(hipMalloc(&f->d_smooth, K * P * (n + nn) * 8), "hipMalloc(filter smoother outputs)")) return -1;
+    f->smooth_cap = f->steps;
+  }
+  double* xs = (double*)f->d_smooth;
+  double* Ws = xs + (size_t)f->smooth_cap * P * n;
+  if (!hip_ok(uk_launch_filter_smooth(&f->st, f->n, f->steps, xs, Ws, g_stream), "filter smoother launch")) return -1;
+  if (states && !hip_ok(hipMemcpyAsync(states, xs, K * P * n * 8, hipMemcpyDeviceToHost, g_stream), "download"))
+    return -1;
+  if (W && !hip_ok(hipMemcpyAsync(W, Ws, K * P * nn * 8, hipMemcpyDeviceToHost, g_stream), "download")) return -1;
+  return hip_ok(hipStreamSynchronize(g_stream), "filter smoother") ? 0 : -1;
 }
 
 extern "C" int ukhip_filter_estimates(ukhip_filter_t* f, double* states, double* R) {
@@ -2442,6 +2596,10 @@ extern "C" void ukhip_filter_free(ukhip_filter_t* f) {
   if (f->d_state) (void)hipFree(f->d_state);
   if (f->d_in) (void)hipFree(f->d_in);
   if (f->h_in) (void)hipHostFree(f->h_in);
+  if (f->d_aux) (void)hipFree(f->d_aux);
+  if (f->d_wch) (void)hipFree(f->d_wch);
+  if (f->d_hist) (void)hipFree(f->d_hist);
+  if (f->d_smooth) (void)hipFree(f->d_smooth);
   delete f;
 }
 

@@ -144,6 +144,16 @@ typedef struct {
   double* y;
   int32_t* z;
   double* x;
+  /* history for the smoother pass (hist != 0): per step slot s (the step's
+   * final factors, fixed by the next step's evolve), Rd (n x n upper
+   * triangular: Rdiag), Rs (n x n: Rsupdiag, the coupling to the next step)
+   * and hy (n); slot s of filter f at (s * nf + f) * (n*n | n) */
+  double* hRd;
+  double* hRs;
+  double* hy;
+  int64_t hslot; /* slot the current launch writes (the previous step) */
+  int32_t hist;
+  int32_t nf;
 } uk_filter_t;
 
 /* One step's inputs for all filters (device pointers): F (n*n per filter, or
@@ -162,6 +172,16 @@ typedef struct {
   const double* C;
   int32_t F_shared;
   int32_t mr;
+  /* pre != 0: the step is already whitened (k_weigh, for K or C given in a
+   * full representation): wB = -V F, wD = V H (n*n per filter), wc = V c (n),
+   * wC = W G (ld mr, an n*n slot per filter), wo = W o (an n slot); F, H, c,
+   * K, G, o and C are then not read */
+  const double* wB;
+  const double* wD;
+  const double* wc;
+  const double* wC;
+  const double* wo;
+  int32_t pre;
 } uk_filter_in_t;
 
 #endif

@@ -90,6 +90,10 @@ def lib():
         L.ukhip_filter_estimates.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ukhip_filter_estimates.restype = C.c_int
         L.ukhip_filter_free.argtypes = [C.c_void_p]
+        L.ukhip_filter_keep_history.argtypes = [C.c_void_p, C.c_int64]
+        L.ukhip_filter_keep_history.restype = C.c_int
+        L.ukhip_filter_smooth.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ukhip_filter_smooth.restype = C.c_int
         capi.bind_smoothers(L, "kalman_smooth_oddeven", "kalman_smooth_associative", "matrix_free")
         _lib = L
     return _lib
@@ -210,7 +214,8 @@ def problem_struct(p, keep):
 class FilterInputs(C.Structure):
     """ukhip_filter_inputs_t (include/ukhip.h)."""
     _fields_ = [("F", C.c_void_p), ("F_shared", C.c_int32), ("H", C.c_void_p), ("c", C.c_void_p),
-                ("K", C.c_void_p), ("G", C.c_void_p), ("o", C.c_void_p), ("m_i", C.c_void_p), ("C", C.c_void_p)]
+                ("K", C.c_void_p), ("G", C.c_void_p), ("o", C.c_void_p), ("m_i", C.c_void_p), ("C", C.c_void_p),
+                ("K_type", C.c_char), ("C_type", C.c_char)]
 
 
 class Filter:
@@ -224,11 +229,28 @@ class Filter:
         if not self.h:
             raise UkhipError("ukhip_filter_create: " + last_error())
         self.nf, self.n, self.m = nfilters, n, m
+        self._steps = 0
 
-    def step(self, K, G, o, C, F=None, H=None, c=None, m_i=None):
+    def keep_history(self, max_steps):
+        """ukhip_filter_keep_history: keep every step's factors for smooth()."""
+        _check(lib().ukhip_filter_keep_history(self.h, int(max_steps)), "ukhip_filter_keep_history")
+
+    def smooth(self, want_W=True):
+        """ukhip_filter_smooth: (states (steps, nf, n), W (steps, nf, n, n) with
+        W^T W the inverse covariance) over the steps so far."""
+        steps = self._steps
+        x = np.empty((steps, self.nf, self.n))
+        W = np.empty((steps, self.nf, self.n, self.n)) if want_W else None
+        _check(lib().ukhip_filter_smooth(self.h, x.ctypes.data, W.ctypes.data if want_W else None),
+               "ukhip_filter_smooth")
+        return x, (np.swapaxes(W, 2, 3) if want_W else None)
+
+    def step(self, K, G, o, C, F=None, H=None, c=None, m_i=None, K_type="w", C_type="w"):
         """One evolve + observe step. Arrays hold all filters: F (nf, n, n) or
-        (n, n) shared, H/c optional, K (nf, n) and C (nf, m) inverse standard
-        deviations, G (nf, m, n), o (nf, m), m_i (nf,) observation counts."""
+        (n, n) shared, H/c optional, G (nf, m, n), o (nf, m), m_i (nf,)
+        observation counts; K and C in the representation K_type / C_type:
+        'w' inverse standard deviations (K (nf, n), C (nf, m)), or 'W', 'C',
+        'U', 'F' matrices (K (nf, n, n), C (nf, m, m))."""
         keep = []
 
         def ptr(a, dtype=np.float64):
@@ -247,12 +269,15 @@ class Filter:
         s.F = ptr(colmajor(Fa))
         s.H = ptr(colmajor(H))
         s.c = ptr(c)
-        s.K = ptr(K)
+        s.K = ptr(K if K_type == "w" else colmajor(K))
         s.G = ptr(colmajor(G))
         s.o = ptr(o)
-        s.C = ptr(C)
+        s.C = ptr(C if C_type == "w" else colmajor(C))
         s.m_i = ptr(m_i, np.int32)
+        s.K_type = K_type.encode()
+        s.C_type = C_type.encode()
         _check(lib().ukhip_filter_step(self.h, _ctypes.byref(s)), "ukhip_filter_step")
+        self._steps += 1
 
     def estimates(self, want_R=True):
         """(states (nf, n), R (nf, n, n) upper triangular with R^T R the inverse covariance)."""
