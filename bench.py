@@ -151,6 +151,28 @@ def traffic_of(kernel, workload):
             "algorithmic_bytes_per_step": e.get("algorithmic_bytes_per_step"), "source": "profiles/traffic.json"}
 
 
+def headline_traffic(kernel, specs, sample=600):
+    """HBM bytes per smoothing of an FP32 elimination kernel over the whole
+    batching workload: the committed PMC summary of one eager mixed smoothing
+    of the workload's first `sample` problems (bench.py --mixed-chunk-only),
+    scaled by the time steps of the kernel's bucket in the workload over those
+    in the sample. None when no summary covers the kernel."""
+    base = kernel.split(",")[0] + ", " + kernel.split(",")[1].strip() + ">" if kernel.count(",") >= 2 else kernel
+    t = traffic_of(base, f"batch_mixed_chunk{sample}")
+    if t is None:
+        return None
+    nb = int(base.split("<")[1].split(",")[0])
+    k_sample = sum(sp["k"] for sp in P.batch_specs(count=sample) if bucket_qr(sp["n"]) == nb)
+    k_all = sum(sp["k"] for sp in specs if bucket_qr(sp["n"]) == nb)
+    if k_sample == 0:
+        return None
+    f = k_all / k_sample
+    return {"bytes_per_step": t["bytes_per_step"] * f, "fetch_bytes": t["fetch_bytes"] * f,
+            "write_bytes": t["write_bytes"] * f, "algorithmic_bytes_per_step": None,
+            "source": f"profiles/traffic.json, batch_mixed_chunk{sample} ({base}), x{f:.3f}: bucket-{nb} time steps "
+                      f"of the workload over the sample's"}
+
+
 def copy_calibration(mib):
     """One device-to-device copy (the HIP runtime's copy kernel) of `mib` MiB
     whose source was last touched before a 1 GiB fill, so that it is read
@@ -398,7 +420,7 @@ def run_headline(args):
     value = total_steps / (acc["mixed"] * 1e-3)
     rates = {k: total_steps / (v * 1e-3) for k, v in acc.items()}
     roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak, "traffic": headline_traffic(dom, specs) if dom in slot_flops else None,
             "launches_per_step": slot_launch[dom], "kernel_ms_per_step": slot_ms[dom],
             "avg_launch_ms": slot_ms[dom] / max(slot_launch[dom], 1),
             "algorithmic_flops_per_step": slot_flops.get(dom), "algorithmic_bytes_per_step": slot_bytes.get(dom),

@@ -237,7 +237,9 @@ __host__ __device__ constexpr int qr_chunk(int nl, int cap) {
  * bit (their products are exact zeros in every dot product and update), and
  * the step code is specialised per segment, so the skipping costs no branch.
  */
-enum { QF_TRI1 = 1, QF_DONE = 2, QF_HALF0 = 4, QF_HALF1 = 8, QF_DIAG0 = 16 };
+/*   QF_SWAP: the registers of column 0 (group 0, slot 0) hold another column
+ *     from the second step on (house_qr_swap): never counted as finished. */
+enum { QF_TRI1 = 1, QF_DONE = 2, QF_HALF0 = 4, QF_HALF1 = 8, QF_DIAG0 = 16, QF_SWAP = 32 };
 __host__ __device__ constexpr int qr_gl(int H) { return H <= 16 ? 2 : 4; }
 /* the granules of a block that QF_HALF0 / QF_HALF1 keep: the first half,
  * rounded up (bucket 24: 2 of 3) */
@@ -382,16 +384,14 @@ __device__ __forceinline__ void house_publish(const Col<NB, SL, T>& col, Refl<NB
   }
 }
 
+/* The reflector published in buffer buf applied to the columns whose bit is
+ * set in upd (uniform over the group's lanes; with CPL == 1 the caller has
+ * checked upd). */
 template <int NB, int SL, int CPL, class T, int SEG, int QF>
-__device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int buf, int sub,
-                                           int piv_slot, unsigned upd, int k, T* V = nullptr, T* tauv = nullptr,
-                                           int ldv = 0) {
+__device__ __forceinline__ void house_update(Col<NB, SL, T> (&cols)[CPL], const Refl<NB, SL, T>& R, int buf, int sub,
+                                             unsigned upd) {
   constexpr int H = NB / SL;
-#pragma unroll
-  for (int s = 0; s < CPL; ++s)
-    if (s == piv_slot) house_publish<NB, SL, T, SEG, QF>(cols[s], R, buf, sub, k, V, tauv, ldv);
-  __syncthreads();
-  if (upd) {
+  {
     const T tau = R.sc[buf][0];
     if (tau != T(0)) {
       /* v is read from LDS in chunks of CH locals issued back to back. With
@@ -474,6 +474,17 @@ __device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB,
   }
 }
 
+template <int NB, int SL, int CPL, class T, int SEG, int QF>
+__device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int buf, int sub,
+                                           int piv_slot, unsigned upd, int k, T* V = nullptr, T* tauv = nullptr,
+                                           int ldv = 0) {
+#pragma unroll
+  for (int s = 0; s < CPL; ++s)
+    if (s == piv_slot) house_publish<NB, SL, T, SEG, QF>(cols[s], R, buf, sub, k, V, tauv, ldv);
+  __syncthreads();
+  if (upd) house_update<NB, SL, CPL, T, SEG, QF>(cols, R, buf, sub, upd);
+}
+
 /* Lanes per column group and threads of a QR kernel over COLS matrix
  * columns: 4 lanes per group up to 1024 threads, else 2. MINW is the waves
  * per SIMD asked of the register allocator so that W workgroups share a CU
@@ -482,11 +493,23 @@ __device__ __forceinline__ void house_step(Col<NB, SL, T> (&cols)[CPL], Refl<NB,
 /* WG > 0 overrides the table's workgroups per CU (the variants the run-time
  * autotuner chooses among); CPL is the number of columns each lane group
  * holds (GRP = ceil(COLS / CPL) groups). */
-template <int NB, int COLS, int PHASE, int WG = 0, int CPL = 1>
+/* SW: the kernel can run its QR with house_qr_swap, which needs the
+ * registers of COLS - 1 columns; SWAP is set where that saves a wave (the
+ * phases' COLS are a multiple of the bucket plus one right-hand side, so at
+ * n == NB the last column would otherwise open a wave of its own). */
+#ifndef UK_QR_SWAP
+#define UK_QR_SWAP 1
+#endif
+template <int NB, int COLS, int PHASE, int WG = 0, int CPL = 1, bool SW = false>
 struct QRGeom {
-  static constexpr int GRP = (COLS + CPL - 1) / CPL;
-  static constexpr int SL = (NB <= 16 || 4 * GRP > 1024) ? 2 : 4;
-  static constexpr int NT = ((SL * GRP + 63) / 64) * 64;
+  static constexpr int GRP0 = (COLS + CPL - 1) / CPL;
+  static constexpr int SL = (NB <= 16 || 4 * GRP0 > 1024) ? 2 : 4;
+  static constexpr int NT0 = ((SL * GRP0 + 63) / 64) * 64;
+  static constexpr int GRP1 = (COLS - 1 + CPL - 1) / CPL;
+  static constexpr int NT1 = ((SL * GRP1 + 63) / 64) * 64;
+  static constexpr bool SWAP = UK_QR_SWAP && SW && NT1 < NT0;
+  static constexpr int GRP = SWAP ? GRP1 : GRP0;
+  static constexpr int NT = SWAP ? NT1 : NT0;
   static constexpr int W = WG > 0 ? WG : 1;
   static constexpr int MINW_ = (W * (NT / 64) + 3) / 4;
   static constexpr int MINW = MINW_ < 1 ? 1 : (MINW_ > 8 ? 8 : MINW_);
@@ -496,23 +519,23 @@ struct QRGeom {
 template <int NB, int SL, int CPL, class T, int QF, int SEG = 0>
 __device__ __forceinline__ void house_qr_segs(Col<NB, SL, T> (&cols)[CPL], Refl<NB, SL, T>& R, int grp, int sub,
                                               int piv0, int n, unsigned upd, T* V = nullptr, T* tauv = nullptr,
-                                              int ldv = 0) {
+                                              int ldv = 0, int kbeg = 0) {
   constexpr int H = NB / SL, ROWS = qr_gl(H) * SL;
   if constexpr (SEG * ROWS < NB) {
     const int k1 = n < (SEG + 1) * ROWS ? n : (SEG + 1) * ROWS;
-    for (int k = SEG * ROWS; k < k1; ++k) {
+    for (int k = SEG * ROWS > kbeg ? SEG * ROWS : kbeg; k < k1; ++k) {
       const int pc = piv0 + k;
       unsigned u = upd;
       if (QF & QF_DONE) {
 #pragma unroll
         for (int s = 0; s < CPL; ++s) {
           const int cc = grp * CPL + s;
-          if (cc >= piv0 && cc < pc) u &= ~(1u << s);
+          if (cc >= piv0 && cc < pc && !((QF & QF_SWAP) && cc == 0)) u &= ~(1u << s);
         }
       }
       house_step<NB, SL, CPL, T, SEG, QF>(cols, R, k & 1, sub, pc / CPL == grp ? pc % CPL : -1, u, k, V, tauv, ldv);
     }
-    house_qr_segs<NB, SL, CPL, T, QF, SEG + 1>(cols, R, grp, sub, piv0, n, upd, V, tauv, ldv);
+    house_qr_segs<NB, SL, CPL, T, QF, SEG + 1>(cols, R, grp, sub, piv0, n, upd, V, tauv, ldv, kbeg);
   }
 }
 
@@ -530,6 +553,48 @@ template <int NB, int SL, class T>
 __device__ __forceinline__ void house_qr(Col<NB, SL, T>& col, Refl<NB, SL, T>& R, int mycol, int sub, int piv0, int n,
                                          bool upd) {
   house_qr<NB, SL, 1, T>(reinterpret_cast<Col<NB, SL, T>(&)[1]>(col), R, mycol, sub, piv0, n, upd ? 1u : 0u);
+}
+
+/*
+ * QR of pivots k = 0..n-1 on columns 0..n-1 (piv0 = 0) with one column more
+ * than the groups' registers hold. Column 0 (group 0, slot 0) is the first
+ * pivot: after the first step it is final, and its one nonzero, R(0,0) in
+ * row 0 (the lanes' first local; the rows below are what store_triu writes as
+ * zero), goes to r00. Its registers then take `extra`, the workgroup's last
+ * column (loaded by group 0 at the start), and the first reflector is applied
+ * to it there: its LDS buffer is rewritten only by the third step's publish,
+ * after a barrier this group has not reached yet. From the second step on the
+ * steps run as in house_qr. Every column sees the same operations in the same
+ * order as with one group more (house_update's order does not depend on CPL).
+ * save0 (LDS, NB entries, or null): receives column 0's block 0 after the
+ * first step, for a second QR over the same pivot columns.
+ */
+template <int NB, int SL, int CPL, class T, int QF>
+__device__ __forceinline__ void house_qr_swap(Col<NB, SL, T> (&cols)[CPL], const Col<NB, SL, T>& extra, T& r00,
+                                              Refl<NB, SL, T>& R, int grp, int sub, int n, unsigned upd,
+                                              T* save0 = nullptr) {
+  house_step<NB, SL, CPL, T, 0, QF>(cols, R, 0, sub, grp == 0 ? 0 : -1, upd, 0);
+  if (grp == 0) {
+    r00 = cols[0].a[0];
+    if (save0) { /* column 0's block 0 as it stands (a second QR over the same pivots starts from it) */
+#pragma unroll
+      for (int l = 0; l < Col<NB, SL, T>::H; ++l) save0[sub * Col<NB, SL, T>::H + l] = cols[0].a[l];
+    }
+    cols[0] = extra;
+    house_update<NB, SL, 1, T, 0, QF>(reinterpret_cast<Col<NB, SL, T>(&)[1]>(cols[0]), R, 0, sub, 1u);
+  }
+  house_qr_segs<NB, SL, CPL, T, QF | QF_SWAP>(cols, R, grp, sub, 0, n, upd, nullptr, nullptr, 0, 1);
+  __syncthreads();
+}
+
+/* Column 0 after house_qr_swap: R(0,0) in row 0 (block 0), zeros below. */
+template <int NB, int SL, class T>
+__device__ __forceinline__ Col<NB, SL, T> swap_col0(T r00, int sub) {
+  Col<NB, SL, T> c;
+#pragma unroll
+  for (int l = 0; l < 2 * Col<NB, SL, T>::H; ++l) c.a[l] = T(0);
+  c.a[0] = sub == 0 ? r00 : T(0);
+  return c;
 }
 
 template <int NB, int SL, int NT, class T>
@@ -561,11 +626,13 @@ __device__ __forceinline__ void refl_clear(Refl<NB, SL, T>& R, int tid) {
  * qr_half_rows rows (level 0 with m <= NB / 2; 16 in bucket 24), 2 upper
  * triangular (the levels above 0) */
 template <int NB, class T, int WG = 0, int CPL = 1, int LV = 0>
-__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::MINW))
+__global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::NT),
+                                  (QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::MINW))
     k_phase1(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, int n) {
-  using G = QRGeom<NB, 2 * NB + 1, 1, WG, CPL>;
+  using G = QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>;
   constexpr int SL = G::SL;
   constexpr int NT = G::NT;
+  constexpr bool SW = G::SWAP; /* R1 (the last column) swapped into column 0's registers */
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t e = blockIdx.x;
@@ -580,13 +647,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
   const bool lone = j + 1 == L;
   refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
   const int64_t sj = s0 + j, sp = s0 + j + 1;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
+  /* column c of the workgroup's matrix: loads it, returns whether it exists */
+  auto load = [&](Col<NB, SL, T>& col, int c) -> bool {
     const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
     const int qc = isP ? c : (isQ ? c - n : 0);
     if (isP) {
@@ -605,17 +668,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
       else col.load_block(0, in.c + sp * n, n, sub);
       col.load_block(1, in.o + sj * n, mr, sub);
     }
-    if (isP || isQ || isR1) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
-  }
-  /* LV 1: the stacked rows C_j, o_j fill at most qr_half_rows rows; LV 2: C_j is
-   * upper triangular (the pivot columns' block 1) */
-  house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp, sub, 0, n,
-                                                                                        upd);
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
+    return isP || isQ || isR1;
+  };
+  auto store = [&](const Col<NB, SL, T>& col, int c) {
     const bool isP = c < n, isQ = !lone && c >= n && c < 2 * n, isR1 = c == 2 * n;
     const int qc = isP ? c : (isQ ? c - n : 0);
     if (isP) {
@@ -633,15 +688,44 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT), (QRG
       col.store_block(0, fac.cf + e * n, n, sub);                       /* o_j */
       if (!lone) col.store_block(1, out.o + t * n, mr, sub);           /* c~_{j+1} */
     }
+  };
+
+  Col<NB, SL, T> cols[CPL];
+  Col<NB, SL, T> extra;
+  unsigned upd = 0;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    if (!(SW && c == 2 * n) && load(cols[q], c)) upd |= 1u << q;
+    else if (CPL > 1) cols[q].zero_block(0), cols[q].zero_block(1); /* computed on as zeros, never stored */
+  }
+  if (SW && grp == 0) load(extra, 2 * n);
+  /* LV 1: the stacked rows C_j, o_j fill at most qr_half_rows rows; LV 2: C_j is
+   * upper triangular (the pivot columns' block 1) */
+  constexpr int QF = QF_DONE | (LV == 1 ? QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0);
+  T r00 = T(0);
+  if constexpr (SW) house_qr_swap<NB, SL, CPL, T, QF>(cols, extra, r00, refl, grp, sub, n, upd);
+  else house_qr<NB, SL, CPL, T, QF>(cols, refl, grp, sub, 0, n, upd);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    if (SW && c == 0) {
+      store(cols[0], 2 * n);
+      store(swap_col0<NB, SL, T>(r00, sub), 0);
+    } else if (!(SW && c == 2 * n)) {
+      store(cols[q], c);
+    }
   }
 }
 
 template <int NB, class T, int WG = 0, int CPL = 1, bool DIAG = false>
-__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::MINW))
+__global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL, true>::NT),
+                                  (QRGeom<NB, 3 * NB + 1, 2, WG, CPL, true>::MINW))
     k_phase2(uk_level_t lv, ChainT<T> in, ChainT<T> out, FactorsT<T> fac, T* __restrict__ fix, int n) {
-  using G = QRGeom<NB, 3 * NB + 1, 2, WG, CPL>;
+  using G = QRGeom<NB, 3 * NB + 1, 2, WG, CPL, true>;
   constexpr int SL = G::SL;
   constexpr int NT = G::NT;
+  constexpr bool SW = G::SWAP; /* the last column of Bl swapped into column 0's registers */
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t e = blockIdx.x;
@@ -655,13 +739,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
   const bool lone = j + 1 == L;
   refl_clear<NB, SL, NT, T>(refl, tid);
 
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
   const int64_t sj = lv.step_start[p] + j;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
+  T* fz = fix + (int64_t)p * (nn + n);
+  auto load = [&](Col<NB, SL, T>& col, int c) -> bool {
     const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
     const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
     if (isP) {
@@ -682,17 +762,9 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
       else col.load_block(0, in.B + sj * nn + (int64_t)qc * n, n, sub);
       col.zero_block(1);
     }
-    if (isP || isQ || isR1 || isBl) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
-  }
-  /* block 1 of the pivot columns is R~_j, upper triangular */
-  /* DIAG: D_j is diagonal (the chain's d_diag) */
-  house_qr<NB, SL, CPL, T, QF_TRI1 | QF_DONE | (DIAG ? QF_DIAG0 : 0)>(cols, refl, grp, sub, 0, n, upd);
-  T* fz = fix + (int64_t)p * (nn + n);
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
+    return isP || isQ || isR1 || isBl;
+  };
+  auto store = [&](const Col<NB, SL, T>& col, int c) {
     const bool isP = c < n, isQ = c >= n && c < 2 * n, isR1 = c == 2 * n, isBl = c > 2 * n && c <= 3 * n;
     const int qc = isP ? c : (isQ ? c - n : (isBl ? c - 2 * n - 1 : 0));
     if (isP) col.store_triu(fac.R + e * nn + (int64_t)qc * n, n, qc, sub);
@@ -708,15 +780,44 @@ __global__ void __launch_bounds__((QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT), (QRG
       col.store_block(0, fac.cf + e * n, n, sub);
       col.store_block(1, lone ? fz + nn : out.c + t * n, n, sub); /* o_j */
     }
+  };
+
+  Col<NB, SL, T> cols[CPL];
+  Col<NB, SL, T> extra;
+  unsigned upd = 0;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    if (!(SW && c == 3 * n) && load(cols[q], c)) upd |= 1u << q;
+    else if (CPL > 1) cols[q].zero_block(0), cols[q].zero_block(1); /* computed on as zeros, never stored */
+  }
+  if (SW && grp == 0) load(extra, 3 * n);
+  /* block 1 of the pivot columns is R~_j, upper triangular */
+  /* DIAG: D_j is diagonal (the chain's d_diag) */
+  constexpr int QF = QF_TRI1 | QF_DONE | (DIAG ? QF_DIAG0 : 0);
+  T r00 = T(0);
+  if constexpr (SW) house_qr_swap<NB, SL, CPL, T, QF>(cols, extra, r00, refl, grp, sub, n, upd);
+  else house_qr<NB, SL, CPL, T, QF>(cols, refl, grp, sub, 0, n, upd);
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    if (SW && c == 0) {
+      store(cols[0], 3 * n);
+      store(swap_col0<NB, SL, T>(r00, sub), 0);
+    } else if (!(SW && c == 3 * n)) {
+      store(cols[q], c);
+    }
   }
 }
 
 template <int NB, class T, int WG = 0, int CPL = 1, int LV = 0>
-__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<NB, NB + 1, 3, WG, CPL>::MINW))
+__global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL, true>::NT),
+                                  (QRGeom<NB, NB + 1, 3, WG, CPL, true>::MINW))
     k_phase3(uk_level_t lv, ChainT<T> in, ChainT<T> out, const T* __restrict__ fix, int n) {
-  using G = QRGeom<NB, NB + 1, 3, WG, CPL>;
+  using G = QRGeom<NB, NB + 1, 3, WG, CPL, true>;
   constexpr int SL = G::SL;
   constexpr int NT = G::NT;
+  constexpr bool SW = G::SWAP; /* the right-hand side swapped into column 0's registers */
   __shared__ Refl<NB, SL, T> refl;
   const int tid = threadIdx.x, sub = tid & (SL - 1), grp = tid / SL;
   const int64_t unit = blockIdx.x;
@@ -732,13 +833,8 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
   refl_clear<NB, SL, NT, T>(refl, tid);
 
   /* Q columns numbered from 0 here */
-  Col<NB, SL, T> cols[CPL];
-  unsigned upd = 0;
   const int64_t sp = lv.step_start[p] + j + 1;
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    Col<NB, SL, T>& col = cols[q];
-    const int c = grp * CPL + q;
+  auto load = [&](Col<NB, SL, T>& col, int c) -> bool {
     const bool isQ = c < n, isR1 = c == n;
     const int qc = isQ ? c : 0;
     if (isQ) {
@@ -749,31 +845,59 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL>::NT), (QRGeom<
       col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
       col.load_block(1, in.o + sp * n, mr, sub);
     }
-    if (isQ || isR1) upd |= 1u << q;
-    else if (CPL > 1) col.zero_block(0), col.zero_block(1); /* computed on as zeros, never stored */
+    return isQ || isR1;
+  };
+  Col<NB, SL, T> cols[CPL];
+  Col<NB, SL, T> extra;
+  unsigned upd = 0;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = grp * CPL + q;
+    if (!(SW && c == n) && load(cols[q], c)) upd |= 1u << q;
+    else if (CPL > 1) cols[q].zero_block(0), cols[q].zero_block(1); /* computed on as zeros, never stored */
   }
+  if (SW && grp == 0) load(extra, n);
   /* LV 1: both blocks (D~_{j+1}, C_{j+1} and the right-hand sides) fill at
    * most qr_half_rows rows; LV 2: C_{j+1} (block 1 of the pivot columns) is upper
    * triangular */
-  house_qr<NB, SL, CPL, T, QF_DONE | (LV == 1 ? QF_HALF0 | QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0)>(cols, refl, grp,
-                                                                                                  sub, 0, n, upd);
+  constexpr int QF = QF_DONE | (LV == 1 ? QF_HALF0 | QF_HALF1 : 0) | (LV == 2 ? QF_TRI1 : 0);
+  T r00 = T(0);
+  __shared__ T p0blk[SW ? NB : 1]; /* column 0's block 0 for the fold */
+  if constexpr (SW) house_qr_swap<NB, SL, CPL, T, QF>(cols, extra, r00, refl, grp, sub, n, upd, do_fix ? p0blk : nullptr);
+  else house_qr<NB, SL, CPL, T, QF>(cols, refl, grp, sub, 0, n, upd);
   if (do_fix) {
+    /* [G~; Z] and [c; o_Z]: block 1 from the lone step's scratch */
     const T* fz = fix + (int64_t)p * (nn + n);
+    if constexpr (SW) {
+      if (grp == 0) { /* the right-hand side leaves column 0's registers for the second QR */
+        extra = cols[0];
+        extra.load_block(1, fz + nn, n, sub);
+#pragma unroll
+        for (int l = 0; l < Col<NB, SL, T>::H; ++l) cols[0].a[l] = p0blk[sub * Col<NB, SL, T>::H + l];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
       const int c = grp * CPL + q;
       const int qc = c < n ? c : 0;
       if (c < n) cols[q].load_block(1, fz + (int64_t)qc * n, n, sub);
-      if (c == n) cols[q].load_block(1, fz + nn, n, sub);
+      if (!SW && c == n) cols[q].load_block(1, fz + nn, n, sub);
     }
-    house_qr<NB, SL, CPL, T, QF_DONE>(cols, refl, grp, sub, 0, n, upd);
+    if constexpr (SW) house_qr_swap<NB, SL, CPL, T, QF_DONE>(cols, extra, r00, refl, grp, sub, n, upd);
+    else house_qr<NB, SL, CPL, T, QF_DONE>(cols, refl, grp, sub, 0, n, upd);
   }
 #pragma unroll
   for (int q = 0; q < CPL; ++q) {
     const int c = grp * CPL + q;
     const int qc = c < n ? c : 0;
-    if (c < n) cols[q].store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
-    if (c == n) cols[q].store_block(0, out.o + t * n, n, sub);
+    if (SW && c == 0) {
+      cols[0].store_block(0, out.o + t * n, n, sub);
+      swap_col0<NB, SL, T>(r00, sub).store_triu(out.C + t * nn, n, 0, sub);
+    } else if (c < n) {
+      cols[q].store_triu(out.C + t * nn + (int64_t)qc * n, n, qc, sub); /* G~ */
+    } else if (!SW && c == n) {
+      cols[q].store_block(0, out.o + t * n, n, sub);
+    }
   }
 }
 
@@ -1035,20 +1159,20 @@ __global__ void __launch_bounds__(WyGeom<NB>::NT, 1)
 template <int NB, class T, int WG, int CPL>
 void phases_nb(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T>& out, const FactorsT<T>& fac, T* fix, int n,
                hipStream_t st, int phase) {
-  const int lvk = in.c_tri ? 2 : (in.mr <= qr_half_rows(NB, QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::SL) ? 1 : 0);
-  static_assert(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::SL == QRGeom<NB, NB + 1, 3, WG, CPL>::SL,
+  const int lvk = in.c_tri ? 2 : (in.mr <= qr_half_rows(NB, QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::SL) ? 1 : 0);
+  static_assert(QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::SL == QRGeom<NB, NB + 1, 3, WG, CPL, true>::SL,
                 "phases 1 and 3 share the half-block row count");
   if (phase == 1) {
-    const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 2 * NB + 1, 1, WG, CPL>::NT);
+    const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::NT);
     if (lvk == 2) k_phase1<NB, T, WG, CPL, 2><<<g, b, 0, st>>>(*lv, in, out, fac, n);
     else if (lvk == 1) k_phase1<NB, T, WG, CPL, 1><<<g, b, 0, st>>>(*lv, in, out, fac, n);
     else k_phase1<NB, T, WG, CPL, 0><<<g, b, 0, st>>>(*lv, in, out, fac, n);
   } else if (phase == 2) {
-    const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 3 * NB + 1, 2, WG, CPL>::NT);
+    const dim3 g((unsigned)lv->evens), b(QRGeom<NB, 3 * NB + 1, 2, WG, CPL, true>::NT);
     if (in.d_diag) k_phase2<NB, T, WG, CPL, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
     else k_phase2<NB, T, WG, CPL, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
   } else {
-    const dim3 g((unsigned)lv->units), b(QRGeom<NB, NB + 1, 3, WG, CPL>::NT);
+    const dim3 g((unsigned)lv->units), b(QRGeom<NB, NB + 1, 3, WG, CPL, true>::NT);
     if (lvk == 2) k_phase3<NB, T, WG, CPL, 2><<<g, b, 0, st>>>(*lv, in, out, fix, n);
     else if (lvk == 1) k_phase3<NB, T, WG, CPL, 1><<<g, b, 0, st>>>(*lv, in, out, fix, n);
     else k_phase3<NB, T, WG, CPL, 0><<<g, b, 0, st>>>(*lv, in, out, fix, n);
