@@ -42,6 +42,7 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 NOCOV = 1 << 16  # KALMAN_NO_COVARIANCE
+COVDIAG = 1 << 25  # UKHIP_COVARIANCE_DIAG: the first level's back substitution forms only diag(S_j)
 
 WORKLOADS = {
     "n48m24_k200000": dict(n=48, m=24, k=200_000, unob10=True, desc="n=48 m=24 k=200000, every 10th step unobserved, "
@@ -602,6 +603,14 @@ def run_long(args):
            "kernels_tflops": {k: fl_of[k] / (kern[k] * 1e-3) / 1e12 for k in fl_of if kern[k] > 0},
            "algorithmic_flops_per_timestep": fl["total"] / pr.k}
     del S64
+    # the diagonal-only covariance path on the same resident batch
+    dms = _time(lambda: b.smooth("oddeven", options=COVDIAG, sync=False), args.steps, 1)
+    xd, dd = b.download(0, want_cov=False)[0], b.download_cov(0, "D")
+    out["diagonal_only"] = {"ms_per_step": dms, "value": pr.k / (dms * 1e-3), "unit": "timesteps/s",
+                            "speedup_vs_full_covariances": ms_per_step / dms,
+                            "states_bit_identical_to_full": bool(np.array_equal(xd, x64)),
+                            "max_rel_err_covdiag_vs_full": step_rel_err(dd, d64)}
+    say(f"n=48 diagonal-only covariances: {dms:.2f} ms/step, diagonals vs full {step_rel_err(dd, d64):.1e}")
     if not args.no_mixed:
         out["mixed_precision"] = mixed_legs(b, pr.k, [x64], [d64], args, "n=48")
     if not args.no_associative:

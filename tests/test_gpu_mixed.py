@@ -91,6 +91,39 @@ def test_mixed_batch_of_dimensions(dev, oracle):
         b.free()
 
 
+def test_sweep_segments(dev):
+    """The refinement sweeps give each step a segment of 16 lanes at n <= 16
+    (four steps per wave) and the whole wave above: every dimension class,
+    lengths whose levels have odd numbers of even steps and pairs (a wave's
+    spare segments), one-step problems. Bars: one refinement step within 1e-9 of the
+    FP64 estimates, two within 1e-11. A one-step problem is a single square
+    random G (its condition number reaches 1e3 at n = 29, and each step
+    multiplies the error by about cond^2 * 6e-8), so it is held to the second
+    bar only."""
+    probs = []
+    for n in (1, 3, 5, 8, 9, 12, 16, 17, 21, 24, 25, 29, 32):
+        F = 0.97 * P.random_orthogonal(P.rng_for(700 + n), n)
+        for k in (1, 2, 3, 7, 45, 131):
+            m = n if k <= 3 else max(1, n // 2)  # every state determined
+            probs.append(P.iid_problem(k, n, m, seed=710 + 10 * n + k, F=F, G_scale=1 / np.sqrt(n),
+                                       Cvar=0.1, unobserved=lambda i: i % 10 == 9, name=f"n{n}_k{k}"))
+    say(f"sweep segments: {len(probs)} problems")
+    b = dev.Batch(probs)
+    try:
+        b.smooth("oddeven")
+        ref = [b.download(i, want_cov=False)[0] for i in range(len(probs))]
+        errs = {}
+        for it in (1, 2):
+            b.smooth_mixed(it)
+            errs[it] = [max_rel_err(list(b.download(i, want_cov=False)[0]), list(ref[i])) for i in range(len(probs))]
+        e1 = max(e for e, pr in zip(errs[1], probs) if pr.k > 1)
+        e1_one = max(e for e, pr in zip(errs[1], probs) if pr.k == 1)
+        say(f"  one refinement: {e1:.2e} (one-step problems {e1_one:.2e}); two: {max(errs[2]):.2e}")
+        assert e1 < TOL and max(errs[2]) < 1e-11, (e1, max(errs[2]))
+    finally:
+        b.free()
+
+
 def test_few_observations_stay_finite(dev):
     """m << n (m = 1, 2, 4 with n = 5, 8, 14): the stacked blocks are nearly
     dependent, and the FP32 Householder reflectors must stay within FP32's

@@ -826,8 +826,11 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64)
  * Element-parallel over spw consecutive steps per workgroup: consecutive
  * threads touch consecutive doubles of every array.
  */
+/* w64 = 0 (F32 only): the FP64 chain is not written (its one reader, the
+ * refinement's k_normal_w, forms it from the raw inputs: from_raw). */
 template <bool F32>
-__global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out, uk_chain32_t out32, int spw) {
+__global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out, uk_chain32_t out32, int spw,
+                                                    int w64) {
   const int n = raw.n, mr = raw.mr, nn = n * n, mn = mr * n;
   const int64_t s0 = (int64_t)blockIdx.x * spw;
   const int cnt = (int)((raw.steps - s0) < spw ? (raw.steps - s0) : spw);
@@ -848,17 +851,17 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
         const double kw = raw.K[s * n + i];
         if (!out.b_synth) { /* else B is formed where it is read */
           const double bv = -kw * raw.F[raw.Fidx[s] * nn + e];
-          out.B[g] = bv;
+          if (w64) out.B[g] = bv;
           if (F32) out32.B[g] = (float)bv;
         }
         if (out.d_diag) { /* D = diag(k): its diagonal in the slot's first n entries (then c == 0, i == e) */
           if (e < n) {
-            out.D[g] = kw;
+            if (w64) out.D[g] = kw;
             if (F32) out32.D[g] = (float)kw;
           }
         } else {
           const double dv = raw.H_ident ? ((c == i) ? kw : 0.0) : kw * raw.H[g];
-          out.D[g] = dv;
+          if (w64) out.D[g] = dv;
           if (F32) out32.D[g] = (float)dv;
         }
       }
@@ -876,7 +879,7 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
     for (int idx = tid; idx < cnt * mn; idx += 256) {
       const int64_t s = s0 + ls;
       const double v = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.G[s * mn + e] : 0.0;
-      out.C[s * nn + e] = v;
+      if (w64) out.C[s * nn + e] = v;
       if (F32) out32.C[s * nn + e] = (float)v;
       ls += dls;
       e += de;
@@ -890,11 +893,11 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
     if (!raw.first[s]) {
       const double kw = raw.K[s0 * n + idx];
       const double v = kw * raw.c[s0 * n + idx];
-      out.c[s0 * n + idx] = v;
+      if (w64) out.c[s0 * n + idx] = v;
       if (F32) out32.c[s0 * n + idx] = (float)v;
       if (d_only) { /* D = diag(k): the slot's first n entries */
         const int64_t g = s * nn + (idx - (s - s0) * n);
-        out.D[g] = kw;
+        if (w64) out.D[g] = kw;
         if (F32) out32.D[g] = (float)kw;
       }
     }
@@ -903,7 +906,7 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
     const int ls = idx / mr, i = idx - ls * mr;
     const int64_t s = s0 + ls;
     const double v = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.o[s * mr + i] : 0.0;
-    out.o[s * n + i] = v;
+    if (w64) out.o[s * n + i] = v;
     if (F32) out32.o[s * n + i] = (float)v;
   }
 }
@@ -1262,7 +1265,7 @@ extern "C" hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out
   if (!raw->K_full && !raw->C_full) { /* every covariance given as 'w' */
     const int spw = weigh_diag_spw(raw, out);
     const int64_t blocks = (raw->steps + spw - 1) / spw;
-    k_weigh_diag<false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, uk_chain32_t{}, spw);
+    k_weigh_diag<false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, uk_chain32_t{}, spw, 1);
     return hipGetLastError();
   }
   const int nb = uk_bucket(raw->n > raw->mr ? raw->n : raw->mr);
@@ -1308,15 +1311,16 @@ extern "C" hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t*
   return hipGetLastError();
 }
 
-/* The level-0 chain in FP64 and, written by the same pass, its FP32 copy
- * (the mixed-precision path) when every K and C is given as 'w'; returns
- * hipErrorNotSupported otherwise (then the FP64 chain is converted afterwards). */
+/* The level-0 chain's FP32 copy (the mixed-precision path) and, written by the
+ * same pass when w64 is set, the FP64 chain, when every K and C is given as
+ * 'w'; returns hipErrorNotSupported otherwise (then the FP64 chain is
+ * converted afterwards). */
 extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32,
-                                          hipStream_t st) {
+                                          int w64, hipStream_t st) {
   if (raw->K_full || raw->C_full) return hipErrorNotSupported;
   const int spw = weigh_diag_spw(raw, out);
   const int64_t blocks = (raw->steps + spw - 1) / spw;
-  k_weigh_diag<true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, *out32, spw);
+  k_weigh_diag<true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, *out32, spw, w64);
   return hipGetLastError();
 }
 

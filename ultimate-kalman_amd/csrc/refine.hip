@@ -42,6 +42,57 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+/* The sweeps at NB <= 32 give each step a segment of S lanes (S = 16 or 32,
+ * 64 / S steps per wave). seg_bcast<S>(v, q): lane q of each segment,
+ * broadcast to that segment (q folds to a constant in the unrolled loops):
+ * S = 32 reads both halves' lane q and each half keeps its own; S = 16 is the
+ * DPP row broadcast (row_newbcast, a row being 16 lanes). */
+template <int Q>
+__device__ __forceinline__ int row_newbcast(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + Q, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_bcast16(int v, int q) {
+  switch (q & 15) {
+    case 0: return row_newbcast<0>(v);
+    case 1: return row_newbcast<1>(v);
+    case 2: return row_newbcast<2>(v);
+    case 3: return row_newbcast<3>(v);
+    case 4: return row_newbcast<4>(v);
+    case 5: return row_newbcast<5>(v);
+    case 6: return row_newbcast<6>(v);
+    case 7: return row_newbcast<7>(v);
+    case 8: return row_newbcast<8>(v);
+    case 9: return row_newbcast<9>(v);
+    case 10: return row_newbcast<10>(v);
+    case 11: return row_newbcast<11>(v);
+    case 12: return row_newbcast<12>(v);
+    case 13: return row_newbcast<13>(v);
+    case 14: return row_newbcast<14>(v);
+    default: return row_newbcast<15>(v);
+  }
+}
+template <int S>
+__device__ __forceinline__ double seg_bcast(double v, int q) {
+  if constexpr (S == 64) {
+    return lane_bcast(v, q);
+  } else if constexpr (S == 32) {
+    const double a = lane_bcast(v, q), b = lane_bcast(v, 32 + q);
+    return (threadIdx.x & 32) ? b : a;
+  } else {
+    static_assert(S == 16, "segments of 16, 32 or 64 lanes");
+    return __hiloint2double(row_bcast16(__double2hiint(v), q), row_bcast16(__double2loint(v), q));
+  }
+}
+/* lanes per step of the sweeps in bucket NB: 16 up to NB = 16 (four steps per
+ * wave), else the whole wave. Segments of 32 lanes (two steps per wave at
+ * NB = 24 and 32) were measured slower on MI355X: each broadcast of the
+ * substitutions' serial chain becomes two readlane pairs and a select, and
+ * the doubled column loads per wave cut the resident waves. */
+#ifndef UK_SWEEP_SEG32
+#define UK_SWEEP_SEG32 0
+#endif
+__host__ __device__ constexpr int sweep_seg(int NB) { return NB <= 16 ? 16 : (NB <= 32 && UK_SWEEP_SEG32 ? 32 : 64); }
+
 __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -51,6 +102,17 @@ __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob,
   }
   return lo;
 }
+/* The problem of step x = x0 + d (0 <= d < G) when x0 is the same for the
+ * whole wave: the binary search runs once, on the scalar unit, and each
+ * segment walks at most G - 1 problems on (every problem has a step). */
+template <int G>
+__device__ __forceinline__ int find_problem_seg(const int64_t* start, int nprob, int64_t x0, int64_t x) {
+  int p = (int)find_problem(start, nprob, x0);
+#pragma unroll
+  for (int t = 0; t < G - 1; ++t)
+    if (p + 1 < nprob && start[p + 1] <= x) ++p;
+  return p;
+}
 
 /* Row and column of element idx = lane + 64 t (t = 0, 1, ...) of a
  * column-major block with `rows` rows: one division per lane, then 64
@@ -58,9 +120,10 @@ __device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob,
  * a long instruction sequence). */
 struct Walk64 {
   int r, c, dr, dc, rows;
-  __device__ __forceinline__ Walk64(int lane, int rows_) : rows(rows_) {
-    dc = 64 / rows;
-    dr = 64 - dc * rows;
+  /* stride: elements per step (64, or a segment's lane count) */
+  __device__ __forceinline__ Walk64(int lane, int rows_, int stride = 64) : rows(rows_) {
+    dc = stride / rows;
+    dr = stride - dc * rows;
     c = lane / rows;
     r = lane - c * rows;
   }
@@ -252,10 +315,21 @@ __global__ void __launch_bounds__(256) k_normal_w(uk_chain_t ch, const int8_t* f
   /* every element of the step's blocks in flight at once (indices clamped
    * into the step's slots; values outside a block are never stored) */
   double tcv[PS], tbv[PS], tdv[DD ? 1 : PS];
+  if (ch.from_raw) { /* C = diag(w) G formed here, the product k_weigh_diag forms */
+    Walk64 wr(lane, mr);
 #pragma unroll
-  for (int t = 0; t < PS; ++t) {
-    const int idx = lane + 64 * t;
-    tcv[t] = ch.C[s * nn + (idx < cn ? idx : 0)];
+    for (int t = 0; t < PS; ++t) {
+      const int idx = lane + 64 * t, ic = idx < cn ? idx : 0, r = idx < cn ? wr.r : 0;
+      const double cw = ch.rC[s * mr + r], gv = ch.rG[s * cn + ic];
+      tcv[t] = r < m ? cw * gv : 0.0;
+      wr.next();
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < PS; ++t) {
+      const int idx = lane + 64 * t;
+      tcv[t] = ch.C[s * nn + (idx < cn ? idx : 0)];
+    }
   }
   if (evo) {
     if (ch.b_synth) {
@@ -275,7 +349,7 @@ __global__ void __launch_bounds__(256) k_normal_w(uk_chain_t ch, const int8_t* f
       }
     }
     if (DD) {
-      if (lane < n) dd[lane] = ch.D[s * nn + lane];
+      if (lane < n) dd[lane] = ch.from_raw ? ch.wk[s * n + lane] : ch.D[s * nn + lane];
     } else {
 #pragma unroll
       for (int t = 0; t < (DD ? 1 : PS); ++t) {
@@ -323,7 +397,7 @@ __global__ void __launch_bounds__(256) k_normal_w(uk_chain_t ch, const int8_t* f
     }
     p += q;
     p += __shfl_xor(p, 32);
-    if (h == 0 && i < n) re[i] = ch.c[s * n + i] - p;
+    if (h == 0 && i < n) re[i] = (ch.from_raw ? ch.wk[s * n + i] * ch.rc[s * n + i] : ch.c[s * n + i]) - p;
     wave_sync();
     /* gA = D^T re, gB = B^T re, column i */
     if (DD && h == 0) ga = dd[ir] * re[ir];
@@ -352,7 +426,8 @@ __global__ void __launch_bounds__(256) k_normal_w(uk_chain_t ch, const int8_t* f
       p = jj < cn_h ? fma(M[irc + j * L], xs[j], p) : p;
     }
     p += __shfl_xor(p, 32);
-    if (h == 0 && i < mr) ro[i] = i < m ? ch.o[s * n + i] - p : 0.0;
+    if (h == 0 && i < mr)
+      ro[i] = i < m ? (ch.from_raw ? ch.rC[s * mr + i] * ch.ro[s * mr + i] : ch.o[s * n + i]) - p : 0.0;
     wave_sync();
     /* gA += C^T ro, column i */
 #pragma unroll
@@ -381,55 +456,60 @@ __global__ void __launch_bounds__(256) k_gather(const double* __restrict__ gA, c
 }
 
 /*
- * Forward sweep, even steps of a level (one wave per even step):
- * y_j = R_j^{-T} g_j. R is staged through LDS in its own precision; lane i
- * holds entry i, the solved y_k reach every lane by readlane, and the pivots'
- * reciprocals are formed in parallel before the substitution.
+ * Forward sweep, even steps of a level: y_j = R_j^{-T} g_j, one segment of
+ * S = sweep_seg(NB) lanes per even step (64 / S steps per wave). R is staged
+ * through LDS in its own precision; lane i of a segment holds entry i, the
+ * solved y_k reach the segment by seg_bcast, and the pivots' reciprocals are
+ * formed in parallel before the substitution.
  */
 template <int NB, class T>
 __global__ void __launch_bounds__(64) k_fwd_even(uk_level_t lv, const T* __restrict__ R, const double* __restrict__ g,
                                                  double* __restrict__ y, int n) {
-  constexpr int L = NB + 1;
-  __shared__ T Rs[NB * L];
-  const int lane = threadIdx.x;
-  const int64_t e = blockIdx.x;
-  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  constexpr int L = NB + 1, S = sweep_seg(NB), G = 64 / S;
+  __shared__ T Rs[G][NB * L];
+  const int lane = threadIdx.x, seg = S == 64 ? 0 : lane / S, i = lane - seg * S; /* seg uniform at S = 64 */
+  const int64_t e0 = (int64_t)blockIdx.x * G + seg;
+  const bool live = e0 < lv.evens; /* a spare segment repeats the last step and stores nothing */
+  const int64_t e = live ? e0 : lv.evens - 1;
+  const int p = find_problem_seg<G>(lv.even_start, lv.nprob, (int64_t)blockIdx.x * G, e);
   const int64_t j = 2 * (e - lv.even_start[p]);
   const int64_t sj = lv.step_start[p] + j;
   const int64_t nn = (int64_t)n * n;
+  T* Rw = Rs[seg];
   {
-    constexpr int PER = (NB * NB + 63) / 64;
+    constexpr int PER = (NB * NB + S - 1) / S;
     T rr[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      const int idx = lane + 64 * t;
+      const int idx = i + S * t;
       rr[t] = idx < n * n ? R[e * nn + idx] : T(0);
     }
-    Walk64 wr(lane, n);
+    Walk64 wr(i, n, S);
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      if (lane + 64 * t < n * n) Rs[wr.r + wr.c * L] = rr[t];
+      if (i + S * t < n * n) Rw[wr.r + wr.c * L] = rr[t];
       wr.next();
     }
   }
-  double v = lane < n ? g[sj * n + lane] : 0.0;
+  double v = i < n ? g[sj * n + i] : 0.0;
   __syncthreads();
-  const double rinv = lane < n ? 1.0 / (double)Rs[lane + lane * L] : 0.0;
+  const double rinv = i < n ? 1.0 / (double)Rw[i + i * L] : 0.0;
   /* R^T is lower triangular: (R^T)_{ik} = R_{ki} */
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     if (k < n) {
-      const double yk = lane_bcast(v * rinv, k);
-      if (lane == k) v = yk;
-      else if (lane > k && lane < n) v = fma(-(double)Rs[k + lane * L], yk, v);
+      const double yk = seg_bcast<S>(v * rinv, k);
+      if (i == k) v = yk;
+      else if (i > k && i < n) v = fma(-(double)Rw[k + i * L], yk, v);
     }
   }
-  if (lane < n) y[e * n + lane] = v;
+  if (i < n && live) y[e * n + i] = v;
 }
 
 /*
- * Forward sweep, odd steps (one wave per pair u, odd step j + 1 = 2u + 1):
- * the next level's right-hand side  g'_{j+1} = g_{j+1} - Y_j^T y_j - B~_{j+2}^T y_{j+2}.
+ * Forward sweep, odd steps (one segment of S lanes per pair u, odd step
+ * j + 1 = 2u + 1): the next level's right-hand side
+ *   g'_{j+1} = g_{j+1} - Y_j^T y_j - B~_{j+2}^T y_{j+2}.
  * Y_j and B~_{j+2} are staged through LDS with coalesced loads (all in
  * flight together); lane i then forms entry i from column i.
  */
@@ -437,15 +517,17 @@ template <int NB, class T>
 __global__ void __launch_bounds__(64) k_fwd_odd(uk_level_t lv, const T* __restrict__ Bt, const T* __restrict__ Y,
                                                 const double* __restrict__ g, const double* __restrict__ y,
                                                 double* __restrict__ gnext, int n) {
-  constexpr int L = NB + 1;
-  constexpr int PER = (NB * NB + 63) / 64;
-  __shared__ T Ys[NB * L], Bs[NB * L];
-  __shared__ double yj[NB], yn[NB];
-  const int lane = threadIdx.x;
-  const int64_t unit = blockIdx.x;
-  const int p = (int)find_problem(lv.unit_start, lv.nprob, unit);
+  constexpr int L = NB + 1, S = sweep_seg(NB), G = 64 / S;
+  constexpr int PER = (NB * NB + S - 1) / S;
+  __shared__ T Ys[G][NB * L], Bs[G][NB * L];
+  __shared__ double yj[G][NB], yn[G][NB];
+  const int lane = threadIdx.x, seg = S == 64 ? 0 : lane / S, i = lane - seg * S; /* seg uniform at S = 64 */
+  const int64_t unit0 = (int64_t)blockIdx.x * G + seg;
+  const int64_t unit = unit0 < lv.units ? unit0 : lv.units - 1;
+  const int p = find_problem_seg<G>(lv.unit_start, lv.nprob, (int64_t)blockIdx.x * G, unit);
   const int64_t Lp = lv.L[p];
-  if (Lp == 1) return;
+  /* a one-step problem has no odd step; a spare segment repeats the last pair */
+  const bool act = unit0 < lv.units && Lp > 1;
   const int64_t u = unit - lv.unit_start[p];
   const int64_t j = 2 * u;
   const int64_t e = lv.even_start[p] + u; /* slot of even step j */
@@ -454,49 +536,53 @@ __global__ void __launch_bounds__(64) k_fwd_odd(uk_level_t lv, const T* __restri
   T ry[PER], rb[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int idx = lane + 64 * t;
+    const int idx = i + S * t;
     const bool in = idx < n * n;
     ry[t] = in ? Y[e * nn + idx] : T(0);
     rb[t] = (in && has_n) ? Bt[(e + 1) * nn + idx] : T(0);
   }
-  if (lane < n) {
-    yj[lane] = y[e * n + lane];
-    yn[lane] = has_n ? y[(e + 1) * n + lane] : 0.0;
+  if (i < n) {
+    yj[seg][i] = y[e * n + i];
+    yn[seg][i] = has_n ? y[(e + 1) * n + i] : 0.0;
   }
-  Walk64 wy(lane, n);
+  Walk64 wy(i, n, S);
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    if (lane + 64 * t < n * n) {
-      Ys[wy.r + wy.c * L] = ry[t];
-      Bs[wy.r + wy.c * L] = rb[t];
+    if (i + S * t < n * n) {
+      Ys[seg][wy.r + wy.c * L] = ry[t];
+      Bs[seg][wy.r + wy.c * L] = rb[t];
     }
     wy.next();
   }
   __syncthreads();
-  if (lane < n) {
-    double v0 = g[(lv.step_start[p] + j + 1) * n + lane], v1 = 0.0;
+  if (i < n && act) {
+    double v0 = g[(lv.step_start[p] + j + 1) * n + i], v1 = 0.0;
     for (int r = 0; r < n; ++r) {
-      v0 = fma(-(double)Ys[r + lane * L], yj[r], v0);
-      v1 = fma(-(double)Bs[r + lane * L], yn[r], v1);
+      v0 = fma(-(double)Ys[seg][r + i * L], yj[seg][r], v0);
+      v1 = fma(-(double)Bs[seg][r + i * L], yn[seg][r], v1);
     }
-    gnext[(lv.next_start[p] + u) * n + lane] = v0 + v1;
+    gnext[(lv.next_start[p] + u) * n + i] = v0 + v1;
   }
 }
 
 /*
- * Backward sweep (one wave per even step): x_j = R^{-1} (y_j - B~ x_{j-1} - Y x_{j+1})
- * with the neighbours' x from the levels above, in the original step order.
- * Lane i owns row i: B~, Y and R stream column by column (coalesced, no
- * LDS), lane i keeps row i of R in registers, and the neighbours' estimates
- * and each solved x_q reach every lane by readlane.
+ * Backward sweep (one segment of S = sweep_seg(NB) lanes per even step):
+ * x_j = R^{-1} (y_j - B~ x_{j-1} - Y x_{j+1}) with the neighbours' x from the
+ * levels above, in the original step order. Lane i of a segment owns row i:
+ * B~, Y and R stream column by column (coalesced, no LDS), lane i keeps row i
+ * of R in registers, and the neighbours' estimates and each solved x_q reach
+ * the segment by seg_bcast.
  */
 template <int NB, class T, class TR>
 __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> fac, const TR* __restrict__ rhs,
                                                   double* __restrict__ x, int n) {
-  const int i = threadIdx.x;
+  constexpr int S = sweep_seg(NB), G = 64 / S;
+  const int lane = threadIdx.x, seg = S == 64 ? 0 : lane / S, i = lane - seg * S; /* seg uniform at S = 64 */
   const bool row = i < n;
-  const int64_t e = blockIdx.x;
-  const int p = (int)find_problem(lv.even_start, lv.nprob, e);
+  const int64_t e0 = (int64_t)blockIdx.x * G + seg;
+  const bool live = e0 < lv.evens; /* a spare segment repeats the last step and stores nothing */
+  const int64_t e = live ? e0 : lv.evens - 1;
+  const int p = find_problem_seg<G>(lv.even_start, lv.nprob, (int64_t)blockIdx.x * G, e);
   const int64_t j = 2 * (e - lv.even_start[p]);
   const int64_t Lp = lv.L[p];
   const bool has_m = j > 0, has_p = j + 1 < Lp;
@@ -528,19 +614,19 @@ __global__ void __launch_bounds__(64) k_solve_est(uk_level_t lv, FactorPtrs<T> f
     rrow[q] = (row && col) ? rv : T(0);
     const double b = (lm && col) ? (double)bv : 0.0;
     const double yv = (lp && col) ? (double)yw : 0.0;
-    v = fma(-b, lane_bcast(xm, q), v);
-    v1 = fma(-yv, lane_bcast(xp, q), v1);
+    v = fma(-b, seg_bcast<S>(xm, q), v);
+    v1 = fma(-yv, seg_bcast<S>(xp, q), v1);
   }
   v += v1;
   const double rinv = row ? 1.0 / (double)Rg[i + (int64_t)i * n] : 0.0;
 #pragma unroll
   for (int q = NB - 1; q >= 0; --q) {
     if (q < n) {
-      const double xq = lane_bcast(v * rinv, q);
+      const double xq = seg_bcast<S>(v * rinv, q);
       v = (i == q) ? xq : (i < q ? fma(-(double)rrow[q], xq, v) : v);
     }
   }
-  if (row) x[oj * n + i] = v;
+  if (row && live) x[oj * n + i] = v;
 }
 
 }  // namespace
@@ -553,6 +639,12 @@ extern "C" int uk_bucket_qr(int n);
 
 #define UK_DISPATCH(NBV, CALL) \
   case NBV: { CALL(NBV); } break;
+
+/* workgroups of a sweep over `count` steps in bucket nb (64 / sweep_seg steps each) */
+static unsigned sweep_grid(int64_t count, int nb) {
+  const int64_t g = 64 / sweep_seg(nb);
+  return (unsigned)((count + g - 1) / g);
+}
 
 extern "C" hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st) {
   if (count <= 0) return hipSuccess;
@@ -607,8 +699,8 @@ static hipError_t launch_fwd(const uk_level_t* lv, const T* R, const T* Bt, cons
                              double* gnext, int n, hipStream_t st) {
   const int nb = uk_bucket_qr(n);
 #define L_(NBV)                                                                                   \
-  k_fwd_even<NBV, T><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, R, g, y, n);            \
-  k_fwd_odd<NBV, T><<<dim3((unsigned)lv->units), dim3(64), 0, st>>>(*lv, Bt, Y, g, y, gnext, n)
+  k_fwd_even<NBV, T><<<dim3(sweep_grid(lv->evens, NBV)), dim3(64), 0, st>>>(*lv, R, g, y, n);            \
+  k_fwd_odd<NBV, T><<<dim3(sweep_grid(lv->units, NBV)), dim3(64), 0, st>>>(*lv, Bt, Y, g, y, gnext, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(24, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_)
     UK_DISPATCH(64, L_)
@@ -634,8 +726,8 @@ static hipError_t launch_solve_est(const uk_level_t* lv, const FactorPtrs<T>& f,
                                    const double* rhs64, double* x, int n, hipStream_t st) {
   const int nb = uk_bucket_qr(n);
 #define L_(NBV)                                                                                                \
-  if (rhs32) k_solve_est<NBV, T, float><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
-  else k_solve_est<NBV, T, double><<<dim3((unsigned)lv->evens), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
+  if (rhs32) k_solve_est<NBV, T, float><<<dim3(sweep_grid(lv->evens, NBV)), dim3(64), 0, st>>>(*lv, f, rhs32, x, n); \
+  else k_solve_est<NBV, T, double><<<dim3(sweep_grid(lv->evens, NBV)), dim3(64), 0, st>>>(*lv, f, rhs64, x, n)
   switch (nb) {
     UK_DISPATCH(8, L_) UK_DISPATCH(16, L_) UK_DISPATCH(24, L_) UK_DISPATCH(32, L_) UK_DISPATCH(48, L_)
     UK_DISPATCH(64, L_)

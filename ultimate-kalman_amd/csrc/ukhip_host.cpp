@@ -31,6 +31,7 @@
 
 extern "C" {
 int uk_bucket(int n);
+int uk_bucket_qr(int n);
 hipError_t uk_launch_weigh(const uk_raw_t* raw, const uk_chain_t* out, hipStream_t st);
 hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t* out, double* wobs, hipStream_t st);
 hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
@@ -49,7 +50,8 @@ void uk_set_error(const char* fmt, ...);
 hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                  const uk_factors32_t* fac, float* fix, int n, hipStream_t st, hipEvent_t* ev);
 hipError_t uk_launch_to32(float* dst, const double* src, int64_t count, hipStream_t st);
-hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32, hipStream_t st);
+hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32, int w64,
+                               hipStream_t st);
 hipError_t uk_launch_add(double* x, const double* d, int64_t count, hipStream_t st);
 hipError_t uk_launch_normal(const uk_chain_t* ch, const int8_t* first, const int32_t* m_obs, const double* x,
                             double* gA, double* gB, double* g, int64_t steps, int n, hipStream_t st);
@@ -1557,8 +1559,13 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
   const uk_chain_t w0 = level0_chain(b, b->chain[0]); /* its flags imply the fused path below */
+  /* With diagonal whitening and n, m <= 32 the refinement's k_normal_w forms the
+   * FP64 level-0 chain from the raw inputs, so the whitening writes only the
+   * FP32 copy. */
+  const bool from_raw = w0.d_diag && w0.b_synth && uk_bucket_qr(std::max(n, b->mr0)) <= 32;
   tb();
-  const hipError_t fused = b->compress ? hipErrorNotSupported : uk_launch_weigh_f32(&b->raw, &w0, &c32, st);
+  const hipError_t fused =
+      b->compress ? hipErrorNotSupported : uk_launch_weigh_f32(&b->raw, &w0, &c32, from_raw ? 0 : 1, st);
   if (fused != hipErrorNotSupported) {
     if (!hip_ok(fused, "k_weigh launch")) return false;
     te(UKHIP_OE_WEIGH, 1);
@@ -1628,7 +1635,12 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
     }
     te(UKHIP_OE_SOLVE, 1);
   }
-  const uk_chain_t r64 = level0_chain(b, b->chain[0]);
+  uk_chain_t r64 = level0_chain(b, b->chain[0]);
+  r64.from_raw = from_raw ? 1 : 0;
+  r64.rG = b->raw.G;
+  r64.rC = b->raw.C;
+  r64.ro = b->raw.o;
+  r64.rc = b->raw.c;
   for (int it = 0; it < refinements; ++it) {
     tb();
     if (!hip_ok(uk_launch_normal(&r64, b->raw.first, b->m0, b->xout, b->gA, b->gB, b->gbuf[0], (int64_t)S, n, st),

@@ -60,6 +60,16 @@ typedef struct {
   const double* wk;
   const double* wF;
   const int64_t* wFidx;
+  /* from_raw = 1 (read by k_normal_w; requires d_diag and b_synth): C, o, c
+   * and D's diagonal are not read from the chain but formed on load from the
+   * raw inputs as k_weigh_diag forms them: C = diag(w) G (rows >= m_s zero),
+   * o = w o, c = k c, diag(D) = k, with w = rC, k = wk (the mixed-precision
+   * path, whose whitening then writes only the FP32 chain) */
+  int32_t from_raw;
+  const double* rG;
+  const double* rC;
+  const double* ro;
+  const double* rc;
 } uk_chain_t;
 
 /* The same chain in single precision (the FP32 elimination). */
