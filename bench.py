@@ -849,10 +849,14 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the long workload's timed region, copy MIB MiB device-to-device once (a plain copy "
                          "kernel of known size for calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="profiling aid: launch kernel by kernel everywhere (no HIP graph capture or replay)")
     ap.add_argument("--mixed-chunk-only", type=int, default=0, metavar="N",
                     help="profiling aid: the first N problems of the batching workload as one device batch, two eager "
                          "mixed smoothings (graphs off; --copy-calibration runs between them), no bench line")
     args = ap.parse_args()
+    if args.no_graphs:
+        device.lib().ukhip_use_graphs(0)
     if args.mixed_chunk_only:
         probs = [P.batch_problem(sp, lists=False) for sp in P.batch_specs(count=args.mixed_chunk_only)]
         b = device.Batch(probs)
@@ -888,7 +892,8 @@ def main():
            "config": {"workload": "batching: " + BATCH_DESC,
                       "algorithm": f"odd-even, FP32 elimination + {args.refine} FP64 refinement step(s) of the "
                                    "estimates, batched (one launch per level per state dimension, dimensions on "
-                                   "forked streams, one HIP graph replay per chunk)"}}
+                                   + ("forked streams, kernel by kernel (--no-graphs))" if args.no_graphs else
+                                      "forked streams, one HIP graph replay per chunk)")}}
     if not args.no_headline:
         h = run_headline(args)
         out["value"] = h["value"]
