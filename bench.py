@@ -131,7 +131,7 @@ def kernel_label(slot, n, m, fp32=False, cov=True, all_w=True, h_ident=True, lev
         cpl, wg = int(v[1]), int(v[3])
     else:
         cpl, wg = 1, 0
-    if cpl == 3:
+    if cpl == 0:
         return f"k_phase2_wy<{nq}, {T}, *>"
     return f"{slot}<{nq}, {T}, {wg}, {cpl}, *>"
 
@@ -288,11 +288,35 @@ def chunked(specs, bytes_per_step, budget):
     return chunks
 
 
-def mixed_pass(specs, args):
+def accumulate_parts(b, ch, R, kern):
+    """Per-kernel device time of one eager mixed smoothing of batch b (chunk
+    specs ch; parts one after another, HIP events around every launch),
+    added into kern = (ms, launches, flops, bytes) dicts keyed by kernel label."""
+    slot_ms, slot_launch, slot_flops, slot_bytes = kern
+    for n_, ms, nl in b.timed_parts(mixed=(R, 0, False)):
+        m_ = max(n_ // 2, 1)
+        kpart = sum(sp["k"] for sp in ch if sp["n"] == n_)
+        fps = oddeven_flops_per_step(n_, m_, False)
+        for slot in device.OE_KERNELS:
+            if slot == "total" or ms[slot] <= 0:
+                continue
+            # the phases of levels 0-7 run in FP32 (the few steps of the levels above in FP64)
+            key = kernel_label(slot, n_, m_, fp32=True, cov=False)
+            slot_ms[key] = slot_ms.get(key, 0.0) + ms[slot]
+            slot_launch[key] = slot_launch.get(key, 0) + nl[slot]
+            if slot.startswith("k_phase"):
+                slot_flops[key] = slot_flops.get(key, 0.0) + fps["phase" + slot[-1]] * kpart
+            rb = refine_bytes_per_step(n_, m_, slot)
+            if rb is not None:  # k_solve_est also forms the first estimates: R + 1 sweeps
+                slot_bytes[key] = slot_bytes.get(key, 0.0) + rb * kpart * (R + (slot == "k_solve_est"))
+
+
+def mixed_pass(specs, args, kern):
     """The headline's timed leg: the mixed-precision path (estimates, R FP64
     refinement steps) over the whole workload in chunks sized for its own
     device memory; returns (ms per smoothing summed over chunks, per-chunk
-    records)."""
+    records). The per-kernel times of one eager smoothing per chunk go into
+    kern (accumulate_parts): the roofline's kernel ran on these chunks."""
     total, rec = 0.0, []
     for ci, ch in enumerate(chunked(specs, mixed_bytes_per_step, args.mixed_hbm_budget_gb * 1e9)):
         t = time.perf_counter()
@@ -300,6 +324,7 @@ def mixed_pass(specs, args):
         tg = time.perf_counter() - t
         b = device.Batch(probs)
         ms = _time(lambda: b.smooth_mixed_opt(args.refine, 0, False, sync=False), args.steps, max(args.warmup, 1))
+        accumulate_parts(b, ch, args.refine, kern)
         b.free()
         del probs
         ksum = sum(sp["k"] for sp in ch)
@@ -338,14 +363,14 @@ def run_headline(args):
     chunks = chunked(specs, device_bytes_per_step, args.hbm_budget_gb * 1e9)
     total_steps = sum(sp["k"] for sp in specs)
     R = args.refine
-    mixed_ms, mixed_chunks = mixed_pass(specs, args)
+    slot_ms, slot_launch, slot_flops, slot_bytes = {}, {}, {}, {}
+    mixed_ms, mixed_chunks = mixed_pass(specs, args, (slot_ms, slot_launch, slot_flops, slot_bytes))
     acc = {"mixed": 0.0, "fp64_states_only": 0.0, "fp64_with_covariances": 0.0, "plain_fp64_states_only": 0.0,
            "mixed_with_covariances": 0.0}
     err_mixed, err_plain, err_mixcov = 0.0, 0.0, 0.0
-    slot_ms, slot_launch, slot_flops, slot_bytes = {}, {}, {}, {}
     per_chunk = []
     oracle_sample = []
-    for ci, ch in enumerate(chunks):
+    for ci, ch in enumerate([] if args.headline_mixed_only else chunks):
         t = time.perf_counter()
         probs = [P.batch_problem(sp, lists=False) for sp in ch]
         tg = time.perf_counter() - t
@@ -365,23 +390,6 @@ def run_headline(args):
         d64 = [b.download_cov(i, "D") for i in range(min(8, len(probs)))]
         b.smooth_mixed_opt(R, 0, True)
         err_mixcov = max(err_mixcov, max(step_rel_err(b.download_cov(i, "D"), d64[i]) for i in range(len(d64))))
-        # per-kernel device time of one eager mixed smoothing, parts one after another
-        for n_, ms, nl in b.timed_parts(mixed=(R, 0, False)):
-            m_ = max(n_ // 2, 1)
-            kpart = sum(sp["k"] for sp in ch if sp["n"] == n_)
-            fps = oddeven_flops_per_step(n_, m_, False)
-            for slot in device.OE_KERNELS:
-                if slot == "total" or ms[slot] <= 0:
-                    continue
-                # the phases of levels 0-7 run in FP32 (the few steps of the levels above in FP64)
-                key = kernel_label(slot, n_, m_, fp32=True, cov=False)
-                slot_ms[key] = slot_ms.get(key, 0.0) + ms[slot]
-                slot_launch[key] = slot_launch.get(key, 0) + nl[slot]
-                if slot.startswith("k_phase"):
-                    slot_flops[key] = slot_flops.get(key, 0.0) + fps["phase" + slot[-1]] * kpart
-                rb = refine_bytes_per_step(n_, m_, slot)
-                if rb is not None:  # k_solve_est also forms the first estimates: R + 1 sweeps
-                    slot_bytes[key] = slot_bytes.get(key, 0.0) + rb * kpart * (R + (slot == "k_solve_est"))
         if ci == 0:  # batched FP64 against the oracle on a few short problems of the first chunk
             oracle_sample = [(i, probs[i]) for i in sorted(range(len(ch)), key=lambda i: ch[i]["k"])[:4]]
             oracle_sample = [(i, p, x64[i]) for i, p in oracle_sample]
@@ -419,14 +427,16 @@ def run_headline(args):
     acc["mixed_at_fp64_chunking"] = acc.pop("mixed")
     acc["mixed"] = mixed_ms
     value = total_steps / (acc["mixed"] * 1e-3)
-    rates = {k: total_steps / (v * 1e-3) for k, v in acc.items()}
+    rates = {k: (total_steps / (v * 1e-3) if v > 0 else None) for k, v in acc.items()}
+    ratio = (lambda a, b: rates[a] / rates[b] if rates[a] and rates[b] else None)
     roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": headline_traffic(dom, specs) if dom in slot_flops else None,
             "launches_per_step": slot_launch[dom], "kernel_ms_per_step": slot_ms[dom],
             "avg_launch_ms": slot_ms[dom] / max(slot_launch[dom], 1),
             "algorithmic_flops_per_step": slot_flops.get(dom), "algorithmic_bytes_per_step": slot_bytes.get(dom),
-            "measured": "HIP events around every launch of one eager mixed smoothing per chunk, parts one after "
-                        "another (the timed region replays graphs with the parts on concurrent streams)",
+            "measured": "HIP events around every launch of one eager mixed smoothing of each of the timed "
+                        "region's chunks, parts one after another (the timed region replays graphs with the parts "
+                        "on concurrent streams)",
             "note": ("FP32 peak is the published 157.3 TFLOP/s, reached only with packed FP32, which this build "
                      "disables; the unpacked FP32 VALU rate equals the FP64 one, 78.6") if "float" in dom else ""}
     kern = {k: {"ms_per_step": v, "launches": slot_launch[k],
@@ -436,8 +446,8 @@ def run_headline(args):
         "value": value, "ms_per_step": acc["mixed"], "total_steps": total_steps, "roofline": roof,
         "headline": {
             "refinements": R,
-            "ratio_vs_plain_fp64": rates["mixed"] / rates["plain_fp64_states_only"],
-            "ratio_vs_batched_fp64": rates["mixed"] / rates["fp64_states_only"],
+            "ratio_vs_plain_fp64": ratio("mixed", "plain_fp64_states_only"),
+            "ratio_vs_batched_fp64": ratio("mixed", "fp64_states_only"),
             "timesteps_per_s": rates,
             "ms_per_step": acc,
             "max_rel_err_states_vs_fp64": err_mixed,
@@ -800,9 +810,32 @@ def run_filter(args):
     flt.close()
     assert np.all(np.isfinite(x))
     say(f"filter: {nf} filters n={n} m={m}, {args.filter_steps} steps in {wall:.3f} s")
+    # the same steps with the inputs already in device memory (ukhip_filter_step_device)
+    ups = [device.Filter.upload(F=F, K=K, C=Cw, **steps[i]) for i in range(8)]
+    flt = device.Filter(nf, n, m)
+    for i in range(2):
+        flt.step_device(ups[i % 8])
+    device.lib().ukhip_synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.filter_steps):
+        flt.step_device(ups[i % 8])
+    device.lib().ukhip_synchronize()
+    wall_d = time.perf_counter() - t0
+    xd, _ = flt.estimates(want_R=False)
+    flt.close()
+    for u in ups:
+        for a in u.values():
+            if isinstance(a, device.DeviceArray):
+                a.free()
+    assert np.all(np.isfinite(xd))
+    say(f"filter, device-resident inputs: {args.filter_steps} steps in {wall_d:.3f} s")
     return {"filters": nf, "n": n, "m": m, "steps": args.filter_steps, "ms_per_step": wall * 1e3 / args.filter_steps,
             "value": nf * args.filter_steps / wall, "unit": "filter-steps/s",
-            "note": "host inputs uploaded every step (PCIe included)"}
+            "note": "host inputs uploaded every step (PCIe included)",
+            "device_inputs": {"ms_per_step": wall_d * 1e3 / args.filter_steps,
+                              "value": nf * args.filter_steps / wall_d, "unit": "filter-steps/s",
+                              "note": "inputs already in device memory (ukhip_filter_step_device), eight step "
+                                      "input sets reused in turn"}}
 
 
 def main():
@@ -849,6 +882,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the long workload's timed region, copy MIB MiB device-to-device once (a plain copy "
                          "kernel of known size for calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
+    ap.add_argument("--headline-mixed-only", action="store_true",
+                    help="profiling aid: the headline runs its timed mixed-precision leg only (no FP64 legs, "
+                         "ratios or errors)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="profiling aid: launch kernel by kernel everywhere (no HIP graph capture or replay)")
     ap.add_argument("--mixed-chunk-only", type=int, default=0, metavar="N",

@@ -69,10 +69,10 @@ void ukhip_use_mfma(int enable);
 /* Launch table of the odd-even QR phase kernels (phase 1..3) for state
  * dimensions n > 16, per precision (fp32 = 0: the FP64 kernels, 1: the FP32
  * elimination of the mixed-precision path). An entry is a compiled variant:
- * the matrix columns each group of lanes holds (columns_per_group; 2 shares
- * one register copy of each Householder vector between two columns) and the
- * workgroups per CU the register allocation leaves room for: 1, 2 or 4 with
- * one column per group, 2, 3 or 4 with two; columns_per_group 3 (phase 2,
+ * the matrix columns each group of lanes holds (columns_per_group; 2 or 3
+ * share one register copy of each Householder vector) and the workgroups per
+ * CU the register allocation leaves room for: 1, 2 or 4 with one column per
+ * group, 2, 3 or 4 with two, 2 or 3 with three; columns_per_group 0 (phase 2,
  * buckets 32, 48, 64, one workgroup per CU) is the blocked-WY phase 2 on the
  * matrix cores. The buckets are n in 17..24, 25..32, 33..48 and 49..64.
  * ukhip_autotune times every variant on a synthetic problem of k steps (0:
@@ -240,6 +240,13 @@ typedef struct {
 ukhip_filter_t* ukhip_filter_create(int32_t nfilters, int32_t n, int32_t m);
 /* One step of every filter (the first call has no evolution equation). */
 int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in);
+/* The same step with every array of `in` (F, H, c, K, G, o, m_i, C) already in
+ * device memory, in the layout above: nothing is copied, the launch reads them
+ * in place on the library stream (they must stay valid until the step has
+ * run; ukhip_synchronize). G, o and C are required; m_i values outside 0..m are
+ * clamped into it. No counterpart in the reference: for callers whose inputs
+ * are produced on the GPU. Returns 0 or -1. */
+int ukhip_filter_step_device(ukhip_filter_t* f, const ukhip_filter_inputs_t* in);
 /* Keeps every step's final factors (Rdiag, Rsupdiag, y of the reference's
  * UltimateKalman step) in device memory for ukhip_filter_smooth: at most
  * max_steps steps (nfilters * (max_steps - 1) * (2 n*n + n) doubles). Call it

@@ -244,7 +244,7 @@ def test_launch_table_set_and_get():
     saved = device.launch_table()
     try:
         assert len(saved) == 24
-        assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4", "c3w1"}
+        assert set(saved.values()) <= {"c1w1", "c1w2", "c1w4", "c2w2", "c2w3", "c2w4", "c3w2", "c3w3", "c0w1"}
         cpl = C.c_int(0)
         assert L.ukhip_qr_launch_set(1, 2, 48, 2, 3) == 0
         assert L.ukhip_qr_launch_get(1, 2, 48, C.byref(cpl)) == 3 and cpl.value == 2

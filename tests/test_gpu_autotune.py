@@ -1,7 +1,8 @@
 """The QR launch table and its run-time autotuner (ukhip_autotune).
 
 Every launch variant (one column per lane group at 1, 2 or 4 workgroups per
-CU asked of the register allocator, two columns per group at 2, 3 or 4) runs
+CU asked of the register allocator, two columns per group at 2, 3 or 4, three
+at 2 or 3) runs
 the same operations on every column: results must be bit-identical across
 variants. The autotuner must leave a valid table.
 """
@@ -15,7 +16,7 @@ from ukhip import problems as P
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(1, 1), (1, 2), (1, 4), (2, 2), (2, 3), (2, 4)]  # (columns per group, workgroups per CU)
+VARIANTS = [(1, 1), (1, 2), (1, 4), (2, 2), (2, 3), (2, 4), (3, 2), (3, 3)]  # (columns per group, workgroups per CU)
 
 
 def say(msg):
@@ -103,6 +104,6 @@ def test_fp32_variants_bit_identical(dev, n, m, illcond):
 def test_autotune_table(dev):
     table = dev.autotune(64, 1024)
     say(f"autotuned launch table: {table}")
-    assert set(table.values()) <= {f"c{c}w{w}" for c, w in VARIANTS} | {"c3w1"}
-    assert all(v != "c3w1" for k, v in table.items() if not k.startswith("phase2")), table
+    assert set(table.values()) <= {f"c{c}w{w}" for c, w in VARIANTS} | {"c0w1"}
+    assert all(v != "c0w1" for k, v in table.items() if not k.startswith("phase2")), table
     assert table["phase2<64, float>"] != "c2w3", table

@@ -264,3 +264,42 @@ def test_smoother_pass_matches_reference(dev, kind):
         flt.close()
         for kal in kals:
             L.kalman_free(kal)
+
+
+def test_device_resident_inputs(dev):
+    """ukhip_filter_step_device reads the step's arrays in device memory in
+    place: the same steps as ukhip_filter_step (host arrays, uploaded) give
+    bit-identical estimates, covariance factors and smoother output, including
+    unobserved filters (m_i = 0) and a step whose K and C come as matrices
+    ('C' and 'W', whitened on the device)."""
+    nf, n, m, k = 6, 5, 3, 24
+    probs = _problems(nf, n, m, k, True)
+    host, devf = dev.Filter(nf, n, m), dev.Filter(nf, n, m)
+    host.keep_history(k)
+    devf.keep_history(k)
+    try:
+        for i in range(k):
+            kw = _inputs(probs, i, n, m)
+            if i == 11:  # matrix representations: K as covariance 'C', C as 'W'
+                kw["K"] = np.stack([np.diag(1.0 / np.square(kv)) for kv in kw["K"]])
+                kw["C"] = np.stack([np.diag(cv) for cv in kw["C"]])
+                kw["K_type"], kw["C_type"] = "C", "W"
+            if i == 0:
+                kw.pop("F")
+            host.step(**kw)
+            up = dev.Filter.upload(**kw)
+            devf.step_device(up)
+            dev.lib().ukhip_synchronize()
+            xh, Rh = host.estimates()
+            xd, Rd = devf.estimates()
+            assert np.array_equal(xh, xd, equal_nan=True) and np.array_equal(Rh, Rd, equal_nan=True), i
+            for a in up.values():
+                if isinstance(a, dev.DeviceArray):
+                    a.free()
+        sh, Wh = host.smooth()
+        sd, Wd = devf.smooth()
+        assert np.array_equal(sh, sd) and np.array_equal(Wh, Wd)
+        say(f"device-resident filter inputs: {k} steps bit-identical to host inputs, smoother too")
+    finally:
+        host.close()
+        devf.close()

@@ -1,5 +1,5 @@
 """Phase 2 of the elimination in blocked WY form on the matrix cores
-(k_phase2_wy, launch-table variant 3) against the register/DPP kernel it
+(k_phase2_wy, launch-table variant c0w1) against the register/DPP kernel it
 replaces and the oracle.
 
 Bars: FP64 estimates and covariances within 1e-9 of the oracle and 1e-12 of
@@ -59,8 +59,8 @@ def test_wy_phase2_matches(dev, oracle, n, m, kind):
         b.smooth("oddeven")
         for i in range(len(probs)):
             ref.append(b.download(i))
-        old64 = _set_phase2(dev, 0, n, (3, 1))
-        old32 = _set_phase2(dev, 1, n, (3, 1))
+        old64 = _set_phase2(dev, 0, n, (0, 1))
+        old32 = _set_phase2(dev, 1, n, (0, 1))
         try:
             b.smooth("oddeven")
             wy = [b.download(i) for i in range(len(probs))]

@@ -741,7 +741,8 @@ __global__ void __launch_bounds__(((3 * NB + 2 + 63) / 64) * 64)
   const int64_t s = blockIdx.x;
   const int n = raw.n, mr = raw.mr;
   const int64_t nn = (int64_t)n * n;
-  const int m = raw.m[s];
+  const int mi = raw.m[s];
+  const int m = mi < 0 ? 0 : (mi > mr ? mr : mi); /* counts from device-resident filter inputs are not host-checked */
   const bool first = raw.first[s] != 0;
   const int kt = raw.Kt[s], ct = raw.Ct[s];
 
@@ -873,14 +874,16 @@ __global__ void __launch_bounds__(256) k_weigh_diag(uk_raw_t raw, uk_chain_t out
       if (e >= nn) e -= nn, ++ls, c -= n;
     }
   }
-  {
+  /* C = W G: not stored in FP32 when the FP32 chain forms it on load (c_synth) */
+  const bool w32c = F32 && !out32.c_synth;
+  if (w64 || w32c) {
     const int dls = 256 / mn, de = 256 - dls * mn, di = de % mr;
     int ls = tid / mn, e = tid - ls * mn, i = e % mr;
     for (int idx = tid; idx < cnt * mn; idx += 256) {
       const int64_t s = s0 + ls;
       const double v = (i < raw.m[s]) ? raw.C[s * mr + i] * raw.G[s * mn + e] : 0.0;
       if (w64) out.C[s * nn + e] = v;
-      if (F32) out32.C[s * nn + e] = (float)v;
+      if (w32c) out32.C[s * nn + e] = (float)v;
       ls += dls;
       e += de;
       i += di;
@@ -1073,7 +1076,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1>::NT))
     }
     if (isB || isY) col.move_1_to_0();
   }
-  const int m = in.m_i ? in.m_i[f] : mr;
+  const int mi = in.m_i ? in.m_i[f] : mr;
+  const int m = mi < 0 ? 0 : (mi > mr ? mr : mi); /* device-resident counts are not checked on the host */
   if (m > 0) {
     if (in.pre) {
       if (isB) col.load_block(1, in.wC + f * nn + (int64_t)bc * mr, m, sub); /* W G, ld mr in an n*n slot */
@@ -1254,9 +1258,9 @@ extern "C" int uk_bucket_qr(int n) {
 /* Steps per workgroup of k_weigh_diag: about 2048 elements of work per
  * 256-thread workgroup (the n x n pass is skipped when B is formed on load and
  * D is stored as its diagonal). */
-static int weigh_diag_spw(const uk_raw_t* raw, const uk_chain_t* out) {
+static int weigh_diag_spw(const uk_raw_t* raw, const uk_chain_t* out, bool c_pass = true) {
   const int n = raw->n, mr = raw->mr;
-  const int per = (out->b_synth && out->d_diag ? 0 : n * n) + mr * n + n + mr;
+  const int per = (out->b_synth && out->d_diag ? 0 : n * n) + (c_pass ? mr * n : 0) + n + mr;
   const int spw = 2048 / (per > 0 ? per : 1);
   return spw < 1 ? 1 : (spw > 64 ? 64 : spw);
 }
@@ -1311,14 +1315,14 @@ extern "C" hipError_t uk_launch_weigh_obs(const uk_raw_t* raw, const uk_chain_t*
   return hipGetLastError();
 }
 
-/* The level-0 chain's FP32 copy (the mixed-precision path) and, written by the
- * same pass when w64 is set, the FP64 chain, when every K and C is given as
- * 'w'; returns hipErrorNotSupported otherwise (then the FP64 chain is
- * converted afterwards). */
+/* The level-0 chain's FP32 copy (the mixed-precision path; its C only when
+ * out32->c_synth is 0) and, written by the same pass when w64 is set, the FP64
+ * chain, when every K and C is given as 'w'; returns hipErrorNotSupported
+ * otherwise (then the FP64 chain is converted afterwards). */
 extern "C" hipError_t uk_launch_weigh_f32(const uk_raw_t* raw, const uk_chain_t* out, const uk_chain32_t* out32,
                                           int w64, hipStream_t st) {
   if (raw->K_full || raw->C_full) return hipErrorNotSupported;
-  const int spw = weigh_diag_spw(raw, out);
+  const int spw = weigh_diag_spw(raw, out, w64 || !out32->c_synth);
   const int64_t blocks = (raw->steps + spw - 1) / spw;
   k_weigh_diag<true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(*raw, *out, *out32, spw, w64);
   return hipGetLastError();
@@ -1335,19 +1339,19 @@ struct QrLaunch {
 };
 static QrLaunch g_qr[2][3][4] = {
     /* FP64:  bucket 32   48        64        24 */
-    {/* phase 1 */ {{2, 3}, {2, 3}, {1, 2}, {1, 2}},
-     /* phase 2 */ {{2, 3}, {1, 1}, {1, 1}, {1, 2}},
-     /* phase 3 */ {{1, 2}, {1, 4}, {2, 3}, {1, 1}}},
+    {/* phase 1 */ {{1, 2}, {2, 4}, {1, 2}, {1, 2}},
+     /* phase 2 */ {{2, 4}, {3, 2}, {3, 2}, {1, 1}},
+     /* phase 3 */ {{1, 4}, {1, 4}, {2, 3}, {1, 2}}},
     /* FP32 */
-    {/* phase 1 */ {{2, 4}, {2, 2}, {2, 4}, {1, 4}},
-     /* phase 2 */ {{2, 4}, {2, 4}, {1, 2}, {2, 3}},
-     /* phase 3 */ {{1, 2}, {2, 2}, {2, 2}, {1, 4}}}};
+    {/* phase 1 */ {{2, 3}, {2, 4}, {2, 4}, {1, 4}},
+     /* phase 2 */ {{2, 2}, {3, 2}, {3, 3}, {2, 4}},
+     /* phase 3 */ {{1, 2}, {1, 1}, {2, 3}, {1, 1}}}};
 static int qr_bucket_index(int nb) {
   return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : (nb == 24 ? 3 : -1)));
 }
 /* the compiled variants: one column per group at 1, 2 or 4 workgroups per CU,
- * two columns per group at 2, 3 or 4 */
-/* cpl 3: phase 2 in blocked WY form on the matrix cores (k_phase2_wy), one
+ * two columns per group at 2, 3 or 4, three columns per group at 2 or 3 */
+/* cpl 0: phase 2 in blocked WY form on the matrix cores (k_phase2_wy), one
  * workgroup per CU asked of the register allocator.
  * Left out: the FP32 phase 2 of bucket 64 with two columns per group at 3
  * workgroups per CU. On MI355X it returned non-finite factors for an
@@ -1358,7 +1362,7 @@ static int qr_bucket_index(int nb) {
 static bool qr_variant(int fp32, int nb, int cpl, int wg, int phase) {
   if (fp32 && nb == 64 && phase == 2 && cpl == 2 && wg == 3) return false;
   return (cpl == 1 && (wg == 1 || wg == 2 || wg == 4)) || (cpl == 2 && (wg == 2 || wg == 3 || wg == 4)) ||
-         (cpl == 3 && wg == 1 && phase == 2 && nb % 16 == 0);
+         (cpl == 3 && (wg == 2 || wg == 3)) || (cpl == 0 && wg == 1 && phase == 2 && nb % 16 == 0);
 }
 
 /* fp32: 0 the FP64 kernels, 1 the FP32 elimination of the mixed-precision path */
@@ -1419,8 +1423,10 @@ hipError_t launch_eliminate(const uk_level_t* lv, const ChainT<T>& in, const Cha
 extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t* in, const uk_chain_t* out,
                                           const uk_factors_t* fac, double* fix, int n, hipStream_t st,
                                           hipEvent_t* ev) {
-  const ChainT<double> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri, in->b_synth, in->wk, in->wF, in->wFidx};
-  const ChainT<double> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri, out->b_synth, out->wk, out->wF, out->wFidx};
+  const ChainT<double> ci{in->B,      in->D,  in->c,  in->C,     in->o,   in->mr,  in->d_diag, in->c_tri,
+                          in->b_synth, in->wk, in->wF, in->wFidx, 0,       nullptr, nullptr,    nullptr};
+  const ChainT<double> co{out->B,      out->D,  out->c,  out->C,     out->o,  out->mr, out->d_diag, out->c_tri,
+                          out->b_synth, out->wk, out->wF, out->wFidx, 0,       nullptr, nullptr,     nullptr};
   const FactorsT<double> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<double>(lv, ci, co, f, fix, n, st, ev);
 }
@@ -1429,8 +1435,10 @@ extern "C" hipError_t uk_launch_eliminate(const uk_level_t* lv, const uk_chain_t
 extern "C" hipError_t uk_launch_eliminate32(const uk_level_t* lv, const uk_chain32_t* in, const uk_chain32_t* out,
                                             const uk_factors32_t* fac, float* fix, int n, hipStream_t st,
                                             hipEvent_t* ev) {
-  const ChainT<float> ci{in->B, in->D, in->c, in->C, in->o, in->mr, in->d_diag, in->c_tri, in->b_synth, in->wk, in->wF, in->wFidx};
-  const ChainT<float> co{out->B, out->D, out->c, out->C, out->o, out->mr, out->d_diag, out->c_tri, out->b_synth, out->wk, out->wF, out->wFidx};
+  const ChainT<float> ci{in->B,      in->D,  in->c,  in->C,     in->o,      in->mr,  in->d_diag, in->c_tri,
+                         in->b_synth, in->wk, in->wF, in->wFidx, in->c_synth, in->rG, in->rC,     in->rm};
+  const ChainT<float> co{out->B,      out->D,  out->c,  out->C,     out->o,      out->mr, out->d_diag, out->c_tri,
+                         out->b_synth, out->wk, out->wF, out->wFidx, out->c_synth, out->rG, out->rC,     out->rm};
   const FactorsT<float> f{fac->R, fac->Bt, fac->Y, fac->cf};
   return launch_eliminate<float>(lv, ci, co, f, fix, n, st, ev);
 }

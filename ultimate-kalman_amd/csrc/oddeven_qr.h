@@ -28,6 +28,10 @@ struct ChainT {
   const double* wk;
   const double* wF;
   const int64_t* wFidx;
+  int32_t c_synth; /* 1: C_s = diag(rC_s) rG_s (mr x n, ld mr; rows >= rm[s] zero), formed on load */
+  const double* rG;
+  const double* rC;
+  const int32_t* rm;
 };
 template <class T>
 struct FactorsT {
@@ -138,6 +142,16 @@ struct Col {
     for (int l = 0; l < H; ++l) {
       const int r = SL * l + sub;
       a[blk * H + l] = (r < n) ? T(-k[r] * F[r + (int64_t)col * n]) : T(0);
+    }
+  }
+  /* block blk <- column col of C = diag(w) G (G mr x n, ld mr; rows >= m zero),
+   * each entry the FP64 product rounded to T, as the whitening pass stores it */
+  __device__ __forceinline__ void load_csynth(int blk, const double* __restrict__ w, const double* __restrict__ G,
+                                              int col, int mr, int m, int sub) {
+#pragma unroll
+    for (int l = 0; l < H; ++l) {
+      const int r = SL * l + sub;
+      a[blk * H + l] = (r < mr && r < m) ? T(w[r] * G[r + (int64_t)col * mr]) : T(0);
     }
   }
   /* block blk <- column col of a diagonal matrix given by its diagonal d */
@@ -656,7 +670,8 @@ __global__ void __launch_bounds__((QRGeom<NB, 2 * NB + 1, 1, WG, CPL, true>::NT)
       if (lone) col.zero_block(0);
       else if (in.b_synth) col.load_bsynth(0, in.wk + sp * n, in.wF + (in.wFidx ? in.wFidx[sp] * nn : 0), qc, n, sub);
       else col.load_block(0, in.B + sp * nn + (int64_t)qc * n, n, sub);
-      col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
+      if (in.c_synth) col.load_csynth(1, in.rC + sj * mr, in.rG + sj * (int64_t)mr * n, qc, mr, in.rm[sj], sub);
+      else col.load_block(1, in.C + sj * nn + (int64_t)qc * mr, mr, sub);
     }
     if (isQ) {
       if (in.d_diag) col.load_diag(0, in.D + sp * nn, qc, sub); /* stored as its diagonal */
@@ -839,7 +854,8 @@ __global__ void __launch_bounds__((QRGeom<NB, NB + 1, 3, WG, CPL, true>::NT),
     const int qc = isQ ? c : 0;
     if (isQ) {
       col.load_block(0, out.C + t * nn + (int64_t)qc * mr, mr, sub); /* D~_{j+1} */
-      col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
+      if (in.c_synth) col.load_csynth(1, in.rC + sp * mr, in.rG + sp * (int64_t)mr * n, qc, mr, in.rm[sp], sub);
+      else col.load_block(1, in.C + sp * nn + (int64_t)qc * mr, mr, sub);
     }
     if (isR1) {
       col.load_block(0, out.o + t * n, mr, sub); /* c~_{j+1} */
@@ -1184,14 +1200,19 @@ void phase_launch_impl(const uk_level_t* lv, const ChainT<T>& in, const ChainT<T
                        int n, hipStream_t st, int phase, ukqr::Launch v) {
   if constexpr (NB >= 24) {
     if constexpr (NB % 16 == 0) {
-      if (v.cpl == 3 && phase == 2) { /* blocked WY on the matrix cores */
+      if (v.cpl == 0 && phase == 2) { /* blocked WY on the matrix cores */
         const dim3 g((unsigned)lv->evens), b(WyGeom<NB>::NT);
         if (in.d_diag) k_phase2_wy<NB, T, true><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
         else k_phase2_wy<NB, T, false><<<g, b, 0, st>>>(*lv, in, out, fac, fix, n);
         return;
       }
     }
-    if (v.cpl == 2) {
+    if (v.cpl == 3) {
+      switch (v.wg) {
+        case 2: phases_nb<NB, T, 2, 3>(lv, in, out, fac, fix, n, st, phase); break;
+        default: phases_nb<NB, T, 3, 3>(lv, in, out, fac, fix, n, st, phase); break;
+      }
+    } else if (v.cpl == 2) {
       switch (v.wg) {
         case 2: phases_nb<NB, T, 2, 2>(lv, in, out, fac, fix, n, st, phase); break;
         case 3: phases_nb<NB, T, 3, 2>(lv, in, out, fac, fix, n, st, phase); break;

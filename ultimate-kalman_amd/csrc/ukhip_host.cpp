@@ -1559,6 +1559,13 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
   const uk_chain_t& c64 = b->chain[0];
   uk_chain32_t c32 = b->chain32[0];
   const uk_chain_t w0 = level0_chain(b, b->chain[0]); /* its flags imply the fused path below */
+  /* With every K and C given as 'w' the FP32 elimination forms the level-0 C =
+   * diag(w) G on load (phases 1 and 3), so the whitening does not store it. */
+  const bool csynth = w0.b_synth != 0;
+  c32.c_synth = csynth ? 1 : 0;
+  c32.rG = b->raw.G;
+  c32.rC = b->raw.C;
+  c32.rm = b->m0;
   /* With diagonal whitening and n, m <= 32 the refinement's k_normal_w forms the
    * FP64 level-0 chain from the raw inputs, so the whitening writes only the
    * FP32 copy. */
@@ -1585,10 +1592,11 @@ static bool run_mixed(ukhip_batch* b, int refinements, int fp32_levels, bool cov
   for (int l = 0; l < nl; ++l) {
     uk_level_t lv = b->levels[l].dev;
     if (l < lsw) {
-      uk_chain32_t in = l == 0 ? level0_chain(b, b->chain32[0]) : b->chain32[l & 1];
+      uk_chain32_t in = l == 0 ? level0_chain(b, c32) : b->chain32[l & 1];
       if (l > 0) {
         in.mr = n;
         in.c_tri = 1;
+        in.c_synth = 0;
       }
       uk_chain32_t out = b->chain32[(l + 1) & 1]; /* its flags stay 0: a dense chain */
       out.mr = n;
@@ -2460,7 +2468,8 @@ static bool filter_aux(ukhip_filter* f) {
 
 static bool full_rep(char t) { return t == 'W' || t == 'C' || t == 'U' || t == 'F'; }
 
-extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in) {
+/* One filter step; dev: the arrays of `in` are device pointers, read in place. */
+static int filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in, bool dev) {
   if (!f || !in) { uk_set_error("NULL filter or inputs"); return -1; }
   const bool first = f->steps == 0;
   const size_t P = (size_t)f->nf, n = (size_t)f->n, nn = n * n, mr = (size_t)f->mr;
@@ -2476,23 +2485,25 @@ extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t*
   }
   if (!in->K && !first) { uk_set_error("ukhip_filter_step: K is required after the first step"); return -1; }
   if (!first && !in->F) { uk_set_error("ukhip_filter_step: F is required after the first step"); return -1; }
-  bool observed = in->m_i == nullptr;
-  for (size_t p = 0; in->m_i && p < P; ++p) {
+  bool observed = in->m_i == nullptr || dev; /* device counts are clamped by the kernel, not read here */
+  for (size_t p = 0; !dev && in->m_i && p < P; ++p) {
     if (in->m_i[p] < 0 || in->m_i[p] > f->mr) { uk_set_error("ukhip_filter_step: m_i outside 0..m"); return -1; }
     observed = observed || in->m_i[p] > 0;
   }
   if (observed && (!in->G || !in->o || !in->C)) { uk_set_error("ukhip_filter_step: G, o and C are required"); return -1; }
-  /* the previous step's upload reads the staging buffer */
-  if (!hip_ok(hipStreamSynchronize(g_stream), "ukhip_filter_step")) return -1;
+  /* the previous step's upload reads the staging buffer (device inputs use it
+   * only for the representation codes of a whitened step) */
+  if ((!dev || full) && !hip_ok(hipStreamSynchronize(g_stream), "ukhip_filter_step")) return -1;
   size_t off = 0;
   uk_filter_in_t din{};
-  auto put = [&](const void* src, size_t bytes) -> const void* {
+  auto stage = [&](const void* src, size_t bytes) -> const void* {
     if (!src) return nullptr;
     std::memcpy(f->h_in + off, src, bytes);
     const void* dp = f->d_in + off;
     off += (bytes + 255) & ~(size_t)255;
     return dp;
   };
+  auto put = [&](const void* src, size_t bytes) -> const void* { return dev ? src : stage(src, bytes); };
   const size_t mcols = (size_t)f->mr; /* rows of G, o, C per filter as given: ld m */
   din.F = (const double*)put(first ? nullptr : in->F, (in->F_shared ? 1 : P) * nn * 8);
   din.F_shared = in->F_shared;
@@ -2511,12 +2522,13 @@ extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t*
     codes.assign(2 * P, 0);
     std::fill(codes.begin(), codes.begin() + P, (int8_t)cov_code(kt));
     std::fill(codes.begin() + P, codes.end(), (int8_t)cov_code(ct));
-    dKt = (const int8_t*)put(codes.data(), P);
-    dCt = (const int8_t*)put(codes.data() + P, P);
+    dKt = (const int8_t*)stage(codes.data(), P);
+    dCt = (const int8_t*)stage(codes.data() + P, P);
   }
   if (!din.K) din.K = f->st.y; /* first step: K is never read */
   if (off > f->in_bytes) { uk_set_error("ukhip_filter_step: staging overflow"); return -1; }
-  if (!hip_ok(hipMemcpyAsync(f->d_in, f->h_in, off, hipMemcpyHostToDevice, g_stream), "filter upload")) return -1;
+  if (off > 0 && !hip_ok(hipMemcpyAsync(f->d_in, f->h_in, off, hipMemcpyHostToDevice, g_stream), "filter upload"))
+    return -1;
   if (full) { /* whiten the step on the device (k_weigh), then the filter step reads it */
     if (!filter_aux(f)) return -1;
     uk_raw_t raw{};
@@ -2552,6 +2564,14 @@ extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t*
     return -1;
   ++f->steps;
   return 0;
+}
+
+extern "C" int ukhip_filter_step(ukhip_filter_t* f, const ukhip_filter_inputs_t* in) {
+  return filter_step(f, in, false);
+}
+
+extern "C" int ukhip_filter_step_device(ukhip_filter_t* f, const ukhip_filter_inputs_t* in) {
+  return filter_step(f, in, true);
 }
 
 extern "C" int ukhip_filter_keep_history(ukhip_filter_t* f, int64_t max_steps) {
@@ -3088,8 +3108,8 @@ extern "C" int ukhip_autotune(int32_t n_max, int32_t k) {
     p.G = G.data(); p.o = o.data(); p.C = Call.data(); p.C_type = 'w';
     ukhip_batch_t* b = ukhip_batch_create(1, &p);
     if (!b) return -1;
-    /* {cpl, wg}; cpl 3 (blocked WY) exists for phase 2 only */
-    static const int kVariants[7][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 2}, {2, 3}, {2, 4}, {3, 1}};
+    /* {cpl, wg}; cpl 0 (blocked WY) exists for phase 2 only */
+    static const int kVariants[9][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 2}, {2, 3}, {2, 4}, {3, 2}, {3, 3}, {0, 1}};
     /* The phase's HIP events of an eager smoothing: FP64 odd-even, or the
      * mixed-precision path without a refinement step for the FP32
      * elimination. One phase varied at a time. */

@@ -72,7 +72,10 @@ typedef struct {
   const double* rc;
 } uk_chain_t;
 
-/* The same chain in single precision (the FP32 elimination). */
+/* The same chain in single precision (the FP32 elimination). c_synth = 1
+ * (level 0, every C given as 'w'): C is not stored; C_s = diag(rC_s) rG_s
+ * (rows >= rm[s] zero) is formed on load, rounded to FP32 as the whitening
+ * would store it. */
 typedef struct {
   float* B;
   float* D;
@@ -86,6 +89,10 @@ typedef struct {
   const double* wk;
   const double* wF;
   const int64_t* wFidx;
+  int32_t c_synth;
+  const double* rG;
+  const double* rC;
+  const int32_t* rm;
 } uk_chain32_t;
 
 /* Per-level factors kept for the back substitution (one slot per even step):

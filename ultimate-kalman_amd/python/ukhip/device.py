@@ -87,6 +87,8 @@ def lib():
         L.ukhip_filter_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
         L.ukhip_filter_step.argtypes = [C.c_void_p, C.c_void_p]
         L.ukhip_filter_step.restype = C.c_int
+        L.ukhip_filter_step_device.argtypes = [C.c_void_p, C.c_void_p]
+        L.ukhip_filter_step_device.restype = C.c_int
         L.ukhip_filter_estimates.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ukhip_filter_estimates.restype = C.c_int
         L.ukhip_filter_free.argtypes = [C.c_void_p]
@@ -211,6 +213,53 @@ def problem_struct(p, keep):
     return s
 
 
+_hip = None
+
+
+def _hip_lib():
+    """The HIP runtime (device memory for the device-resident entry points)."""
+    global _hip
+    if _hip is None:
+        h = C.CDLL("libamdhip64.so")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipMalloc.restype = C.c_int
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipFree.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A host array copied once into device memory (hipMalloc + hipMemcpy),
+    for the entry points that read device-resident inputs."""
+
+    def __init__(self, a, dtype=np.float64):
+        lib()
+        a = np.ascontiguousarray(a, dtype=dtype)
+        h = _hip_lib()
+        p = C.c_void_p()
+        if h.hipMalloc(C.byref(p), max(a.nbytes, 1)) != 0:
+            raise UkhipError(f"hipMalloc of {a.nbytes} bytes failed")
+        self.ptr = p.value
+        if h.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) != 0:  # hipMemcpyHostToDevice
+            h.hipFree(self.ptr)
+            self.ptr = None
+            raise UkhipError("hipMemcpy to the device failed")
+
+    def free(self):
+        if self.ptr:
+            _hip_lib().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class FilterInputs(C.Structure):
     """ukhip_filter_inputs_t (include/ukhip.h)."""
     _fields_ = [("F", C.c_void_p), ("F_shared", C.c_int32), ("H", C.c_void_p), ("c", C.c_void_p),
@@ -245,38 +294,58 @@ class Filter:
                "ukhip_filter_smooth")
         return x, (np.swapaxes(W, 2, 3) if want_W else None)
 
-    def step(self, K, G, o, C, F=None, H=None, c=None, m_i=None, K_type="w", C_type="w"):
-        """One evolve + observe step. Arrays hold all filters: F (nf, n, n) or
-        (n, n) shared, H/c optional, G (nf, m, n), o (nf, m), m_i (nf,)
-        observation counts; K and C in the representation K_type / C_type:
-        'w' inverse standard deviations (K (nf, n), C (nf, m)), or 'W', 'C',
-        'U', 'F' matrices (K (nf, n, n), C (nf, m, m))."""
-        keep = []
+    @staticmethod
+    def layout(K, G, o, C, F=None, H=None, c=None, m_i=None, K_type="w", C_type="w"):
+        """One step's inputs in the C layout of ukhip_filter_inputs_t: a dict
+        of contiguous arrays (matrices column-major, filter after filter) plus
+        F_shared and the representation codes. Arrays hold all filters: F
+        (nf, n, n) or (n, n) shared, H/c optional, G (nf, m, n), o (nf, m),
+        m_i (nf,) observation counts; K and C in the representation K_type /
+        C_type: 'w' inverse standard deviations (K (nf, n), C (nf, m)), or 'W',
+        'C', 'U', 'F' matrices (K (nf, n, n), C (nf, m, m))."""
 
-        def ptr(a, dtype=np.float64):
-            if a is None:
-                return None
-            a = np.ascontiguousarray(a, dtype=dtype)
-            keep.append(a)
-            return a.ctypes.data
+        def arr(a, dtype=np.float64):
+            return None if a is None else np.ascontiguousarray(a, dtype=dtype)
 
         def colmajor(a):  # (..., rows, cols) -> column-major per matrix
-            return None if a is None else np.swapaxes(np.asarray(a, dtype=np.float64), -1, -2)
+            return None if a is None else arr(np.swapaxes(np.asarray(a, dtype=np.float64), -1, -2))
 
-        s = FilterInputs()
         Fa = None if F is None else np.asarray(F, dtype=np.float64)
-        s.F_shared = 1 if (Fa is not None and Fa.ndim == 2) else 0
-        s.F = ptr(colmajor(Fa))
-        s.H = ptr(colmajor(H))
-        s.c = ptr(c)
-        s.K = ptr(K if K_type == "w" else colmajor(K))
-        s.G = ptr(colmajor(G))
-        s.o = ptr(o)
-        s.C = ptr(C if C_type == "w" else colmajor(C))
-        s.m_i = ptr(m_i, np.int32)
-        s.K_type = K_type.encode()
-        s.C_type = C_type.encode()
+        return {"F": colmajor(Fa), "F_shared": 1 if (Fa is not None and Fa.ndim == 2) else 0, "H": colmajor(H),
+                "c": arr(c), "K": arr(K) if K_type == "w" else colmajor(K), "G": colmajor(G), "o": arr(o),
+                "C": arr(C) if C_type == "w" else colmajor(C), "m_i": arr(m_i, np.int32),
+                "K_type": K_type, "C_type": C_type}
+
+    @staticmethod
+    def _struct(lay, ptr):
+        s = FilterInputs()
+        for k in ("F", "H", "c", "K", "G", "o", "C", "m_i"):
+            setattr(s, k, ptr(lay[k]))
+        s.F_shared = lay["F_shared"]
+        s.K_type = lay["K_type"].encode()
+        s.C_type = lay["C_type"].encode()
+        return s
+
+    def step(self, K, G, o, C, F=None, H=None, c=None, m_i=None, K_type="w", C_type="w"):
+        """One evolve + observe step from host arrays (layout() gives the
+        shapes)."""
+        lay = self.layout(K, G, o, C, F, H, c, m_i, K_type, C_type)
+        s = self._struct(lay, lambda a: None if a is None else a.ctypes.data)
         _check(lib().ukhip_filter_step(self.h, _ctypes.byref(s)), "ukhip_filter_step")
+        self._steps += 1
+
+    @staticmethod
+    def upload(**kw):
+        """One step's inputs (the arguments of step()) copied to device memory
+        once: a dict of DeviceArray for step_device."""
+        lay = Filter.layout(**kw)
+        return {k: (DeviceArray(v, v.dtype) if isinstance(v, np.ndarray) else v) for k, v in lay.items()}
+
+    def step_device(self, dev_inputs):
+        """ukhip_filter_step_device: one step from inputs already in device
+        memory (upload())."""
+        s = self._struct(dev_inputs, lambda a: None if a is None else a.ptr)
+        _check(lib().ukhip_filter_step_device(self.h, _ctypes.byref(s)), "ukhip_filter_step_device")
         self._steps += 1
 
     def estimates(self, want_R=True):
