@@ -250,10 +250,14 @@ def test_launch_table_set_and_get():
         assert L.ukhip_qr_launch_get(1, 2, 48, C.byref(cpl)) == 3 and cpl.value == 2
         assert L.ukhip_qr_launch_set(0, 2, 48, 1, 3) == -1   # not compiled
         assert L.ukhip_qr_launch_set(0, 2, 48, 2, 1) == -1
-        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 2) == -1   # blocked WY: one workgroup per CU only
-        assert L.ukhip_qr_launch_set(0, 1, 48, 3, 1) == -1   # and for phase 2 only
-        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 1) == 0
-        assert L.ukhip_qr_launch_get(0, 2, 48, C.byref(cpl)) == 1 and cpl.value == 3
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 2) == 0    # three columns per group
+        assert L.ukhip_qr_launch_get(0, 2, 48, C.byref(cpl)) == 2 and cpl.value == 3
+        assert L.ukhip_qr_launch_set(0, 2, 48, 3, 1) == -1   # not compiled
+        assert L.ukhip_qr_launch_set(0, 2, 48, 0, 2) == -1   # blocked WY: one workgroup per CU only
+        assert L.ukhip_qr_launch_set(0, 1, 48, 0, 1) == -1   # and for phase 2 only
+        assert L.ukhip_qr_launch_set(0, 2, 24, 0, 1) == -1   # and buckets 32, 48, 64
+        assert L.ukhip_qr_launch_set(0, 2, 48, 0, 1) == 0
+        assert L.ukhip_qr_launch_get(0, 2, 48, C.byref(cpl)) == 1 and cpl.value == 0
         assert L.ukhip_qr_launch_set(0, 4, 48, 1, 1) == -1
         assert L.ukhip_qr_launch_set(1, 2, 64, 2, 3) == -1   # left out of the FP32 table at bucket 64
         assert L.ukhip_qr_launch_set(0, 2, 64, 2, 3) == 0
