@@ -291,9 +291,12 @@ def chunked(specs, bytes_per_step, budget):
 def accumulate_parts(b, ch, R, kern):
     """Per-kernel device time of one eager mixed smoothing of batch b (chunk
     specs ch; parts one after another, HIP events around every launch),
-    added into kern = (ms, launches, flops, bytes) dicts keyed by kernel label."""
+    added into kern = (ms, launches, flops, bytes) dicts keyed by kernel label.
+    Returns the smoothing's summed device time (ms)."""
     slot_ms, slot_launch, slot_flops, slot_bytes = kern
+    total = 0.0
     for n_, ms, nl in b.timed_parts(mixed=(R, 0, False)):
+        total += ms["total"]
         m_ = max(n_ // 2, 1)
         kpart = sum(sp["k"] for sp in ch if sp["n"] == n_)
         fps = oddeven_flops_per_step(n_, m_, False)
@@ -309,6 +312,7 @@ def accumulate_parts(b, ch, R, kern):
             rb = refine_bytes_per_step(n_, m_, slot)
             if rb is not None:  # k_solve_est also forms the first estimates: R + 1 sweeps
                 slot_bytes[key] = slot_bytes.get(key, 0.0) + rb * kpart * (R + (slot == "k_solve_est"))
+    return total
 
 
 def mixed_pass(specs, args, kern):
@@ -316,15 +320,22 @@ def mixed_pass(specs, args, kern):
     refinement steps) over the whole workload in chunks sized for its own
     device memory; returns (ms per smoothing summed over chunks, per-chunk
     records). The per-kernel times of one eager smoothing per chunk go into
-    kern (accumulate_parts): the roofline's kernel ran on these chunks."""
+    kern (accumulate_parts): the roofline's kernel ran on these chunks. With
+    --mixed-parts-only that eager smoothing is the only one, and its summed
+    device time stands for the chunk's time (a profiling aid: a kernel trace
+    of that run holds exactly the launches the roofline times)."""
     total, rec = 0.0, []
     for ci, ch in enumerate(chunked(specs, mixed_bytes_per_step, args.mixed_hbm_budget_gb * 1e9)):
         t = time.perf_counter()
         probs = [P.batch_problem(sp, lists=False) for sp in ch]
         tg = time.perf_counter() - t
         b = device.Batch(probs)
-        ms = _time(lambda: b.smooth_mixed_opt(args.refine, 0, False, sync=False), args.steps, max(args.warmup, 1))
-        accumulate_parts(b, ch, args.refine, kern)
+        if args.mixed_parts_only:
+            ms = accumulate_parts(b, ch, args.refine, kern)
+        else:
+            ms = _time(lambda: b.smooth_mixed_opt(args.refine, 0, False, sync=False), args.steps,
+                       max(args.warmup, 1))
+            accumulate_parts(b, ch, args.refine, kern)
         b.free()
         del probs
         ksum = sum(sp["k"] for sp in ch)
@@ -436,7 +447,8 @@ def run_headline(args):
             "algorithmic_flops_per_step": slot_flops.get(dom), "algorithmic_bytes_per_step": slot_bytes.get(dom),
             "measured": "HIP events around every launch of one eager mixed smoothing of each of the timed "
                         "region's chunks, parts one after another (the timed region replays graphs with the parts "
-                        "on concurrent streams)",
+                        "on concurrent streams); the launches are the FP32 levels' and, for the problems with more "
+                        "than 8 levels, the FP64 top levels' (a few steps each), as the flops count",
             "note": ("FP32 peak is the published 157.3 TFLOP/s, reached only with packed FP32, which this build "
                      "disables; the unpacked FP32 VALU rate equals the FP64 one, 78.6") if "float" in dom else ""}
     kern = {k: {"ms_per_step": v, "launches": slot_launch[k],
@@ -885,12 +897,18 @@ def main():
     ap.add_argument("--headline-mixed-only", action="store_true",
                     help="profiling aid: the headline runs its timed mixed-precision leg only (no FP64 legs, "
                          "ratios or errors)")
+    ap.add_argument("--mixed-parts-only", action="store_true",
+                    help="profiling aid: with --headline-mixed-only, the mixed leg runs only the eager smoothing per "
+                         "chunk (parts one after another) whose per-launch times give the roofline; ms_per_step is "
+                         "then that smoothing's summed device time")
     ap.add_argument("--no-graphs", action="store_true",
                     help="profiling aid: launch kernel by kernel everywhere (no HIP graph capture or replay)")
     ap.add_argument("--mixed-chunk-only", type=int, default=0, metavar="N",
                     help="profiling aid: the first N problems of the batching workload as one device batch, two eager "
                          "mixed smoothings (graphs off; --copy-calibration runs between them), no bench line")
     args = ap.parse_args()
+    if args.mixed_parts_only and not args.headline_mixed_only:
+        ap.error("--mixed-parts-only times the mixed leg alone: give --headline-mixed-only too")
     if args.no_graphs:
         device.lib().ukhip_use_graphs(0)
     if args.mixed_chunk_only:
@@ -928,7 +946,8 @@ def main():
            "config": {"workload": "batching: " + BATCH_DESC,
                       "algorithm": f"odd-even, FP32 elimination + {args.refine} FP64 refinement step(s) of the "
                                    "estimates, batched (one launch per level per state dimension, dimensions on "
-                                   + ("forked streams, kernel by kernel (--no-graphs))" if args.no_graphs else
+                                   + ("one after another, eager (--mixed-parts-only))" if args.mixed_parts_only
+                                      else "forked streams, kernel by kernel (--no-graphs))" if args.no_graphs else
                                       "forked streams, one HIP graph replay per chunk)")}}
     if not args.no_headline:
         h = run_headline(args)
