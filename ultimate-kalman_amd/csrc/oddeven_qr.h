@@ -98,12 +98,22 @@ struct Vec2<float> {
   using type = float2;
 };
 
-__device__ __forceinline__ int64_t find_problem(const int64_t* start, int nprob, int64_t x) {
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= x) lo = mid;
-    else hi = mid - 1;
+/* The problem of work item x: the last p < nprob with start[p] <= x (start
+ * non-decreasing, start[0] = 0 <= x; x the same on every lane). The wave
+ * searches in rounds of 64 probes, one per lane, so a search costs
+ * ceil(log64 nprob) dependent loads instead of a binary search's
+ * ceil(log2 nprob): 2 up to 4096 problems. Called with the whole wave active
+ * (at kernel start, every kernel's workgroup a multiple of 64 threads). */
+__device__ __forceinline__ int64_t find_problem(const int64_t* __restrict__ start, int nprob, int64_t x) {
+  const int lane = (int)(threadIdx.x & 63);
+  int lo = 0, cnt = nprob; /* the answer lies in [lo, lo + cnt) and start[lo] <= x */
+  while (cnt > 1) {
+    const int step = (cnt + 63) >> 6;
+    const int off = lane * step;
+    const bool ok = off < cnt && start[lo + off] <= x; /* true on a prefix of the lanes, lane 0 included */
+    const int j = __popcll(__ballot(ok)) - 1;
+    lo += j * step;
+    cnt = min(step, cnt - j * step);
   }
   return lo;
 }
