@@ -131,3 +131,43 @@ def test_shared_F_is_the_expanded_F(dev):
         out.append(b.download(0))
         b.free()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_many_problems_one_dimension(dev, oracle):
+    """4,200 problems of one state dimension in one batch: at level 0 the
+    kernels' 64-way problem search runs three rounds (beyond 64 * 64
+    problems), and every unit must land in its own problem. Lengths cycle
+    through 1..9 with a longer problem every 97th; a sample spread over the
+    whole index range is checked against the oracle, every result for
+    finiteness."""
+    n = 5
+    F = 0.97 * P.random_orthogonal(P.rng_for(1201), n)
+    probs = []
+    for i in range(4200):
+        k = 40 if i % 97 == 0 else 1 + i % 9
+        probs.append(P.iid_problem(k, n, n, seed=1300 + i, F=F, G_scale=1 / np.sqrt(n), Cvar=0.1,
+                                   name=f"p{i}_k{k}"))
+    say(f"one dimension, {len(probs)} problems, {sum(p.k for p in probs)} steps")
+    b = dev.Batch(probs)
+    try:
+        b.smooth("oddeven")
+        for i in range(len(probs)):
+            x, _ = b.download(i, want_cov=False)
+            assert np.all(np.isfinite(x)), probs[i].name
+        sample = [probs[i] for i in range(0, len(probs), 71)] + [probs[-1]]
+        idx = list(range(0, len(probs), 71)) + [len(probs) - 1]
+        sub_b = _Sub(b, idx)
+        worst = _check_all(sub_b, sample, oracle, "oddeven")
+        say(f"  {len(sample)} sampled problems vs oracle: worst {worst:.2e}")
+    finally:
+        b.free()
+
+
+class _Sub:
+    """A view of some problems of a batch, for _check_all."""
+
+    def __init__(self, b, idx):
+        self.b, self.idx = b, idx
+
+    def download(self, i, want_cov=True):
+        return self.b.download(self.idx[i], want_cov)
