@@ -168,8 +168,15 @@ def headline_traffic(kernel, specs, sample=600):
     if k_sample == 0:
         return None
     f = k_all / k_sample
-    return {"bytes_per_step": t["bytes_per_step"] * f, "fetch_bytes": t["fetch_bytes"] * f,
-            "write_bytes": t["write_bytes"] * f, "algorithmic_bytes_per_step": None,
+    alg = None
+    if base.startswith("k_phase2<") and ", float>" in base:
+        # the FP32 levels (0-7) of the mixed path; level 0 with D diagonal and B formed on load (K, C given as 'w')
+        alg = sum(flops.oddeven_phase2_bytes(sp["n"], sp["k"], fp32_levels=8, level0_diag_synth=True)
+                  for sp in specs if bucket_qr(sp["n"]) == nb)
+    meas = t["bytes_per_step"] * f
+    return {"bytes_per_step": meas, "fetch_bytes": t["fetch_bytes"] * f,
+            "write_bytes": t["write_bytes"] * f, "algorithmic_bytes_per_step": alg,
+            "measured_over_algorithmic": (meas / alg) if alg else None,
             "source": f"profiles/traffic.json, batch_mixed_chunk{sample} ({base}), x{f:.3f}: bucket-{nb} time steps "
                       f"of the workload over the sample's"}
 

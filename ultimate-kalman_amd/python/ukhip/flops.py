@@ -206,6 +206,32 @@ def oddeven_backsub_bytes(n, k, cov=True):
     return 8.0 * total
 
 
+def oddeven_phase2_bytes(n, k, fp32_levels=0, level0_diag_synth=False):
+    """Algorithmic HBM bytes of the odd-even phase 2 (QR of [D_j; R~_j], the
+    reference's c/kalman_oddeven_smoother.c:460-523) over all levels of one
+    problem of k steps: per even step j > 0 of a level it reads D_j, R~_j,
+    X_j, B_j (n x n each), c_j and o_j, and writes R, B~, Y, Z_j, X~_j (n x n
+    each; Z_j and o_j of a lone last step go to a scratch of the same size)
+    and cf_j, c~_j. Levels below fp32_levels hold 4-byte elements, the others
+    8. level0_diag_synth: at level 0, D_j is stored as its diagonal and B_j is
+    formed from the FP64 K_j scaling (n values; F, one block per problem, is
+    not counted)."""
+    nn = n * n
+    total = 0.0
+    L, lev = k, 0
+    while L >= 2:
+        e = 4.0 if lev < fp32_levels else 8.0
+        for j in range(2, L, 2):
+            if lev == 0 and level0_diag_synth:
+                r = e * (n + 2 * nn + 2 * n) + 8.0 * n
+            else:
+                r = e * (4 * nn + 2 * n)
+            total += r + e * (5 * nn + 2 * n)
+        L //= 2
+        lev += 1
+    return total
+
+
 def _mldivide(N, P):
     """matrix_create_mldivide(A (N x N), B (N x P)) of the reference
     (c/matrix_ops.c:610-630): dgeqrf, dormqr, dtrtrs."""
