@@ -221,12 +221,11 @@ def oddeven_phase2_bytes(n, k, fp32_levels=0, level0_diag_synth=False):
     L, lev = k, 0
     while L >= 2:
         e = 4.0 if lev < fp32_levels else 8.0
-        for j in range(2, L, 2):
-            if lev == 0 and level0_diag_synth:
-                r = e * (n + 2 * nn + 2 * n) + 8.0 * n
-            else:
-                r = e * (4 * nn + 2 * n)
-            total += r + e * (5 * nn + 2 * n)
+        if lev == 0 and level0_diag_synth:
+            r = e * (n + 2 * nn + 2 * n) + 8.0 * n
+        else:
+            r = e * (4 * nn + 2 * n)
+        total += (L - 1) // 2 * (r + e * (5 * nn + 2 * n))  # even j = 2, 4, ... < L
         L //= 2
         lev += 1
     return total
