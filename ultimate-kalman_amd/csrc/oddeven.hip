@@ -1344,7 +1344,7 @@ static QrLaunch g_qr[2][3][4] = {
      /* phase 3 */ {{1, 4}, {1, 4}, {2, 3}, {1, 2}}},
     /* FP32 */
     {/* phase 1 */ {{2, 3}, {2, 4}, {2, 4}, {1, 4}},
-     /* phase 2 */ {{2, 2}, {3, 2}, {3, 3}, {2, 4}},
+     /* phase 2 */ {{2, 3}, {3, 2}, {3, 3}, {2, 4}},
      /* phase 3 */ {{1, 2}, {1, 1}, {2, 3}, {1, 1}}}};
 static int qr_bucket_index(int nb) {
   return nb == 32 ? 0 : (nb == 48 ? 1 : (nb == 64 ? 2 : (nb == 24 ? 3 : -1)));
