@@ -220,10 +220,10 @@ struct Col {
  * LDS, and the workgroup stores the block with RW consecutive lanes per
  * column (consecutive addresses, whole cache lines) instead of each group's
  * 16-byte pieces of 16 columns per wave store. Stores need no wait, so this
- * adds no latency to the chain; measured 1.6% faster on the mixed-precision
- * smoothing, and 2% slower in FP64, whose kernels store directly. (Loads
- * through the stage were slower: block after block behind barriers, they
- * wait one HBM round trip each.)
+ * adds no latency to the chain; measured 2.8% faster on the mixed-precision
+ * smoothing of a 600-problem chunk of the batching workload, and slower in
+ * FP64, whose kernels store directly. (Loads through the stage were slower:
+ * block after block behind barriers, they wait one HBM round trip each.)
  * Buffer: column c at c * LDP. LDP is RW plus 4 (plus 2 with two columns per
  * group): at one local, the groups of a wave (columns g * CPL + q, rows
  * SL * l + sub) then start on different banks.
